@@ -1,0 +1,162 @@
+/* libgmod.so -- MI355X-native drop-in for the chi0/RPA, FLEX self-energy and
+ * linearized-Eliashberg entry points of pyrpa's libfmod.so.
+ *
+ * Calling convention (identical to the reference's bind(C) Fortran
+ * subroutines): every argument by pointer, void return, arrays column-major
+ * with the reference's Fortran shapes, caller-allocated outputs.
+ *   int64_t*  : integer(c_int64_t)      double* : real(c_double)
+ *   bool*     : logical(1) / ctypes c_bool (1 byte)
+ *   gmod_z*   : complex(c_double), interleaved (re, im)
+ *   int32_t*  : integer(c_int32_t) (irr_chi, chi_map only)
+ * Orbital indices inside olist are 1-based like the reference.
+ *
+ * Citations are to the reference source (ktszk/pyrpa) that each entry point
+ * replaces; the ctypes wrapper that binds it in the reference is given in [].
+ */
+#ifndef GMOD_H
+#define GMOD_H
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } gmod_z;
+
+/* ---------------- FLEX self-energy loop ---------------------------------- */
+/* libs/src/fself.f90:287-448 (mkself) [libs/flibs/_selfenergy.py:76-127, argtypes :14-37]
+ * sigmak(Nk,Nw,Norb,Norb) out; mu out.  sw_in reads sigma.bin from the CWD
+ * (fself.f90:742-780) unless an in-memory seed was given with
+ * gmod_set_sigma_seed; sw_out writes it. */
+void mkself(gmod_z* sigmak, double* mu, const double* Smat, const double* Cmat, const int64_t* kmap,
+            const int64_t* invk, const int64_t* olist, const gmod_z* hamk, const double* eig, const gmod_z* uni,
+            const double* mu_init, const double* rfill, const double* temp, const int64_t* scf_loop,
+            const double* pp, const double* eps, const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw,
+            const int64_t* Norb, const int64_t* Nchi, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+            const int64_t* sub_sigma, const bool* sw_out, const bool* sw_in, const int64_t* m_diis,
+            const bool* sw_rescale, const bool* sw_tail, const double* sigma_scale);
+
+/* ---------------- chi0 ---------------------------------------------------- */
+/* libs/src/fchi.f90:401-419 (get_chi0 = get_chi_map + get_chi0_conv + ckchi_impl)
+ * [libs/flibs/_susceptibility.py:432-458] */
+void get_chi0(gmod_z* chi, const double* Smat, const double* Cmat, const gmod_z* Gk, const int64_t* kmap,
+              const int64_t* invk, const int64_t* olist, const double* temp, const int64_t* Nx,
+              const int64_t* Ny, const int64_t* Nz, const int64_t* Nw, const int64_t* Nk, const int64_t* Nkall,
+              const int64_t* Norb, const int64_t* Nchi, double* maxchi0s_out);
+/* libs/src/fchi.f90:894-918 (tail-corrected chi0 = conv[G]-conv[G0]+ref[G0], fchi.f90:852-892)
+ * [libs/flibs/_susceptibility.py:460-494] */
+void get_chi0_tail(gmod_z* chi, const double* Smat, const double* Cmat, const gmod_z* Gk, const double* eig,
+                   const gmod_z* uni, const int64_t* kmap, const int64_t* invk, const int64_t* olist,
+                   const double* mu, const double* temp, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+                   const int64_t* Nw, const int64_t* Nk, const int64_t* Nkall, const int64_t* Norb,
+                   const int64_t* Nchi, double* maxchi0s_out);
+/* libs/src/fchi.f90:615-635 (bind name get_chi0_conv_) [libs/flibs/_susceptibility.py:244-269] */
+void get_chi0_conv_(gmod_z* chi, const gmod_z* Gk, const int64_t* kmap, const int64_t* invk,
+                    const int32_t* irr_chi, const int32_t* chi_map, const int64_t* olist, const double* temp,
+                    const int64_t* Nx, const int64_t* Ny, const int64_t* Nz, const int64_t* Nw, const int64_t* Nk,
+                    const int64_t* Nkall, const int64_t* Norb, const int64_t* Nchi);
+
+/* ---------------- RPA vertices / susceptibilities ------------------------- */
+/* libs/src/fUJ.f90:318-365 (bind name get_vsigma_flex_nosoc_) [libs/flibs/_selfenergy.py:62-74] */
+void get_vsigma_flex_nosoc_(gmod_z* chi, const double* Smat, const double* Cmat, const int64_t* Nk,
+                            const int64_t* Nw, const int64_t* Nchi);
+/* libs/src/fchi.f90:319-350 [libs/flibs/_susceptibility.py:419-430] */
+void get_chis(gmod_z* chis, const gmod_z* chi0, const double* Smat, const int64_t* Nchi, const int64_t* Nw);
+/* libs/src/fself.f90:783-816 [libs/flibs/_susceptibility.py:568-583] */
+void get_chis_chic(gmod_z* chis, gmod_z* chic, const gmod_z* chi, const double* Smat, const double* Cmat,
+                   const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi);
+/* libs/src/fself.f90:818-850 [libs/flibs/_susceptibility.py:615-629] */
+void get_eig_or_tr_chi(gmod_z* chiqout, const gmod_z* chi, const int64_t* invk, const int64_t* Nkall,
+                       const int64_t* Nk, const int64_t* Nchi, const bool* sw_eig);
+
+/* ---------------- Green functions ---------------------------------------- */
+/* libs/src/fself.f90:1-41 (bind name gen_green0_) [libs/flibs/_green.py:80-97] */
+void gen_green0_(gmod_z* Gk, const double* eig, const gmod_z* uni, const double* mu, const double* temp,
+                 const int64_t* Nk, const int64_t* Nw, const int64_t* Norb);
+/* libs/src/fself.f90:43-83 (bind name gen_green_inv_) [libs/flibs/_green.py:99-117] */
+void gen_green_inv_(gmod_z* Gk, const gmod_z* self, const gmod_z* hamk, const double* mu, const double* temp,
+                    const int64_t* Nk, const int64_t* Nw, const int64_t* Norb);
+/* libs/src/fself.f90:131-157 (bind name getinv_) [libs/flibs/_green.py:116] */
+void getinv_(gmod_z* Gk, const int64_t* Nk, const int64_t* Nw, const int64_t* Norb);
+
+/* ---------------- linearized Eliashberg ---------------------------------- */
+/* libs/src/feliash.f90:1-443 [libs/flibs/_eliashberg.py:145-181].  chi is
+ * overwritten with the pairing vertex V_Delta, as in the reference. */
+void lin_eliash(gmod_z* delta, gmod_z* chi, const gmod_z* Gk, const gmod_z* uni, const double* init_delta,
+                const double* Smat, const double* Cmat, const int64_t* olist, const double* prt,
+                const int64_t* kmap, const int64_t* invk, const double* temp, const double* eps,
+                const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi,
+                const int64_t* Norb, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+                const int64_t* itemax, const int64_t* gap_sym, const int64_t* arnoldi_m, double* lambda_out);
+/* libs/src/feliash.f90:445-484 (bind name mkfk_trs_nsoc_) [libs/flibs/_eliashberg.py:343-355] */
+void mkfk_trs_nsoc_(gmod_z* fk, const gmod_z* Gk, const gmod_z* delta, const int64_t* Nk, const int64_t* Nw,
+                    const int64_t* Norb);
+/* libs/src/feliash.f90:637-731 (bind name get_initial_delta_) [libs/flibs/_eliashberg.py:378-399] */
+void get_initial_delta_(gmod_z* delta, const double* init_delta, const gmod_z* uni, const int64_t* kmap,
+                        const int64_t* invk, const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw,
+                        const int64_t* Norb, const int64_t* gap_sym);
+/* libs/src/feliash.f90:733-785 [libs/flibs/_eliashberg.py:306-323] */
+void conv_delta_orb_to_band(gmod_z* deltab, const gmod_z* delta, const gmod_z* uni, const double* prt,
+                            const int64_t* invk, const int64_t* Norb, const int64_t* Nkall, const int64_t* Nk,
+                            const int64_t* Nw, const int64_t* gap_sym);
+/* libs/src/feliash.f90:787-821 (bind name remap_delta_) [libs/flibs/_eliashberg.py:426-441] */
+void remap_delta_(gmod_z* delta, const gmod_z* delta0, const double* prt, const int64_t* invk,
+                  const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Norb,
+                  const int64_t* gap_sym);
+
+/* ---------------- interaction vertices and setup ------------------------- */
+/* libs/src/fUJ.f90:1-57 [libs/flibs/_interaction.py:47-62] */
+void get_scmat(double* Smat, double* Cmat, const int64_t* ol, const int64_t* site, const double* Uval,
+               const double* Jval, const int64_t* Nchi);
+/* libs/src/fUJ.f90:59-107 [libs/flibs/_interaction.py:64-81] */
+void get_scmat_orb(double* Smat, double* Cmat, const int64_t* ol, const int64_t* site, const double* Umat,
+                   const double* Jmat, const int64_t* Nchi, const int64_t* Norb);
+/* libs/src/finv.f90:1-290 [libs/flibs/_hamiltonian.py:356-382] */
+void generate_irr_kpoint_inv(double* klist, int64_t* kmap, int64_t* invk, const int64_t* Nk, const int64_t* Nx,
+                             const int64_t* Ny, const int64_t* Nz);
+/* libs/src/finv.f90:292-305 [libs/flibs/_hamiltonian.py:384-395] */
+void get_kweight(double* weight, const int64_t* invk, const int64_t* Nk, const int64_t* Nkall);
+/* libs/src/fmod.f90:162-207 [libs/flibs/_hamiltonian.py:134-158] */
+void gen_ham(gmod_z* ham_k, const double* klist, const gmod_z* ham_r, const double* rvec, const int64_t* Nk,
+             const int64_t* Nr, const int64_t* Norb);
+/* libs/src/fmod.f90:209-248 [libs/flibs/_hamiltonian.py:160-184] */
+void get_eig(double* eig, gmod_z* uni, const gmod_z* ham_k, const int64_t* Nk, const int64_t* Norb);
+/* libs/src/fmod.f90:312-340 [libs/flibs/_hamiltonian.py:186-200] */
+void get_ffermi(double* ffermi, const double* eig, const double* mu, const double* temp, const int64_t* Nk,
+                const int64_t* Norb);
+/* libs/src/fmod.f90:7-20 [libs/flibs/_hamiltonian.py:122-132]: reports the HIP device count */
+void openmp_params(int64_t* omp_num, bool* omp_check);
+
+/* ---------------- additions (no reference counterpart) ------------------- */
+/* Sticky error channel replacing the reference's print+STOP: returns the
+ * error count since the last call and copies the last message. */
+int gmod_last_error(char* buf, int len);
+/* 0: FP64 everywhere (default).  1: FP32 FFT convolutions with FP64 LU,
+ * Dyson, mu search, tail and eigenvalue; the FLEX loop then stores the
+ * convolutions as FP64 reference + FP32 increments so that it still reaches
+ * esterr < 1e-8.  Also read from the environment variable GMOD_PRECISION. */
+void gmod_set_precision(int mode);
+int gmod_get_precision(void);
+/* In-memory warm start for mkself(sw_in=true): Sigma(Nk,Nw,Norb,Norb), mu, mu_OLD.
+ * Consumed by the next mkself call instead of sigma.bin. */
+void gmod_set_sigma_seed(const gmod_z* sigma, double mu, double mu_old, int64_t count);
+/* Statistics of the last mkself call: [iterations, esterr, stoner, mu,
+ * seconds, seconds in chi0, seconds in vertex, seconds in sigma, seconds in
+ * mu search + Dyson]; returns the number of values written. */
+int gmod_flex_stats(double* out, int n);
+/* Kernel profiling with hipEvents on the library stream.  gmod_profile(1)
+ * resets and enables; gmod_profile_read returns, per kernel class
+ * [fft_pass, rpa_vertex, dyson, scatter, product, gather]: ms, launches,
+ * algorithmic bytes; 3 values per class. */
+void gmod_profile(int on);
+int gmod_profile_read(double* out, int n);
+void gmod_set_verbose(int level);
+/* Device memory held by the library's caching pool, in bytes. */
+double gmod_pool_bytes(void);
+void gmod_release_memory(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GMOD_H */

@@ -1,0 +1,52 @@
+// Convolution engine: chi0 (sharp cutoff / tail corrected), Sigma = V (*) G and
+// the Eliashberg gap convolution on the full (k-mesh x tau) grid.
+#pragma once
+#include <array>
+
+#include "fft.h"
+#include "kernels.h"
+#include "linalg.h"
+
+namespace gmod {
+
+struct OrbPairs {
+  int No = 0, Nchi = 0;
+  std::vector<std::array<int, 2>> ol;       // 0-based (o1, o2) per chi index (Fortran olist(:,1), olist(:,2))
+  std::vector<std::array<int, 2>> irr;      // irreducible (l, m) pairs  (fchi.f90:470-480)
+  std::vector<std::array<int, 2>> partner;  // partner[(m + Nchi*l)] = (m2, l2)  (fchi.f90:456-457)
+  void from_olist(const int64_t* olist, int Nchi, int No);
+  void from_maps(const int32_t* irr_chi, const int32_t* chi_map);  // caller-provided tables
+};
+
+template <typename R>
+class ConvEngine {
+ public:
+  ConvEngine(const Mesh& m, const OrbPairs& op);
+  // Fourier transform every full-grid G_ab (a,b = 0..No-1) into Fg (or F0g)
+  void transform_green(const zd* G, bool g0, hipStream_t s);
+  // chi(Nk,Nw,Nchi,Nchi) <- bubble of the transformed Fg (and, if tail, minus
+  // the G0 bubble plus the analytic tau reference).  Thresholds at 1e-9.
+  void chi0(zd* chi, double temp, bool tail, const double* eig_d, const zd* uni_d, const double* eig_h,
+            const zd* uni_h, const int64_t* invk, double mu, hipStream_t s);
+  // sigma(Nk,Nw,No,No) <- (T/Nkall) sum_pairs V (*) G, using Fg; thresholds at 1e-9
+  void sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s);
+  // Eliashberg: cache FFT[V_Delta] of the qualifying pairs (feliash.f90:540)
+  void prepare_gap(const zd* Vd, const double* S_h, const double* C_h, bool singlet, hipStream_t s);
+  // newdelta (un-weighted, Hermitised) <- sum V_Delta (*) F  (feliash.f90:486-635)
+  void gap(zd* nd, const zd* F, const double* prt, bool singlet, hipStream_t s);
+
+  const Mesh& mesh;
+  const OrbPairs& op;
+  long N;
+  const FFTPlan<R>* p4;
+  const FFTPlan<R>* p3;
+  DBuf<cpx<R>> Fg, F0g, TA, TB, work, tmp, acc, Vft, Ff;
+  std::vector<std::array<int, 2>> gap_pairs;
+  bool vft_cached = false;
+  const zd* Vd_dev = nullptr;
+  std::vector<double> S_cache, C_cache;
+  cpx<R>* grid(DBuf<cpx<R>>& b, int a, int c) { return b.p + (long)(a + op.No * c) * N; }
+  void fft(cpx<R>* g, int sign, bool spatial_only, hipStream_t s);
+};
+
+}  // namespace gmod

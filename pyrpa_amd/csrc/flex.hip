@@ -1,0 +1,1017 @@
+// C-ABI entry points of the hot path and the device-resident drivers:
+// chi0 (sharp / tail), RPA vertices and susceptibilities, Green functions,
+// the FLEX self-energy loop (mkself) and the linearized Eliashberg solver.
+#include <chrono>
+#include <cmath>
+#include <fstream>
+
+#include "../../include/gmod.h"
+#include "engine.h"
+
+namespace gmod {
+
+using clk = std::chrono::steady_clock;
+static double secs(clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); }
+
+// ---------------------------------------------------------------- shared state
+struct Seed {
+  bool valid = false;
+  std::vector<zd> sigma;
+  double mu = 0, mu_old = 0;
+};
+static Seed g_seed;
+static double g_stats[16];
+
+static void check_err(DBuf<int>& err, hipStream_t s, const char* where) {
+  int h = 0;
+  err.download(&h, 1, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  if (h) {
+    std::string w = where;
+    if (h & 1) throw Error(w + ": zgesv failed (singular RPA denominator)");
+    if (h & 2) throw Error(w + ": eigenvalue iteration did not converge");
+    if (h & 4) throw Error(w + ": singular matrix in the Dyson inverse (zgetrf)");
+    if (h & 8) throw Error(w + ": singular matrix in n(mu)");
+    throw Error(w + ": device error");
+  }
+}
+
+struct Upload {
+  hipStream_t s;
+  template <typename T>
+  void operator()(DBuf<T>& d, const T* h, size_t n) {
+    d.alloc(n);
+    d.upload(h, n, s);
+  }
+};
+
+// Stoner factor: max_q max Re eig(chi0(q,0) S) (fchi.f90:352-399)
+static double stoner_factor(const zd* chi, long Nk, long Nw, int Nchi, const double* S_d, hipStream_t s) {
+  DBuf<double> out(Nk);
+  DBuf<int> err(1);
+  err.zero(s);
+  max_re_eig(out.p, chi, BatchLayout{1, Nk * Nw}, Nk, Nchi, S_d, 1.0, err.p, s);
+  check_err(err, s, "ckchi");
+  std::vector<double> h(Nk);
+  out.download(h.data(), Nk, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  double best = -1.0e5;
+  for (long i = 0; i < Nk; ++i)
+    if (h[i] > best) best = h[i];
+  return best;
+}
+
+template <typename R>
+static void chi0_driver(zd* chi_h, const zd* Gk, const int64_t* kmap, const int64_t* invk, OrbPairs& op,
+                        double temp, long Nx, long Ny, long Nz, long Nw, long Nk, long Nkall, int No, bool tail,
+                        const double* eig, const zd* uni, double mu, const double* S, double* stoner) {
+  hipStream_t s = ctx().stream;
+  Mesh mesh;
+  mesh.build(kmap, invk, Nkall, Nk, (int)Nx, (int)Ny, (int)Nz, (int)Nw, s);
+  const long NkNw = Nk * Nw;
+  const int Nchi = op.Nchi;
+  Upload up{s};
+  DBuf<zd> G, chi(NkNw * Nchi * Nchi), uni_d;
+  DBuf<double> eig_d, S_d;
+  up(G, Gk, NkNw * No * No);
+  if (tail) { up(eig_d, eig, (size_t)No * Nk); up(uni_d, uni, (size_t)No * No * Nk); }
+  ConvEngine<R> eng(mesh, op);
+  eng.transform_green(G.p, false, s);
+  eng.chi0(chi.p, temp, tail, eig_d.p, uni_d.p, eig, uni, invk, mu, s);
+  if (stoner) {
+    up(S_d, S, (size_t)Nchi * Nchi);
+    *stoner = stoner_factor(chi.p, Nk, Nw, Nchi, S_d.p, s);
+  }
+  chi.download(chi_h, NkNw * Nchi * Nchi, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- sigma.bin (fself.f90:742-780)
+static void write_record(std::ofstream& f, const void* data, size_t bytes) {
+  // gfortran-style records; > 2^31-1 bytes split into sub-records whose
+  // length marker is negative except for the last one
+  const char* p = static_cast<const char*>(data);
+  const size_t maxrec = 2147483639;
+  do {
+    size_t chunk = std::min(bytes, maxrec);
+    bool more = bytes > chunk;
+    int32_t mk = more ? -(int32_t)chunk : (int32_t)chunk;
+    f.write(reinterpret_cast<const char*>(&mk), 4);
+    f.write(p, chunk);
+    f.write(reinterpret_cast<const char*>(&mk), 4);
+    p += chunk;
+    bytes -= chunk;
+  } while (bytes > 0);
+}
+
+static bool read_record(std::ifstream& f, std::vector<char>& out) {
+  out.clear();
+  while (true) {
+    int32_t mk = 0, mk2 = 0;
+    if (!f.read(reinterpret_cast<char*>(&mk), 4)) return !out.empty();
+    size_t n = (size_t)(mk < 0 ? -(long)mk : mk);
+    size_t old = out.size();
+    out.resize(old + n);
+    if (!f.read(out.data() + old, n)) return false;
+    f.read(reinterpret_cast<char*>(&mk2), 4);
+    if (mk >= 0) return true;
+  }
+}
+
+// ---------------------------------------------------------------- FLEX loop
+struct FlexIn {
+  const double *Smat, *Cmat, *eig;
+  const int64_t *kmap, *invk, *olist;
+  const zd *hamk, *uni;
+  double mu_init, rfill, temp, pp, eps, sigma_scale;
+  long scf_loop, Nkall, Nk, Nw, Nx, Ny, Nz, sub_sigma, m_diis;
+  int No, Nchi;
+  bool sw_out, sw_in, sw_rescale, sw_tail;
+};
+
+template <typename R>
+static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out) {
+  auto t_start = clk::now();
+  hipStream_t s = ctx().stream;
+  const int verbose = ctx().verbose;
+  const int No = A.No, Nchi = A.Nchi;
+  const long Nk = A.Nk, Nw = A.Nw, NkNw = Nk * Nw;
+  const size_t nS = (size_t)NkNw * No * No, nC = (size_t)NkNw * Nchi * Nchi;
+  const double T = A.temp;
+  if (A.m_diis < 1) throw Error("m_diis must be >= 1");
+  Mesh mesh;
+  mesh.build(A.kmap, A.invk, A.Nkall, Nk, (int)A.Nx, (int)A.Ny, (int)A.Nz, (int)Nw, s);
+  OrbPairs op;
+  op.from_olist(A.olist, Nchi, No);
+  Upload up{s};
+  DBuf<double> S_d, C_d, eig_d;
+  DBuf<zd> ham_d, uni_d;
+  up(S_d, A.Smat, (size_t)Nchi * Nchi);
+  up(C_d, A.Cmat, (size_t)Nchi * Nchi);
+  up(eig_d, A.eig, (size_t)No * Nk);
+  up(ham_d, A.hamk, (size_t)No * No * Nk);
+  up(uni_d, A.uni, (size_t)No * No * Nk);
+  DBuf<zd> G(nS), sig(nS), sig0(nS), sd(nS), chi(nC);
+  DBuf<int> err(1);
+  err.zero(s);
+  std::vector<DBuf<zd>> xout(A.m_diis), res(A.m_diis);
+  for (long i = 0; i < A.m_diis; ++i) { xout[i].alloc(nS); res[i].alloc(nS); }
+  ConvEngine<R> eng(mesh, op);
+
+  double mu = A.mu_init, mu_old = 0.0;
+  if (A.sw_in) {
+    std::vector<zd> h;
+    if (g_seed.valid) {
+      if (g_seed.sigma.size() != nS) throw Error("in-memory sigma seed has the wrong size");
+      h = g_seed.sigma;
+      mu = g_seed.mu;
+      mu_old = g_seed.mu_old;
+      g_seed.valid = false;
+    } else {
+      std::ifstream f("sigma.bin", std::ios::binary);
+      if (!f) throw Error("sw_in: cannot open sigma.bin");
+      std::vector<char> r;
+      if (!read_record(f, r) || r.size() != 8) throw Error("sigma.bin: bad mu record");
+      memcpy(&mu, r.data(), 8);
+      if (!read_record(f, r) || r.size() != 8) throw Error("sigma.bin: bad mu_OLD record");
+      memcpy(&mu_old, r.data(), 8);
+      if (!read_record(f, r) || r.size() < nS * sizeof(zd)) throw Error("sigma.bin: sigma record too short");
+      h.resize(nS);
+      memcpy(h.data(), r.data(), nS * sizeof(zd));
+      if (read_record(f, r) && r.size() == 32) {  // metadata footer (fself.f90:761-780)
+        double meta[4];
+        memcpy(meta, r.data(), 32);
+        if (std::lround(meta[1]) != Nw || std::lround(meta[2]) != No || std::lround(meta[3]) != Nk)
+          throw Error("io_sigma: sigma.bin was written on a different mesh");
+      }
+    }
+    sig.upload(h.data(), nS, s);
+    if (A.sigma_scale != 1.0) scale(sig.p, nS, A.sigma_scale, s);
+    copy(sig0.p, sig.p, nS, s);
+    dyson(G.p, sig.p, ham_d.p, mu, T, Nk, (int)Nw, No, err.p, s);
+  } else {
+    mu_old = mu * 1.2;
+    GMOD_HIP(hipMemsetAsync(sig0.p, 0, nS * sizeof(zd), s));
+    green0(G.p, eig_d.p, uni_d.p, mu, T, Nk, (int)Nw, No, s);
+  }
+
+  // ---- mixing state (compare_sigma, fself.f90:449-542)
+  int n_hist = 0, i_hist = 0;
+  const int m = (int)A.m_diis;
+  double esterr = 0.0;
+  // ---- chemical potential state (renew_mu, fself.f90:544-685)
+  bool bracket_valid = false;
+  double muS_cache = 0.0, muL_cache = 0.0;
+  auto get_rn = [&](double rmu) {
+    double parts[2];
+    filling_parts(parts, sig.p, ham_d.p, eig_d.p, rmu, T, Nk, (int)Nw, No, err.p, s);
+    return (parts[0] + 2.0 * T * parts[1]) / (double)Nk;
+  };
+  const double rfill = A.rfill;
+  auto renew_mu = [&]() {
+    double eps = esterr > 1.0e-2 ? 1.0e-8 : esterr * 0.1;
+    double dmu = std::fabs(mu - mu_old) * 2.0;
+    if (dmu < eps * 4.0) dmu = eps * 4.0;
+    mu_old = mu;
+    double rnS = 0, rnL = 0, muS = 0, muL = 0;
+    bool found = false;
+    if (bracket_valid) {
+      rnS = get_rn(muS_cache);
+      rnL = get_rn(muL_cache);
+      if (rnS < rfill && rnL > rfill) { muS = muS_cache; muL = muL_cache; found = true; }
+    }
+    if (!found) {
+      muL = mu + dmu;
+      muS = mu - dmu;
+      for (int it = 1; it <= 100; ++it) {
+        mu = muL;
+        rnL = get_rn(mu);
+        if (rnL < rfill) {
+          if (std::fabs(rfill - rnL) > 0.5) muL += 1.0;
+          else if (std::fabs(rfill - rnL) > dmu) muL += 0.5;
+          else muL += dmu;
+        } else break;
+        if (it == 100) throw Error("renew_mu: Too many (upper bracket)");
+      }
+      for (int it = 1; it <= 100; ++it) {
+        mu = muS;
+        rnS = get_rn(mu);
+        if (rnS > rfill) {
+          if (std::fabs(rnS - rfill) > 0.5) muS -= 1.0;
+          else if (std::fabs(rnS - rfill) > dmu) muS -= 0.5;
+          else muS -= dmu;
+        } else break;
+        if (it == 100) throw Error("renew_mu: Too many (lower bracket)");
+      }
+    }
+    muS_cache = muS;
+    muL_cache = muL;
+    bracket_valid = true;
+    rnL -= rfill;
+    rnS -= rfill;
+    double rnc = rnS, muc = muS, mud = 0.0, muM = 0.0, rnM = 0.0;
+    bool flag = false;
+    for (int it = 1; it <= 100; ++it) {
+      if (rnc != rnS && rnc != rnL)
+        muM = (muL * rnS * rnc * (rnS - rnc) + muS * rnL * rnc * (rnc - rnL) + muc * rnL * rnS * (rnL - rnS)) /
+              ((rnL - rnS) * (rnc - rnL) * (rnc - rnS));
+      else
+        muM = muL - rnL * (muL - muS) / (rnL - rnS);
+      if ((0.25 * (3.0 * muS + muL) > muM || muM > muL) ||
+          (flag && std::fabs(muM - muL) >= std::fabs(muL - muc) * 0.5) ||
+          (!flag && std::fabs(muM - muL) >= std::fabs(muc - mud) * 0.5) ||
+          (flag && std::fabs(muL - muc) < 1.0e-8) || (!flag && std::fabs(muc - mud) < 1.0e-8)) {
+        muM = (muL + muS) * 0.5;
+        flag = true;
+      } else {
+        flag = false;
+      }
+      if (std::fabs(muL - muM) < eps) break;
+      mu = muM;
+      rnM = get_rn(muM);
+      mud = muc;
+      muc = muL;
+      rnc = rnL;
+      if (rnS * (rnM - rfill) < 0) { muL = muM; rnL = rnM - rfill; }
+      else { muS = muM; rnS = rnM - rfill; }
+      if (std::fabs(rnS) < std::fabs(rnL)) {
+        std::swap(muL, muS);
+        std::swap(rnL, rnS);
+      }
+      if (it == 100) throw Error("renew_mu: Too many loop!");
+    }
+    if (rnL == rnS) mu = (muS + muL) * 0.5;
+    else mu = (muS * rnL - muL * rnS) / (rnL - rnS);
+    double fill = get_rn(mu);
+    mu_old = mu;
+    if (verbose) printf("mu  =%8.4f filling =%8.4f\n", mu, fill);
+  };
+
+  double stoner = 0.0;
+  double t_chi = 0, t_vert = 0, t_sig = 0, t_mu = 0;
+  int iters = 0;
+  for (long it = 1; it <= A.scf_loop; ++it) {
+    iters = (int)it;
+    if (verbose) printf("iter=%5ld\n", it);
+    auto t0 = clk::now();
+    eng.transform_green(G.p, false, s);
+    eng.chi0(chi.p, T, A.sw_tail, eig_d.p, uni_d.p, A.eig, A.uni, A.invk, mu, s);
+    stoner = stoner_factor(chi.p, Nk, Nw, Nchi, S_d.p, s);
+    t_chi += secs(t0);
+    if (verbose) printf("SDW max eig(chi0 S) = %12.8f\n", stoner);
+    if (A.sw_rescale && stoner >= 1.0) {
+      const double rescale_t = 1.0 - 1.0e-4;  // fself.f90:371-380
+      if (verbose) printf("[FLEX] Stoner factor=%10.6f >= 1: rescaling chi0 to%7.4f\n", stoner, rescale_t);
+      scale(chi.p, nC, rescale_t / stoner, s);
+    }
+    t0 = clk::now();
+    {
+      ProfScope ps(PROF_VERTEX, (double)nC * 2 * sizeof(zd), s);
+      rpa_vertex(chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 0, err.p, s);
+    }
+    check_err(err, s, "get_vsigma_flex_nosoc");
+    t_vert += secs(t0);
+    if (verbose)
+      printf("Re V_sigma: max:%12.4E min:%12.4E\n", max_real(chi.p, nC, s), min_real(chi.p, nC, s));
+    t0 = clk::now();
+    eng.sigma(sig.p, chi.p, A.Smat, A.Cmat, T, s);
+    if (A.sub_sigma > 0) subtract_static(sig.p, Nk, (int)Nw, No, (int)A.sub_sigma, s);
+    // ---- compare_sigma
+    i_hist = i_hist % m + 1;
+    n_hist = std::min(n_hist + 1, m);
+    copy(xout[i_hist - 1].p, sig.p, nS, s);
+    lincomb(res[i_hist - 1].p, sig.p, 1.0, sig0.p, -1.0, nS, s);
+    const int n = n_hist;
+    std::vector<int> idx(n);
+    for (int ih = 1; ih <= n; ++ih) idx[ih - 1] = ((i_hist - n + ih - 1) % m + m) % m;
+    std::vector<double> B((n + 1) * (n + 1), 0.0), c(n + 1, 0.0);
+    for (int a = 0; a < n; ++a) {
+      for (int b = a; b < n; ++b) {
+        double v = dot_w(res[idx[a]].p, res[idx[b]].p, nullptr, (long)nS, 1, s).re;
+        B[a + (n + 1) * b] = B[b + (n + 1) * a] = v;
+      }
+      B[a + (n + 1) * n] = B[n + (n + 1) * a] = 1.0;
+    }
+    for (int a = 0; a < n; ++a) B[a + (n + 1) * a] += 1.0e-12;
+    c[n] = 1.0;
+    if (!host_dgesv(B, n + 1, c)) {
+      if (verbose) printf("DIIS: dgesv failed, fallback to most-recent entry\n");
+      std::fill(c.begin(), c.end(), 0.0);
+      c[n - 1] = 1.0;
+    }
+    copy(sd.p, xout[idx[0]].p, nS, s);
+    scale(sd.p, nS, c[0], s);
+    for (int a = 1; a < n; ++a) axpy(sd.p, xout[idx[a]].p, zd(c[a], 0), nS, s);
+    esterr = max_rel_err(sig0.p, sd.p, nS, 1.0e-10, s, nullptr);
+    if (n >= 2) copy(sig.p, sd.p, nS, s);
+    else lincomb(sig.p, sd.p, A.pp, sig0.p, 1.0 - A.pp, nS, s);
+    t_sig += secs(t0);
+    if (verbose) printf("esterr=%9.2E\n", esterr);
+    if (esterr < A.eps) break;
+    t0 = clk::now();
+    renew_mu();
+    {
+      ProfScope ps(PROF_DYSON, (double)nS * 2 * sizeof(zd), s);
+      dyson(G.p, sig.p, ham_d.p, mu, T, Nk, (int)Nw, No, err.p, s);
+    }
+    check_err(err, s, "getinv");
+    copy(sig0.p, sig.p, nS, s);
+    t_mu += secs(t0);
+  }
+  {
+    auto t0 = clk::now();
+    renew_mu();
+    t_mu += secs(t0);
+  }
+  check_err(err, s, "mkself");
+  if (stoner >= 1.0 && verbose) printf("[FLEX] WARNING: Stoner factor >= 1 at the final iteration\n");
+  std::vector<zd> h(nS);
+  sig.download(h.data(), nS, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  memcpy(sig_out, h.data(), nS * sizeof(zd));
+  *mu_out = mu;
+  if (A.sw_out) {
+    std::ofstream f("sigma.bin", std::ios::binary);
+    write_record(f, &mu, 8);
+    write_record(f, &mu_old, 8);
+    write_record(f, h.data(), nS * sizeof(zd));
+    double meta[4] = {T, (double)Nw, (double)No, (double)Nk};
+    write_record(f, meta, 32);
+  }
+  double st[9] = {(double)iters, esterr, stoner, mu, secs(t_start), t_chi, t_vert, t_sig, t_mu};
+  std::copy(st, st + 9, g_stats);
+}
+
+// ---------------------------------------------------------------- Eliashberg
+__global__ void k_broadcast_w(zd* D, const zd* D0, long Nk, long Nw, int No) {
+  const long NkNw = Nk * Nw;
+  const long tot = NkNw * No * No;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    long k = e % Nk;
+    long pq = e / NkNw;
+    D[e] = D0[k + Nk * pq];
+  }
+}
+
+// orbital-basis initial gap (feliash.f90:637-731) into device D (Nk,Nw,No,No)
+static void initial_delta(zd* D, const double* init_delta, const zd* uni, const Mesh& mesh, int No,
+                          long gap_sym, hipStream_t s) {
+  const long Nk = mesh.Nk, Nw = mesh.Nw;
+  std::vector<zd> D0((size_t)Nk * No * No, zd(0, 0));
+  if (gap_sym == 0) {
+    for (long k = 0; k < Nk; ++k)
+      for (int l = 0; l < No; ++l) D0[k + Nk * (l + No * l)] = zd(1.0, 0);
+  } else {
+    for (long k = 0; k < Nk; ++k) {
+      for (int l = 0; l < No; ++l)
+        for (int mm = 0; mm < No; ++mm) {
+          zd acc(0, 0);
+          for (int n = 0; n < No; ++n)
+            acc += uni[l + No * (n + No * k)] * init_delta[k + Nk * n] * conj(uni[mm + No * (n + No * k)]);
+          D0[k + Nk * (l + No * mm)] = acc;
+        }
+      for (int l = 0; l < No; ++l) {
+        D0[k + Nk * (l + No * l)].im = 0.0;
+        for (int mm = l + 1; mm < No; ++mm) D0[k + Nk * (l + No * mm)] = conj(D0[k + Nk * (mm + No * l)]);
+      }
+    }
+  }
+  DBuf<zd> d0(D0.size());
+  d0.upload(D0.data(), D0.size(), s);
+  hipLaunchKernelGGL(k_broadcast_w, dim3(4096), dim3(256), 0, s, D, d0.p, Nk, Nw, No);
+  GMOD_CHECK_LAUNCH();
+  const long rest = Nw * No * No;
+  double nrm = dot_w(D, D, mesh.kw.p, Nk, rest, s).re;
+  if (nrm <= 0.0) throw Error("get_initial_delta: norm <= 0");
+  scale(D, Nk * rest, 1.0 / std::sqrt(2.0 * nrm), s);
+}
+
+struct EliashIn {
+  zd* chi;
+  const zd *Gk, *uni;
+  const double *init_delta, *Smat, *Cmat, *prt;
+  const int64_t *olist, *kmap, *invk;
+  double temp, eps;
+  long Nkall, Nk, Nw, Nx, Ny, Nz, itemax, gap_sym, arnoldi_m;
+  int No, Nchi;
+};
+
+template <typename R>
+static double eliash_driver(const EliashIn& A, zd* delta_out) {
+  hipStream_t s = ctx().stream;
+  const int verbose = ctx().verbose;
+  const int No = A.No, Nchi = A.Nchi;
+  const long Nk = A.Nk, Nw = A.Nw, NkNw = Nk * Nw;
+  const size_t nS = (size_t)NkNw * No * No, nC = (size_t)NkNw * Nchi * Nchi;
+  const bool singlet = A.gap_sym >= 0;
+  const double weight = A.temp / (double)A.Nkall;
+  Mesh mesh;
+  mesh.build(A.kmap, A.invk, A.Nkall, Nk, (int)A.Nx, (int)A.Ny, (int)A.Nz, (int)Nw, s);
+  OrbPairs op;
+  op.from_olist(A.olist, Nchi, No);
+  Upload up{s};
+  DBuf<zd> chi, G, Fk(nS), nd(nS);
+  DBuf<double> S_d, C_d;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(chi, A.chi, nC);
+  up(G, A.Gk, nS);
+  up(S_d, A.Smat, (size_t)Nchi * Nchi);
+  up(C_d, A.Cmat, (size_t)Nchi * Nchi);
+  rpa_vertex(chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, singlet ? 1 : 2, err.p, s);
+  check_err(err, s, "get_V_delta_nsoc_flex");
+  chi.download(A.chi, nC, s);  // chi is inout: it leaves holding V_Delta (feliash.f90:42,61)
+  ConvEngine<R> eng(mesh, op);
+  eng.prepare_gap(chi.p, A.Smat, A.Cmat, singlet, s);
+  const long rest = Nw * No * No;
+  auto apply = [&](zd* w_out, const zd* v_in) {
+    anomalous_f(Fk.p, G.p, v_in, Nk, (int)Nw, No, s);
+    eng.gap(w_out, Fk.p, A.prt, singlet, s);
+    scale(w_out, nS, weight, s);
+  };
+  auto norm = [&](const zd* f) { return std::sqrt(2.0 * dot_w(f, f, mesh.kw.p, Nk, rest, s).re); };
+  auto inner = [&](const zd* u, const zd* v) { return dot_w(u, v, mesh.kw.p, Nk, rest, s) * 2.0; };
+  DBuf<zd> D(nS);
+  double lam_out = 0.0;
+  if (A.arnoldi_m > 0) {
+    const int md = (int)A.arnoldi_m;
+    std::vector<DBuf<zd>> V(md);
+    for (auto& v : V) v.alloc(nS);
+    DBuf<zd> d1(nS);
+    auto arnoldi = [&](double shift, bool defl, std::vector<zd>& Hm, int& m_act) {
+      Hm.assign((size_t)(md + 1) * md, zd(0, 0));
+      m_act = md;
+      for (int j = 0; j < md; ++j) {
+        apply(nd.p, V[j].p);
+        if (defl) {
+          axpy(nd.p, V[j].p, zd(shift, 0), nS, s);
+          zd h = inner(d1.p, nd.p);
+          axpy(nd.p, d1.p, zd(0, 0) - h, nS, s);
+        }
+        for (int ii = 0; ii <= j; ++ii) {
+          zd h = inner(V[ii].p, nd.p);
+          Hm[ii + (md + 1) * j] = h;
+          axpy(nd.p, V[ii].p, zd(0, 0) - h, nS, s);
+        }
+        double beta = norm(nd.p);
+        Hm[j + 1 + (md + 1) * j] = zd(beta, 0);
+        if (beta < 1.0e-14) { m_act = j + 1; break; }
+        if (j < md - 1) {
+          copy(V[j + 1].p, nd.p, nS, s);
+          scale(V[j + 1].p, nS, 1.0 / beta, s);
+        }
+      }
+    };
+    auto ritz = [&](const std::vector<zd>& Hm, int m_act, std::vector<zd>& ev) {
+      std::vector<zd> Am((size_t)m_act * m_act);
+      for (int q = 0; q < m_act; ++q)
+        for (int p = 0; p < m_act; ++p) Am[p + m_act * q] = Hm[p + (md + 1) * q];
+      if (host_eigvals(Am, m_act, ev)) throw Error("ZGEEV (Hessenberg QR) failed in Arnoldi");
+      return Am;
+    };
+    auto build = [&](zd* out, const std::vector<zd>& Am, int m_act, zd lam) {
+      std::vector<zd> y = host_eigvec(Am, m_act, lam);
+      GMOD_HIP(hipMemsetAsync(out, 0, nS * sizeof(zd), s));
+      for (int ii = 0; ii < m_act; ++ii) axpy(out, V[ii].p, y[ii], nS, s);
+      scale(out, nS, 1.0 / norm(out), s);
+    };
+    initial_delta(V[0].p, A.init_delta, A.uni, mesh, No, A.gap_sym, s);
+    std::vector<zd> Hm, ev;
+    int m_act;
+    arnoldi(0.0, false, Hm, m_act);
+    auto Am = ritz(Hm, m_act, ev);
+    int idx = 0;
+    for (int i = 1; i < m_act; ++i)
+      if (ev[i].re < ev[idx].re) idx = i;
+    const double lam1 = ev[idx].re;
+    double maxre = ev[0].re;
+    int imax = 0;
+    for (int i = 1; i < m_act; ++i)
+      if (ev[i].re > maxre) { maxre = ev[i].re; imax = i; }
+    if (verbose) {
+      printf("Pass 1 Ritz values (real):\n");
+      for (int i = 0; i < m_act; ++i) printf("%4d%14.6f\n", i + 1, ev[i].re);
+    }
+    if (maxre >= 0.1) {
+      lam_out = ev[imax].re;
+      build(D.p, Am, m_act, ev[imax]);
+    } else {
+      build(d1.p, Am, m_act, ev[idx]);
+      const double shift = -lam1;
+      initial_delta(V[0].p, A.init_delta, A.uni, mesh, No, A.gap_sym, s);
+      zd h = inner(d1.p, V[0].p);
+      axpy(V[0].p, d1.p, zd(0, 0) - h, nS, s);
+      scale(V[0].p, nS, 1.0 / norm(V[0].p), s);
+      arnoldi(shift, true, Hm, m_act);
+      Am = ritz(Hm, m_act, ev);
+      imax = 0;
+      for (int i = 1; i < m_act; ++i)
+        if (ev[i].re > ev[imax].re) imax = i;
+      lam_out = ev[imax].re - shift;
+      build(D.p, Am, m_act, ev[imax]);
+    }
+  } else {
+    // power method with shift + deflation (feliash.f90:67-140)
+    double lam1 = 0.0, lam_phys = 0.0, lam_rq = 0.0, inorm = 1.0;
+    DBuf<zd> d1(nS);
+    for (int i_eig = 1; i_eig <= 2; ++i_eig) {
+      initial_delta(D.p, A.init_delta, A.uni, mesh, No, A.gap_sym, s);
+      if (i_eig == 2) {
+        double proj = inner(d1.p, D.p).re;
+        axpy(D.p, d1.p, zd(-proj, 0), nS, s);
+        scale(D.p, nS, 1.0 / norm(D.p), s);
+      }
+      double lam_prev = 0.0;
+      for (long it = 1; it <= A.itemax; ++it) {
+        apply(nd.p, D.p);
+        if (i_eig == 2) {
+          axpy(nd.p, D.p, zd(-lam1, 0), nS, s);
+          double proj = inner(d1.p, nd.p).re;
+          axpy(nd.p, d1.p, zd(-proj, 0), nS, s);
+        }
+        double nrm = norm(nd.p);
+        if (nrm <= 0.0) throw Error("Error: norm <= 0 in iter_loop");
+        inorm = 1.0 / nrm;
+        lam_rq = inner(D.p, nd.p).re;
+        lam_phys = lam_rq + lam1;
+        // vec_err = sqrt(2 min(|nd/|nd| - D|^2, |nd/|nd| + D|^2))
+        double nn = 2.0 * dot_w(nd.p, nd.p, mesh.kw.p, Nk, rest, s).re;
+        double dd = 2.0 * dot_w(D.p, D.p, mesh.kw.p, Nk, rest, s).re;
+        double xr = 2.0 * dot_w(nd.p, D.p, mesh.kw.p, Nk, rest, s).re;
+        double tpos = 0.5 * (nn * inorm * inorm + dd - 2.0 * inorm * xr);
+        double tneg = 0.5 * (nn * inorm * inorm + dd + 2.0 * inorm * xr);
+        double verr = std::sqrt(2.0 * std::max(0.0, std::min(tpos, tneg)));
+        if (verbose) printf("%4ld lambda_elsh=%16.8E%16.8E  vec_err=%12.4E\n", it, lam_phys, std::fabs(lam_phys - lam_prev), verr);
+        if (lam_phys < 0.0 && verr < A.eps) break;
+        if (verr < A.eps && std::fabs(lam_phys - lam_prev) < A.eps * (std::fabs(lam_phys) + 1.0e-10)) break;
+        lam_prev = lam_phys;
+        copy(D.p, nd.p, nS, s);
+        scale(D.p, nS, inorm, s);
+      }
+      if (i_eig == 1) {
+        if (lam_phys > 0.0) { lam_out = lam_phys; break; }
+        lam1 = lam_rq;
+        copy(d1.p, nd.p, nS, s);
+        scale(d1.p, nS, inorm, s);
+      } else {
+        lam_out = lam_phys;
+      }
+    }
+    double nrm = norm(nd.p);
+    if (nrm <= 0.0) throw Error("Error: final norm <= 0");
+    copy(D.p, nd.p, nS, s);
+    scale(D.p, nS, 1.0 / nrm, s);
+  }
+  D.download(delta_out, nS, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  return lam_out;
+}
+
+}  // namespace gmod
+
+using namespace gmod;
+
+#define GMOD_TRY try {
+#define GMOD_CATCH                                       \
+  }                                                      \
+  catch (std::exception & e) { set_error(e.what()); }    \
+  catch (...) { set_error("unknown error"); }
+
+static std::lock_guard<std::mutex> api_lock() { return std::lock_guard<std::mutex>(ctx().lock); }
+
+// ====================================================================== chi0
+extern "C" void get_chi0(gmod_z* chi, const double* Smat, const double* Cmat, const gmod_z* Gk, const int64_t* kmap,
+                         const int64_t* invk, const int64_t* olist, const double* temp, const int64_t* Nx,
+                         const int64_t* Ny, const int64_t* Nz, const int64_t* Nw, const int64_t* Nk,
+                         const int64_t* Nkall, const int64_t* Norb, const int64_t* Nchi, double* maxchi0s_out) {
+  GMOD_TRY
+  auto g = api_lock();
+  (void)Cmat;
+  OrbPairs op;
+  op.from_olist(olist, (int)*Nchi, (int)*Norb);
+  if (ctx().precision == 1)
+    chi0_driver<float>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
+                       false, nullptr, nullptr, 0.0, Smat, maxchi0s_out);
+  else
+    chi0_driver<double>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
+                        false, nullptr, nullptr, 0.0, Smat, maxchi0s_out);
+  GMOD_CATCH
+}
+
+extern "C" void get_chi0_tail(gmod_z* chi, const double* Smat, const double* Cmat, const gmod_z* Gk,
+                              const double* eig, const gmod_z* uni, const int64_t* kmap, const int64_t* invk,
+                              const int64_t* olist, const double* mu, const double* temp, const int64_t* Nx,
+                              const int64_t* Ny, const int64_t* Nz, const int64_t* Nw, const int64_t* Nk,
+                              const int64_t* Nkall, const int64_t* Norb, const int64_t* Nchi, double* maxchi0s_out) {
+  GMOD_TRY
+  auto g = api_lock();
+  (void)Cmat;
+  OrbPairs op;
+  op.from_olist(olist, (int)*Nchi, (int)*Norb);
+  if (ctx().precision == 1)
+    chi0_driver<float>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
+                       true, eig, (const zd*)uni, *mu, Smat, maxchi0s_out);
+  else
+    chi0_driver<double>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
+                        true, eig, (const zd*)uni, *mu, Smat, maxchi0s_out);
+  GMOD_CATCH
+}
+
+extern "C" void get_chi0_conv_(gmod_z* chi, const gmod_z* Gk, const int64_t* kmap, const int64_t* invk,
+                               const int32_t* irr_chi, const int32_t* chi_map, const int64_t* olist,
+                               const double* temp, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+                               const int64_t* Nw, const int64_t* Nk, const int64_t* Nkall, const int64_t* Norb,
+                               const int64_t* Nchi) {
+  GMOD_TRY
+  auto g = api_lock();
+  OrbPairs op;
+  op.from_olist(olist, (int)*Nchi, (int)*Norb);
+  op.from_maps(irr_chi, chi_map);
+  if (ctx().precision == 1)
+    chi0_driver<float>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
+                       false, nullptr, nullptr, 0.0, nullptr, nullptr);
+  else
+    chi0_driver<double>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
+                        false, nullptr, nullptr, 0.0, nullptr, nullptr);
+  GMOD_CATCH
+}
+
+// ====================================================================== vertices / RPA
+extern "C" void get_vsigma_flex_nosoc_(gmod_z* chi, const double* Smat, const double* Cmat, const int64_t* Nk,
+                                       const int64_t* Nw, const int64_t* Nchi) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const long NkNw = *Nk * *Nw;
+  const int n = (int)*Nchi;
+  Upload up{s};
+  DBuf<zd> c;
+  DBuf<double> S, C;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(c, (const zd*)chi, (size_t)NkNw * n * n);
+  up(S, Smat, (size_t)n * n);
+  up(C, Cmat, (size_t)n * n);
+  rpa_vertex(c.p, NkNw, BatchLayout{1, NkNw}, n, S.p, C.p, 0, err.p, s);
+  check_err(err, s, "get_vsigma_flex_nosoc");
+  c.download((zd*)chi, (size_t)NkNw * n * n, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+extern "C" void get_chis(gmod_z* chis, const gmod_z* chi0, const double* Smat, const int64_t* Nchi,
+                         const int64_t* Nw) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int n = (int)*Nchi;
+  const long nw = *Nw;
+  Upload up{s};
+  DBuf<zd> a, o((size_t)n * n * nw);
+  DBuf<double> S;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(a, (const zd*)chi0, (size_t)n * n * nw);
+  up(S, Smat, (size_t)n * n);
+  rpa_solve(o.p, BatchLayout{(long)n * n, 1}, a.p, BatchLayout{(long)n * n, 1}, nw, n, S.p, -1.0, err.p, s);
+  check_err(err, s, "get_chis");
+  o.download((zd*)chis, (size_t)n * n * nw, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+extern "C" void get_chis_chic(gmod_z* chis, gmod_z* chic, const gmod_z* chi, const double* Smat, const double* Cmat,
+                              const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int n = (int)*Nchi;
+  const long nk = *Nk, NkNw = *Nk * *Nw;
+  Upload up{s};
+  DBuf<zd> a, os((size_t)nk * n * n), oc((size_t)nk * n * n);
+  DBuf<double> S, C;
+  DBuf<int> err(1);
+  err.zero(s);
+  // only the static slice chi(:,1,:,:) is read: copy it with the full layout
+  up(a, (const zd*)chi, (size_t)NkNw * n * n);
+  up(S, Smat, (size_t)n * n);
+  up(C, Cmat, (size_t)n * n);
+  rpa_solve(os.p, BatchLayout{1, nk}, a.p, BatchLayout{1, NkNw}, nk, n, S.p, -1.0, err.p, s);
+  rpa_solve(oc.p, BatchLayout{1, nk}, a.p, BatchLayout{1, NkNw}, nk, n, C.p, 1.0, err.p, s);
+  check_err(err, s, "get_chis_chic");
+  os.download((zd*)chis, (size_t)nk * n * n, s);
+  oc.download((zd*)chic, (size_t)nk * n * n, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+__global__ void k_trace_map(zd* out, const zd* chi, const int64_t* invk, long Nkall, long Nk, int n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < Nkall; i += (long)gridDim.x * blockDim.x) {
+    long k = invk[3 * i] - 1;
+    zd t(0, 0);
+    for (int l = 0; l < n; ++l) t += chi[k + Nk * (l + (long)n * l)];
+    out[i] = t;
+  }
+}
+__global__ void k_eig_map(zd* out, const double* v, const int64_t* invk, long Nkall) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < Nkall; i += (long)gridDim.x * blockDim.x)
+    out[i] = zd(v[invk[3 * i] - 1], 0.0);
+}
+
+extern "C" void get_eig_or_tr_chi(gmod_z* chiqout, const gmod_z* chi, const int64_t* invk, const int64_t* Nkall,
+                                  const int64_t* Nk, const int64_t* Nchi, const bool* sw_eig) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int n = (int)*Nchi;
+  const long nk = *Nk, nka = *Nkall;
+  Upload up{s};
+  DBuf<zd> a, o(nka);
+  DBuf<int64_t> iv;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(a, (const zd*)chi, (size_t)nk * n * n);
+  up(iv, invk, (size_t)3 * nka);
+  if (*sw_eig) {
+    DBuf<double> v(nk);
+    max_re_eig(v.p, a.p, BatchLayout{1, nk}, nk, n, nullptr, 1.0, err.p, s);
+    check_err(err, s, "get_eig_or_tr_chi");
+    hipLaunchKernelGGL(k_eig_map, dim3(256), dim3(256), 0, s, o.p, v.p, iv.p, nka);
+  } else {
+    hipLaunchKernelGGL(k_trace_map, dim3(256), dim3(256), 0, s, o.p, a.p, iv.p, nka, nk, n);
+  }
+  GMOD_CHECK_LAUNCH();
+  o.download((zd*)chiqout, nka, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+// ====================================================================== Green functions
+extern "C" void gen_green0_(gmod_z* Gk, const double* eig, const gmod_z* uni, const double* mu, const double* temp,
+                            const int64_t* Nk, const int64_t* Nw, const int64_t* Norb) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int No = (int)*Norb;
+  const size_t n = (size_t)*Nk * *Nw * No * No;
+  Upload up{s};
+  DBuf<double> e;
+  DBuf<zd> u, G(n);
+  up(e, eig, (size_t)No * *Nk);
+  up(u, (const zd*)uni, (size_t)No * No * *Nk);
+  green0(G.p, e.p, u.p, *mu, *temp, *Nk, (int)*Nw, No, s);
+  G.download((zd*)Gk, n, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+extern "C" void gen_green_inv_(gmod_z* Gk, const gmod_z* self, const gmod_z* hamk, const double* mu,
+                               const double* temp, const int64_t* Nk, const int64_t* Nw, const int64_t* Norb) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int No = (int)*Norb;
+  const size_t n = (size_t)*Nk * *Nw * No * No;
+  Upload up{s};
+  DBuf<zd> sg, h, G(n);
+  up(sg, (const zd*)self, n);
+  up(h, (const zd*)hamk, (size_t)No * No * *Nk);
+  green_inv(G.p, sg.p, h.p, *mu, *temp, *Nk, (int)*Nw, No, s);
+  G.download((zd*)Gk, n, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+extern "C" void getinv_(gmod_z* Gk, const int64_t* Nk, const int64_t* Nw, const int64_t* Norb) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int No = (int)*Norb;
+  const long tot = *Nk * *Nw;
+  const size_t n = (size_t)tot * No * No;
+  Upload up{s};
+  DBuf<zd> G;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(G, (const zd*)Gk, n);
+  batched_inv(G.p, tot, No, err.p, s);
+  check_err(err, s, "getinv");
+  G.download((zd*)Gk, n, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+// ====================================================================== FLEX
+extern "C" void mkself(gmod_z* sigmak, double* mu, const double* Smat, const double* Cmat, const int64_t* kmap,
+                       const int64_t* invk, const int64_t* olist, const gmod_z* hamk, const double* eig,
+                       const gmod_z* uni, const double* mu_init, const double* rfill, const double* temp,
+                       const int64_t* scf_loop, const double* pp, const double* eps, const int64_t* Nkall,
+                       const int64_t* Nk, const int64_t* Nw, const int64_t* Norb, const int64_t* Nchi,
+                       const int64_t* Nx, const int64_t* Ny, const int64_t* Nz, const int64_t* sub_sigma,
+                       const bool* sw_out, const bool* sw_in, const int64_t* m_diis, const bool* sw_rescale,
+                       const bool* sw_tail, const double* sigma_scale) {
+  GMOD_TRY
+  auto g = api_lock();
+  FlexIn A;
+  A.Smat = Smat; A.Cmat = Cmat; A.eig = eig;
+  A.kmap = kmap; A.invk = invk; A.olist = olist;
+  A.hamk = (const zd*)hamk; A.uni = (const zd*)uni;
+  A.mu_init = *mu_init; A.rfill = *rfill; A.temp = *temp; A.pp = *pp; A.eps = *eps; A.sigma_scale = *sigma_scale;
+  A.scf_loop = *scf_loop; A.Nkall = *Nkall; A.Nk = *Nk; A.Nw = *Nw; A.Nx = *Nx; A.Ny = *Ny; A.Nz = *Nz;
+  A.sub_sigma = *sub_sigma; A.m_diis = *m_diis;
+  A.No = (int)*Norb; A.Nchi = (int)*Nchi;
+  A.sw_out = *sw_out; A.sw_in = *sw_in; A.sw_rescale = *sw_rescale; A.sw_tail = *sw_tail;
+  if (ctx().precision == 1) mkself_driver<float>(A, (zd*)sigmak, mu);
+  else mkself_driver<double>(A, (zd*)sigmak, mu);
+  GMOD_CATCH
+}
+
+extern "C" void gmod_set_sigma_seed(const gmod_z* sigma, double mu, double mu_old, int64_t count) {
+  GMOD_TRY
+  g_seed.sigma.assign((const zd*)sigma, (const zd*)sigma + count);
+  g_seed.mu = mu;
+  g_seed.mu_old = mu_old;
+  g_seed.valid = true;
+  GMOD_CATCH
+}
+
+extern "C" int gmod_flex_stats(double* out, int n) {
+  int m = std::min(n, 9);
+  std::copy(g_stats, g_stats + m, out);
+  return m;
+}
+
+// ====================================================================== Eliashberg
+extern "C" void lin_eliash(gmod_z* delta, gmod_z* chi, const gmod_z* Gk, const gmod_z* uni, const double* init_delta,
+                           const double* Smat, const double* Cmat, const int64_t* olist, const double* prt,
+                           const int64_t* kmap, const int64_t* invk, const double* temp, const double* eps,
+                           const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi,
+                           const int64_t* Norb, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+                           const int64_t* itemax, const int64_t* gap_sym, const int64_t* arnoldi_m,
+                           double* lambda_out) {
+  GMOD_TRY
+  auto g = api_lock();
+  EliashIn A;
+  A.chi = (zd*)chi; A.Gk = (const zd*)Gk; A.uni = (const zd*)uni;
+  A.init_delta = init_delta; A.Smat = Smat; A.Cmat = Cmat; A.prt = prt;
+  A.olist = olist; A.kmap = kmap; A.invk = invk;
+  A.temp = *temp; A.eps = *eps;
+  A.Nkall = *Nkall; A.Nk = *Nk; A.Nw = *Nw; A.Nx = *Nx; A.Ny = *Ny; A.Nz = *Nz;
+  A.itemax = *itemax; A.gap_sym = *gap_sym; A.arnoldi_m = *arnoldi_m;
+  A.No = (int)*Norb; A.Nchi = (int)*Nchi;
+  *lambda_out = 0.0;
+  if (ctx().precision == 1) *lambda_out = eliash_driver<float>(A, (zd*)delta);
+  else *lambda_out = eliash_driver<double>(A, (zd*)delta);
+  GMOD_CATCH
+}
+
+extern "C" void mkfk_trs_nsoc_(gmod_z* fk, const gmod_z* Gk, const gmod_z* delta, const int64_t* Nk,
+                               const int64_t* Nw, const int64_t* Norb) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int No = (int)*Norb;
+  const size_t n = (size_t)*Nk * *Nw * No * No;
+  Upload up{s};
+  DBuf<zd> G, D, F(n);
+  up(G, (const zd*)Gk, n);
+  up(D, (const zd*)delta, n);
+  anomalous_f(F.p, G.p, D.p, *Nk, (int)*Nw, No, s);
+  F.download((zd*)fk, n, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+extern "C" void get_initial_delta_(gmod_z* delta, const double* init_delta, const gmod_z* uni, const int64_t* kmap,
+                                   const int64_t* invk, const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw,
+                                   const int64_t* Norb, const int64_t* gap_sym) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int No = (int)*Norb;
+  // only the multiplicities are needed from the mesh
+  Mesh mesh;
+  long Nkall_ = *Nkall;
+  long nx = Nkall_, ny = 1, nz = 1;
+  std::vector<int64_t> km(3 * Nkall_);
+  for (long i = 0; i < Nkall_; ++i) { km[3 * i] = i; km[3 * i + 1] = 0; km[3 * i + 2] = 0; }
+  (void)kmap;
+  mesh.build(km.data(), invk, Nkall_, *Nk, (int)nx, (int)ny, (int)nz, (int)*Nw, s);
+  const size_t n = (size_t)*Nk * *Nw * No * No;
+  DBuf<zd> D(n);
+  initial_delta(D.p, init_delta, (const zd*)uni, mesh, No, *gap_sym, s);
+  D.download((zd*)delta, n, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+// band-basis gap at w_0 on the full BZ (feliash.f90:733-785): output formatting, host
+extern "C" void conv_delta_orb_to_band(gmod_z* deltab, const gmod_z* delta, const gmod_z* uni, const double* prt,
+                                       const int64_t* invk, const int64_t* Norb, const int64_t* Nkall,
+                                       const int64_t* Nk, const int64_t* Nw, const int64_t* gap_sym) {
+  GMOD_TRY
+  const int No = (int)*Norb;
+  const long nka = *Nkall, nk = *Nk, NkNw = *Nk * *Nw;
+  const double dprt = *gap_sym < 0 ? -1.0 : 1.0;
+  const zd* D = (const zd*)delta;
+  const zd* U = (const zd*)uni;
+  zd* out = (zd*)deltab;
+  std::vector<zd> t(No * No);
+  for (long i = 0; i < nka; ++i) {
+    long k = invk[3 * i] - 1;
+    auto Uk = [&](int a, int b) { return U[a + No * (b + (long)No * k)]; };
+    auto Dk = [&](int a, int b) { return D[k + NkNw * (a + (long)No * b)]; };
+    for (int p = 0; p < No; ++p)
+      for (int q = 0; q < No; ++q) out[i + nka * (p + (long)No * q)] = zd(0, 0);
+    if (invk[3 * i + 1] == 0) {
+      for (int p = 0; p < No; ++p)
+        for (int q = 0; q < No; ++q) {
+          zd acc(0, 0);
+          for (int r = 0; r < No; ++r) acc += conj(Uk(r, p)) * Dk(r, q);
+          t[p + No * q] = acc;
+        }
+      for (int p = 0; p < No; ++p)
+        for (int q = 0; q < No; ++q) {
+          zd acc(0, 0);
+          for (int r = 0; r < No; ++r) acc += t[p + No * r] * Uk(r, q);
+          out[i + nka * (p + (long)No * q)] = acc;
+        }
+    } else if (invk[3 * i + 1] == 1) {
+      for (int p = 0; p < No; ++p)
+        for (int q = 0; q < No; ++q) {
+          zd acc(0, 0);
+          for (int r = 0; r < No; ++r) acc += Uk(r, p) * prt[r] * Dk(r, q);
+          t[p + No * q] = acc * (dprt * prt[q]);
+        }
+      for (int p = 0; p < No; ++p)
+        for (int q = 0; q < No; ++q) {
+          zd acc(0, 0);
+          for (int r = 0; r < No; ++r) acc += t[p + No * r] * conj(Uk(r, q));
+          out[i + nka * (p + (long)No * q)] = acc;
+        }
+    }
+  }
+  (void)nk;
+  GMOD_CATCH
+}
+
+extern "C" void remap_delta_(gmod_z* delta, const gmod_z* delta0, const double* prt, const int64_t* invk,
+                             const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Norb,
+                             const int64_t* gap_sym) {
+  GMOD_TRY
+  const int No = (int)*Norb;
+  const long nka = *Nkall, nk = *Nk, nw = *Nw;
+  const double dprt = *gap_sym < 0 ? -1.0 : 1.0;
+  const zd* D0 = (const zd*)delta0;
+  zd* D = (zd*)delta;
+  for (int l = 0; l < No; ++l)
+    for (int m = 0; m < No; ++m)
+      for (long j = 0; j < nw; ++j)
+        for (long i = 0; i < nka; ++i) {
+          long k = invk[3 * i] - 1;
+          zd v = D0[k + nk * (j + nw * (m + (long)No * l))];
+          if (invk[3 * i + 1] == 1) v = v * (dprt * prt[m] * prt[l]);
+          if (invk[3 * i + 1] == 0 || invk[3 * i + 1] == 1) D[i + nka * (j + nw * (m + (long)No * l))] = v;
+        }
+  GMOD_CATCH
+}

@@ -1,0 +1,56 @@
+// Batched small dense complex linear algebra on the device (FP64).
+#pragma once
+#include "gmod_common.h"
+
+namespace gmod {
+
+// Batched matrix addressing: matrix b, element (p,q) (column-major, p + n*q)
+// lives at base[b*bs + (p + n*q)*es].
+struct BatchLayout {
+  long bs, es;
+};
+
+// RPA vertex per (q,i nu), in place (fUJ.f90:253-365):
+//  mode 0: V_sigma = (1.5S-0.5C) + 1.5 S X_s + 0.5 C X_c - 0.25 (S+C)(chi0 S + chi0 C)
+//  mode 1: V_Delta singlet = (S+C)/2 + 1.5 S X_s - 0.5 C X_c
+//  mode 2: V_Delta triplet = (S-C)/2 - 0.5 S X_s - 0.5 C X_c
+//  X_s = (I - chi0 S)^-1 chi0 S,  X_c = (I + chi0 C)^-1 chi0 C  (LU with partial pivoting)
+void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, const double* C, int mode,
+                int* err, hipStream_t s);
+// X = (I + alpha A W)^-1 A  (fchi.f90:319-350, fself.f90:783-816); out may alias nothing
+void rpa_solve(zd* out, BatchLayout Lo, const zd* A, BatchLayout La, long batch, int n, const double* W,
+               double alpha, int* err, hipStream_t s);
+// max Re eig(alpha * A * W) per matrix (W = nullptr: identity)  (fchi.f90:373-389)
+void max_re_eig(double* out, const zd* A, BatchLayout La, long batch, int n, const double* W, double alpha,
+                int* err, hipStream_t s);
+
+// Dyson: G = [(i w_j + mu) I - H(k) - Sigma(k, i w_j)]^-1   (fself.f90:43-83, 131-157)
+void dyson(zd* G, const zd* sigma, const zd* hamk, double mu, double temp, long Nk, int Nw, int No, int* err,
+           hipStream_t s);
+// n(mu) pieces (fself.f90:687-714): returns {sum_k,n f(e), sum_{k,j} Re[Tr(z-H-Sigma)^-1 - sum_n 1/(z-e_n)]}
+void filling_parts(double out[2], const zd* sigma, const zd* hamk, const double* eig, double rmu, double temp,
+                   long Nk, int Nw, int No, int* err, hipStream_t s);
+// G0 (fself.f90:1-41)
+void green0(zd* G, const double* eig, const zd* uni, double mu, double temp, long Nk, int Nw, int No,
+            hipStream_t s);
+// in-place inverse of a (tot, No, No) batch (zgetrf + zgetri, fself.f90:131-157)
+void batched_inv(zd* A, long tot, int No, int* err, hipStream_t s);
+// G^-1 = (i w + mu) I - H - Sigma (fself.f90:43-83)
+void green_inv(zd* G, const zd* sigma, const zd* hamk, double mu, double temp, long Nk, int Nw, int No,
+               hipStream_t s);
+// F = -G Delta G^dagger, Hermitised (feliash.f90:445-484)
+void anomalous_f(zd* F, const zd* G, const zd* D, long Nk, int Nw, int No, hipStream_t s);
+// Hermitise: upper = conj(lower), diagonal real  (feliash.f90:618-634)
+void hermitize_lower(zd* D, long Nk, int Nw, int No, hipStream_t s);
+// sub_sigma = 1 / 2 static subtraction (fself.f90:384-417)
+void subtract_static(zd* sigma, long Nk, int Nw, int No, int mode, hipStream_t s);
+
+// Host-side complex Hessenberg QR eigen-solver (also compiled for the device).
+int host_eigvals(std::vector<zd>& A, int n, std::vector<zd>& w);
+// Right eigenvector of A (n x n) for eigenvalue lam, LAPACK zgeev normalisation
+// (unit 2-norm, largest-modulus component real and positive).
+std::vector<zd> host_eigvec(const std::vector<zd>& A, int n, zd lam);
+// Real linear solve with partial pivoting (dgesv); returns false if singular.
+bool host_dgesv(std::vector<double>& A, int n, std::vector<double>& b);
+
+}  // namespace gmod

@@ -1,0 +1,750 @@
+// Batched small dense complex linear algebra (FP64) for gfx950.
+//
+// * rpa_vertex / rpa_solve: one 64-lane wavefront per (q, i nu) matrix.  The
+//   matrix and both right-hand sides live in LDS; the LU with partial pivoting
+//   (pivot = max |Re|+|Im|, as LAPACK's izamax) updates the trailing block with
+//   one lane per element.
+// * dyson / filling_parts / green0 / anomalous_f: one lane per (k, i w) with
+//   the No x No matrix held in registers (No <= 8).
+// * max_re_eig: one lane per matrix; Hessenberg reduction + shifted complex QR.
+#include <cmath>
+
+#include "linalg.h"
+
+namespace gmod {
+
+constexpr double PI_D = 3.141592653589793238462643383279;  // fmod.f90:4
+
+static inline unsigned nblk(long n, int tpb, long cap = 1L << 20) {
+  long b = (n + tpb - 1) / tpb;
+  if (b > cap) b = cap;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+// ====================================================================== LDS LU
+__device__ bool lds_lu_solve(zd* Mx, zd* X, int n, int nrhs, int* shi) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = 0; k < n; ++k) {
+    if (tid == 0) {
+      int p = k;
+      double best = abs1(Mx[k + n * k]);
+      for (int r = k + 1; r < n; ++r) {
+        double v = abs1(Mx[r + n * k]);
+        if (v > best) { best = v; p = r; }
+      }
+      shi[0] = p;
+      shi[1] = best == 0.0 ? 1 : 0;
+    }
+    __syncthreads();
+    const int p = shi[0];
+    if (shi[1]) return false;
+    if (p != k) {
+      for (int c = tid; c < n; c += nt) { zd t = Mx[k + n * c]; Mx[k + n * c] = Mx[p + n * c]; Mx[p + n * c] = t; }
+      for (int c = tid; c < nrhs; c += nt) { zd t = X[k + n * c]; X[k + n * c] = X[p + n * c]; X[p + n * c] = t; }
+    }
+    __syncthreads();
+    const zd d = Mx[k + n * k];
+    for (int r = k + 1 + tid; r < n; r += nt) Mx[r + n * k] = Mx[r + n * k] / d;
+    __syncthreads();
+    const int rows = n - k - 1;
+    const int cols = rows + nrhs;
+    for (int e = tid; e < rows * cols; e += nt) {
+      int r = k + 1 + e % rows;
+      int cc = e / rows;
+      zd l = Mx[r + n * k];
+      if (cc < rows) {
+        int c = k + 1 + cc;
+        Mx[r + n * c] -= l * Mx[k + n * c];
+      } else {
+        int c = cc - rows;
+        X[r + n * c] -= l * X[k + n * c];
+      }
+    }
+    __syncthreads();
+  }
+  for (int c = tid; c < nrhs; c += nt) {
+    for (int k = n - 1; k >= 0; --k) {
+      zd x = X[k + n * c];
+      for (int j = k + 1; j < n; ++j) x -= Mx[k + n * j] * X[j + n * c];
+      X[k + n * c] = x / Mx[k + n * k];
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
+__global__ void __launch_bounds__(64)
+k_rpa_vertex(zd* chi, BatchLayout L, long batch, int n, const double* S, const double* C, int mode, int* err) {
+  extern __shared__ unsigned char sm[];
+  const int nn = n * n;
+  double* Ss = reinterpret_cast<double*>(sm);
+  double* Cs = Ss + nn;
+  zd* A = reinterpret_cast<zd*>(Cs + nn);
+  zd* M1 = A + nn;
+  zd* X1 = M1 + nn;
+  zd* M2 = X1 + nn;
+  zd* X2 = M2 + nn;
+  __shared__ int shi[2];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < nn; e += nt) { Ss[e] = S[e]; Cs[e] = C[e]; }
+  for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+    zd* base = chi + b * L.bs;
+    for (int e = tid; e < nn; e += nt) A[e] = base[e * L.es];
+    __syncthreads();
+    for (int e = tid; e < nn; e += nt) {
+      int p = e % n, q = e / n;
+      zd as(0, 0), ac(0, 0);
+      for (int r = 0; r < n; ++r) {
+        zd a = A[p + n * r];
+        as += a * Ss[r + n * q];
+        ac += a * Cs[r + n * q];
+      }
+      double dl = (p == q) ? 1.0 : 0.0;
+      M1[e] = zd(dl, 0) - as;
+      X1[e] = as;
+      M2[e] = zd(dl, 0) + ac;
+      X2[e] = ac;
+    }
+    __syncthreads();
+    if (mode == 0) {
+      for (int e = tid; e < nn; e += nt) A[e] = X1[e] + X2[e];  // chi0 S + chi0 C
+      __syncthreads();
+    }
+    bool ok = lds_lu_solve(M1, X1, n, n, shi);
+    ok = ok && lds_lu_solve(M2, X2, n, n, shi);
+    if (!ok) {
+      if (tid == 0) atomicOr(err, 1);
+      __syncthreads();
+      continue;
+    }
+    for (int e = tid; e < nn; e += nt) {
+      int p = e % n, q = e / n;
+      zd v;
+      if (mode == 0) {
+        v = zd(1.5 * Ss[e] - 0.5 * Cs[e], 0);
+        for (int r = 0; r < n; ++r) {
+          double s = Ss[p + n * r], c = Cs[p + n * r];
+          v += X1[r + n * q] * (1.5 * s) + X2[r + n * q] * (0.5 * c) - A[r + n * q] * (0.25 * (s + c));
+        }
+      } else if (mode == 1) {
+        v = zd(0.5 * (Ss[e] + Cs[e]), 0);
+        for (int r = 0; r < n; ++r)
+          v += X1[r + n * q] * (1.5 * Ss[p + n * r]) - X2[r + n * q] * (0.5 * Cs[p + n * r]);
+      } else {
+        v = zd(0.5 * (Ss[e] - Cs[e]), 0);
+        for (int r = 0; r < n; ++r)
+          v -= X1[r + n * q] * (0.5 * Ss[p + n * r]) + X2[r + n * q] * (0.5 * Cs[p + n * r]);
+      }
+      base[e * L.es] = v;
+    }
+    __syncthreads();
+  }
+}
+
+static void check_lds(size_t bytes) {
+  if (bytes > 160 * 1024) throw Error("Nchi too large for the LDS-resident solver");
+}
+
+void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, const double* C, int mode, int* err,
+                hipStream_t s) {
+  if (batch <= 0) return;
+  size_t shm = 2 * n * n * sizeof(double) + 5 * n * n * sizeof(zd);
+  check_lds(shm);
+  unsigned grid = (unsigned)std::min<long>(batch, 256L * 64);
+  hipLaunchKernelGGL(k_rpa_vertex, dim3(grid), dim3(64), shm, s, chi, L, batch, n, S, C, mode, err);
+  GMOD_CHECK_LAUNCH();
+}
+
+__global__ void __launch_bounds__(64)
+k_rpa_solve(zd* out, BatchLayout Lo, const zd* Ain, BatchLayout La, long batch, int n, const double* W,
+            double alpha, int* err) {
+  extern __shared__ unsigned char sm[];
+  const int nn = n * n;
+  double* Ws = reinterpret_cast<double*>(sm);
+  zd* A = reinterpret_cast<zd*>(Ws + nn);
+  zd* Mx = A + nn;
+  zd* X = Mx + nn;
+  __shared__ int shi[2];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < nn; e += nt) Ws[e] = W[e];
+  for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+    const zd* base = Ain + b * La.bs;
+    for (int e = tid; e < nn; e += nt) A[e] = base[e * La.es];
+    __syncthreads();
+    for (int e = tid; e < nn; e += nt) {
+      int p = e % n, q = e / n;
+      zd aw(0, 0);
+      for (int r = 0; r < n; ++r) aw += A[p + n * r] * Ws[r + n * q];
+      Mx[e] = zd(p == q ? 1.0 : 0.0, 0) + aw * alpha;
+      X[e] = A[e];
+    }
+    __syncthreads();
+    if (!lds_lu_solve(Mx, X, n, n, shi)) {
+      if (tid == 0) atomicOr(err, 1);
+      __syncthreads();
+      continue;
+    }
+    zd* ob = out + b * Lo.bs;
+    for (int e = tid; e < nn; e += nt) ob[e * Lo.es] = X[e];
+    __syncthreads();
+  }
+}
+
+void rpa_solve(zd* out, BatchLayout Lo, const zd* A, BatchLayout La, long batch, int n, const double* W,
+               double alpha, int* err, hipStream_t s) {
+  if (batch <= 0) return;
+  size_t shm = n * n * sizeof(double) + 3 * n * n * sizeof(zd);
+  check_lds(shm);
+  unsigned grid = (unsigned)std::min<long>(batch, 256L * 64);
+  hipLaunchKernelGGL(k_rpa_solve, dim3(grid), dim3(64), shm, s, out, Lo, A, La, batch, n, W, alpha, err);
+  GMOD_CHECK_LAUNCH();
+}
+
+// ====================================================================== eigenvalues
+// Complex general eigenvalues: Gaussian reduction to Hessenberg form with
+// partial pivoting, then single-shift QR (Wilkinson shift, exceptional shift
+// every 10 sweeps) with Givens rotations restricted to the active block.
+// Column-major H (ld = n).  Returns 0 on success.
+__host__ __device__ inline zd zsqrt_(zd a) {
+  double r = hypot(a.re, a.im);
+  if (r == 0.0) return zd(0, 0);
+  double x = sqrt(0.5 * (r + fabs(a.re)));
+  double y = 0.5 * a.im / x;
+  if (a.re >= 0) return zd(x, y);
+  return zd(fabs(y), a.im >= 0 ? x : -x);
+}
+
+__host__ __device__ int zhess_eig(zd* H, int n, zd* w) {
+#define HH(i, j) H[(i) + n * (j)]
+  // Hessenberg reduction (elementary stabilised transformations)
+  for (int m = 1; m < n - 1; ++m) {
+    int piv = m;
+    double best = abs1(HH(m, m - 1));
+    for (int i = m + 1; i < n; ++i) {
+      double v = abs1(HH(i, m - 1));
+      if (v > best) { best = v; piv = i; }
+    }
+    if (piv != m) {
+      for (int j = m - 1; j < n; ++j) { zd t = HH(piv, j); HH(piv, j) = HH(m, j); HH(m, j) = t; }
+      for (int i = 0; i < n; ++i) { zd t = HH(i, piv); HH(i, piv) = HH(i, m); HH(i, m) = t; }
+    }
+    zd x = HH(m, m - 1);
+    if (x.re != 0.0 || x.im != 0.0) {
+      for (int i = m + 1; i < n; ++i) {
+        zd y = HH(i, m - 1);
+        if (y.re == 0.0 && y.im == 0.0) continue;
+        y = y / x;
+        HH(i, m - 1) = zd(0, 0);
+        for (int j = m; j < n; ++j) HH(i, j) -= y * HH(m, j);
+        for (int j = 0; j < n; ++j) HH(j, m) += y * HH(j, i);
+      }
+    }
+  }
+  double anorm = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n && i <= j + 1; ++i) anorm = fmax(anorm, abs1(HH(i, j)));
+  const double eps = 2.220446049250313e-16;
+  int hi = n - 1;
+  int iter = 0, total = 0;
+  while (hi >= 0) {
+    if (hi == 0) { w[0] = HH(0, 0); break; }
+    int l = hi;
+    for (; l > 0; --l) {
+      double sref = abs1(HH(l - 1, l - 1)) + abs1(HH(l, l));
+      if (sref == 0.0) sref = anorm;
+      if (abs1(HH(l, l - 1)) <= eps * sref) break;
+    }
+    if (l == hi) { w[hi] = HH(hi, hi); --hi; iter = 0; continue; }
+    ++iter;
+    if (++total > 60 * n) return 1;
+    zd sigma;
+    if (iter % 10 == 0) {
+      double ex = fabs(HH(hi, hi - 1).re) + (hi >= 2 ? fabs(HH(hi - 1, hi - 2).re) : 0.0);
+      sigma = HH(hi, hi) + zd(ex, 0);
+    } else {
+      zd a = HH(hi - 1, hi - 1), b = HH(hi - 1, hi), c = HH(hi, hi - 1), d = HH(hi, hi);
+      zd half = (a - d) * 0.5;
+      zd disc = zsqrt_(half * half + b * c);
+      zd m = (a + d) * 0.5;
+      zd e1 = m + disc, e2 = m - disc;
+      sigma = abs1(e1 - d) < abs1(e2 - d) ? e1 : e2;
+    }
+    for (int k = l; k <= hi; ++k) HH(k, k) -= sigma;
+    // QR by Givens on rows (store rotations in the subdiagonal workspace)
+    double cs[64];
+    zd sn[64];
+    const int nrot = hi - l;
+    if (nrot > 64) return 2;
+    for (int k = l; k < hi; ++k) {
+      zd x = HH(k, k), y = HH(k + 1, k);
+      double ax = hypot(x.re, x.im), ay = hypot(y.re, y.im);
+      double r = hypot(ax, ay);
+      double c;
+      zd sv;
+      if (r == 0.0) { c = 1.0; sv = zd(0, 0); }
+      else if (ax == 0.0) { c = 0.0; sv = conj(y) * (1.0 / ay); }
+      else { zd ph = x * (1.0 / ax); c = ax / r; sv = ph * conj(y) * (1.0 / r); }
+      cs[k - l] = c;
+      sn[k - l] = sv;
+      for (int j = k; j <= hi; ++j) {
+        zd h1 = HH(k, j), h2 = HH(k + 1, j);
+        HH(k, j) = h1 * c + sv * h2;
+        HH(k + 1, j) = h2 * c - conj(sv) * h1;
+      }
+    }
+    for (int k = l; k < hi; ++k) {
+      double c = cs[k - l];
+      zd sv = sn[k - l];
+      int rmax = k + 1 < hi ? k + 1 : hi;
+      for (int i = l; i <= rmax; ++i) {
+        zd h1 = HH(i, k), h2 = HH(i, k + 1);
+        HH(i, k) = h1 * c + h2 * conj(sv);
+        HH(i, k + 1) = h2 * c - h1 * sv;
+      }
+    }
+    for (int k = l; k <= hi; ++k) HH(k, k) += sigma;
+  }
+  return 0;
+#undef HH
+}
+
+__global__ void k_max_re_eig(double* out, const zd* A, BatchLayout La, long batch, int n, const double* W,
+                             double alpha, zd* scratch, int* err) {
+  for (long b = blockIdx.x * (long)blockDim.x + threadIdx.x; b < batch; b += (long)gridDim.x * blockDim.x) {
+    zd* H = scratch + b * (long)(n * n + n);
+    zd* w = H + n * n;
+    const zd* base = A + b * La.bs;
+    for (int q = 0; q < n; ++q)
+      for (int p = 0; p < n; ++p) {
+        zd v(0, 0);
+        if (W) {
+          for (int r = 0; r < n; ++r) v += base[(p + n * r) * La.es] * W[r + n * q];
+        } else {
+          v = base[(p + n * q) * La.es];
+        }
+        H[p + n * q] = v * alpha;
+      }
+    if (zhess_eig(H, n, w)) atomicOr(err, 2);
+    double m = -HUGE_VAL;
+    for (int i = 0; i < n; ++i) m = fmax(m, w[i].re);
+    out[b] = m;
+  }
+}
+
+void max_re_eig(double* out, const zd* A, BatchLayout La, long batch, int n, const double* W, double alpha,
+                int* err, hipStream_t s) {
+  if (batch <= 0) return;
+  DBuf<zd> scratch(batch * (long)(n * n + n));
+  hipLaunchKernelGGL(k_max_re_eig, dim3(nblk(batch, 64)), dim3(64), 0, s, out, A, La, batch, n, W, alpha,
+                     scratch.p, err);
+  GMOD_CHECK_LAUNCH();
+  GMOD_HIP(hipStreamSynchronize(s));
+}
+
+// ====================================================================== register kernels
+template <int NO>
+__device__ inline bool gj_inverse(zd (&a)[NO][NO]) {
+  zd inv[NO][NO];
+#pragma unroll
+  for (int p = 0; p < NO; ++p)
+#pragma unroll
+    for (int q = 0; q < NO; ++q) inv[p][q] = zd(p == q ? 1.0 : 0.0, 0);
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    int piv = k;
+    double best = abs1(a[k][k]);
+#pragma unroll
+    for (int r = k + 1; r < NO; ++r) {
+      double v = abs1(a[r][k]);
+      if (v > best) { best = v; piv = r; }
+    }
+    if (best == 0.0) return false;
+#pragma unroll
+    for (int r = k + 1; r < NO; ++r) {
+      if (r == piv) {
+#pragma unroll
+        for (int c = 0; c < NO; ++c) {
+          zd t = a[k][c]; a[k][c] = a[r][c]; a[r][c] = t;
+          t = inv[k][c]; inv[k][c] = inv[r][c]; inv[r][c] = t;
+        }
+      }
+    }
+    zd d = zd(1.0, 0) / a[k][k];
+#pragma unroll
+    for (int c = 0; c < NO; ++c) { a[k][c] = a[k][c] * d; inv[k][c] = inv[k][c] * d; }
+#pragma unroll
+    for (int r = 0; r < NO; ++r) {
+      if (r == k) continue;
+      zd f = a[r][k];
+#pragma unroll
+      for (int c = 0; c < NO; ++c) { a[r][c] -= f * a[k][c]; inv[r][c] -= f * inv[k][c]; }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NO; ++p)
+#pragma unroll
+    for (int q = 0; q < NO; ++q) a[p][q] = inv[p][q];
+  return true;
+}
+
+template <int NO>
+__global__ void k_dyson(zd* G, const zd* sig, const zd* H, double mu, double temp, long Nk, int Nw, int* err) {
+  const long tot = Nk * (long)Nw;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    long k = e % Nk;
+    long j = e / Nk;
+    zd z(mu, (double)(2 * j + 1) * PI_D * temp);
+    zd a[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        zd v = zd(0, 0) - (H[p + NO * (q + NO * k)] + sig[e + tot * (p + NO * q)]);
+        if (p == q) v += z;
+        a[p][q] = v;
+      }
+    if (!gj_inverse<NO>(a)) atomicOr(err, 4);
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) G[e + tot * (p + NO * q)] = a[p][q];
+  }
+}
+
+template <int NO>
+__global__ void k_filling(const zd* sig, const zd* H, const double* eig, double rmu, double temp, long Nk, int Nw,
+                          double* part, int* err) {
+  __shared__ double sh[2][256];
+  const long tot = Nk * (long)Nw;
+  double fsum = 0.0, dsum = 0.0;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    long k = e % Nk;
+    long j = e / Nk;
+    zd z(rmu, (double)(2 * j + 1) * PI_D * temp);
+    zd a[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        zd v = zd(0, 0) - (H[p + NO * (q + NO * k)] + sig[e + tot * (p + NO * q)]);
+        if (p == q) v += z;
+        a[p][q] = v;
+      }
+    if (!gj_inverse<NO>(a)) atomicOr(err, 8);
+    double d = 0.0;
+#pragma unroll
+    for (int n = 0; n < NO; ++n) {
+      d += a[n][n].re;
+      d -= (zd(1.0, 0) / (z - zd(eig[n + NO * k], 0))).re;
+    }
+    dsum += d;
+    if (j == 0) {
+#pragma unroll
+      for (int n = 0; n < NO; ++n) fsum += 0.5 * (1.0 - tanh(0.5 * (eig[n + NO * k] - rmu) / temp));
+    }
+  }
+  sh[0][threadIdx.x] = fsum;
+  sh[1][threadIdx.x] = dsum;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = sh[0][0]; part[2 * blockIdx.x + 1] = sh[1][0]; }
+}
+
+template <int NO>
+__global__ void k_green0(zd* G, const double* eig, const zd* U, double mu, double temp, long Nk, int Nw) {
+  const long tot = Nk * (long)Nw;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    long k = e % Nk;
+    long j = e / Nk;
+    zd z(mu, (double)(2 * j + 1) * PI_D * temp);
+    zd r[NO];
+#pragma unroll
+    for (int n = 0; n < NO; ++n) r[n] = zd(1.0, 0) / (z - zd(eig[n + NO * k], 0));
+#pragma unroll
+    for (int l = 0; l < NO; ++l)
+#pragma unroll
+      for (int m = 0; m < NO; ++m) {
+        zd acc(0, 0);
+#pragma unroll
+        for (int n = 0; n < NO; ++n) acc += U[m + NO * (n + NO * k)] * conj(U[l + NO * (n + NO * k)]) * r[n];
+        G[e + tot * (m + NO * l)] = acc;
+      }
+  }
+}
+
+template <int NO>
+__global__ void k_anom_f(zd* F, const zd* G, const zd* D, long Nk, int Nw) {
+  const long tot = Nk * (long)Nw;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    zd g[NO][NO], gd[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) g[p][q] = G[e + tot * (p + NO * q)];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        zd acc(0, 0);
+#pragma unroll
+        for (int r = 0; r < NO; ++r) acc += g[p][r] * D[e + tot * (r + NO * q)];
+        gd[p][q] = acc;
+      }
+    zd f[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        zd acc(0, 0);
+#pragma unroll
+        for (int r = 0; r < NO; ++r) acc += gd[p][r] * conj(g[q][r]);
+        f[p][q] = zd(0, 0) - acc;
+      }
+#pragma unroll
+    for (int l = 0; l < NO; ++l) {
+      f[l][l].im = 0.0;
+#pragma unroll
+      for (int m = l + 1; m < NO; ++m) f[l][m] = conj(f[m][l]);
+    }
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) F[e + tot * (p + NO * q)] = f[p][q];
+  }
+}
+
+__global__ void k_hermitize_lower(zd* D, long tot, int No) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    for (int l = 0; l < No; ++l) {
+      D[e + tot * (l + No * l)].im = 0.0;
+      for (int m = l + 1; m < No; ++m) D[e + tot * (l + No * m)] = conj(D[e + tot * (m + No * l)]);
+    }
+  }
+}
+
+__global__ void k_sub_static(zd* sig, long Nk, int Nw, int No, int mode) {
+  const long tot = Nk * (long)Nw;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < Nk; k += (long)gridDim.x * blockDim.x) {
+    zd sub[8][8];
+    for (int l = 0; l < No; ++l)
+      for (int m = l; m < No; ++m) {
+        zd v(0, 0);
+        if (mode == 1) {
+          v = (sig[k + tot * (l + No * m)] + conj(sig[k + tot * (m + No * l)])) * 0.5;
+        } else {
+          for (int j = 0; j < Nw; ++j)
+            v += sig[k + Nk * j + tot * (l + No * m)] + conj(sig[k + Nk * j + tot * (m + No * l)]);
+          v = v * (0.5 / (double)Nw);
+        }
+        sub[l][m] = v;
+        if (l != m) sub[m][l] = conj(v);
+      }
+    for (int j = 0; j < Nw; ++j)
+      for (int l = 0; l < No; ++l)
+        for (int m = 0; m < No; ++m) sig[k + Nk * j + tot * (l + No * m)] -= sub[l][m];
+  }
+}
+
+#define LAUNCH_NO(No, KERNEL, GRID, BLOCK, ...)                                            \
+  switch (No) {                                                                             \
+    case 1: hipLaunchKernelGGL((KERNEL<1>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+    case 2: hipLaunchKernelGGL((KERNEL<2>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+    case 3: hipLaunchKernelGGL((KERNEL<3>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+    case 4: hipLaunchKernelGGL((KERNEL<4>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+    case 5: hipLaunchKernelGGL((KERNEL<5>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+    case 6: hipLaunchKernelGGL((KERNEL<6>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+    case 7: hipLaunchKernelGGL((KERNEL<7>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+    case 8: hipLaunchKernelGGL((KERNEL<8>), GRID, BLOCK, 0, s, __VA_ARGS__); break;         \
+    default: throw Error("Norb > 8 is not supported by the register kernels");             \
+  }
+
+void dyson(zd* G, const zd* sigma, const zd* hamk, double mu, double temp, long Nk, int Nw, int No, int* err,
+           hipStream_t s) {
+  long tot = Nk * (long)Nw;
+  LAUNCH_NO(No, k_dyson, dim3(nblk(tot, 128)), dim3(128), G, sigma, hamk, mu, temp, Nk, Nw, err);
+  GMOD_CHECK_LAUNCH();
+}
+
+void filling_parts(double out[2], const zd* sigma, const zd* hamk, const double* eig, double rmu, double temp,
+                   long Nk, int Nw, int No, int* err, hipStream_t s) {
+  constexpr int NB = 1024;
+  DBuf<double> part(2 * NB);
+  long tot = Nk * (long)Nw;
+  LAUNCH_NO(No, k_filling, dim3(NB), dim3(256), sigma, hamk, eig, rmu, temp, Nk, Nw, part.p, err);
+  GMOD_CHECK_LAUNCH();
+  (void)tot;
+  std::vector<double> h(2 * NB);
+  GMOD_HIP(hipMemcpyAsync(h.data(), part.p, 2 * NB * sizeof(double), hipMemcpyDeviceToHost, s));
+  GMOD_HIP(hipStreamSynchronize(s));
+  out[0] = out[1] = 0.0;
+  for (int b = 0; b < NB; ++b) { out[0] += h[2 * b]; out[1] += h[2 * b + 1]; }
+}
+
+void green0(zd* G, const double* eig, const zd* uni, double mu, double temp, long Nk, int Nw, int No,
+            hipStream_t s) {
+  long tot = Nk * (long)Nw;
+  LAUNCH_NO(No, k_green0, dim3(nblk(tot, 128)), dim3(128), G, eig, uni, mu, temp, Nk, Nw);
+  GMOD_CHECK_LAUNCH();
+}
+
+void anomalous_f(zd* F, const zd* G, const zd* D, long Nk, int Nw, int No, hipStream_t s) {
+  long tot = Nk * (long)Nw;
+  LAUNCH_NO(No, k_anom_f, dim3(nblk(tot, 128)), dim3(128), F, G, D, Nk, Nw);
+  GMOD_CHECK_LAUNCH();
+}
+
+void hermitize_lower(zd* D, long Nk, int Nw, int No, hipStream_t s) {
+  long tot = Nk * (long)Nw;
+  hipLaunchKernelGGL(k_hermitize_lower, dim3(nblk(tot, 256)), dim3(256), 0, s, D, tot, No);
+  GMOD_CHECK_LAUNCH();
+}
+
+void subtract_static(zd* sigma, long Nk, int Nw, int No, int mode, hipStream_t s) {
+  if (No > 8) throw Error("Norb > 8 not supported");
+  hipLaunchKernelGGL(k_sub_static, dim3(nblk(Nk, 128)), dim3(128), 0, s, sigma, Nk, Nw, No, mode);
+  GMOD_CHECK_LAUNCH();
+}
+
+template <int NO>
+__global__ void k_inv(zd* A, long tot, int* err) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    zd a[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) a[p][q] = A[e + tot * (p + NO * q)];
+    if (!gj_inverse<NO>(a)) atomicOr(err, 4);
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) A[e + tot * (p + NO * q)] = a[p][q];
+  }
+}
+
+void batched_inv(zd* A, long tot, int No, int* err, hipStream_t s) {
+  LAUNCH_NO(No, k_inv, dim3(nblk(tot, 128)), dim3(128), A, tot, err);
+  GMOD_CHECK_LAUNCH();
+}
+
+// G^-1 = (i w_j + mu) I - H(k) - Sigma  (fself.f90:43-83)
+__global__ void k_green_inv(zd* G, const zd* sig, const zd* H, double mu, double temp, long Nk, int Nw, int No) {
+  const long tot = Nk * (long)Nw;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    long k = e % Nk, j = e / Nk;
+    zd z(mu, (double)(2 * j + 1) * PI_D * temp);
+    for (int p = 0; p < No; ++p)
+      for (int q = 0; q < No; ++q) {
+        zd v = zd(0, 0) - (H[p + No * (q + (long)No * k)] + sig[e + tot * (p + No * q)]);
+        if (p == q) v += z;
+        G[e + tot * (p + No * q)] = v;
+      }
+  }
+}
+
+void green_inv(zd* G, const zd* sigma, const zd* hamk, double mu, double temp, long Nk, int Nw, int No,
+               hipStream_t s) {
+  long tot = Nk * (long)Nw;
+  hipLaunchKernelGGL(k_green_inv, dim3(nblk(tot, 128)), dim3(128), 0, s, G, sigma, hamk, mu, temp, Nk, Nw, No);
+  GMOD_CHECK_LAUNCH();
+}
+
+// ====================================================================== host helpers
+int host_eigvals(std::vector<zd>& A, int n, std::vector<zd>& w) {
+  w.assign(n, zd(0, 0));
+  std::vector<zd> H = A;
+  return zhess_eig(H.data(), n, w.data());
+}
+
+static bool host_lu_solve(std::vector<zd> M, int n, std::vector<zd>& b) {
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = abs1(M[k + n * k]);
+    for (int r = k + 1; r < n; ++r)
+      if (abs1(M[r + n * k]) > best) { best = abs1(M[r + n * k]); p = r; }
+    if (best == 0.0) return false;
+    if (p != k) {
+      for (int c = 0; c < n; ++c) std::swap(M[k + n * c], M[p + n * c]);
+      std::swap(b[k], b[p]);
+    }
+    for (int r = k + 1; r < n; ++r) {
+      zd l = M[r + n * k] / M[k + n * k];
+      for (int c = k + 1; c < n; ++c) M[r + n * c] -= l * M[k + n * c];
+      b[r] -= l * b[k];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    zd x = b[k];
+    for (int j = k + 1; j < n; ++j) x -= M[k + n * j] * b[j];
+    b[k] = x / M[k + n * k];
+  }
+  return true;
+}
+
+std::vector<zd> host_eigvec(const std::vector<zd>& A, int n, zd lam) {
+  // inverse iteration on (A - lam I) with a tiny relative perturbation
+  double scale = 0.0;
+  for (auto v : A) scale = std::max(scale, abs1(v));
+  if (scale == 0.0) scale = 1.0;
+  std::vector<zd> M = A;
+  zd shift = lam + zd(scale * 1e-13, scale * 1e-13);
+  for (int i = 0; i < n; ++i) M[i + n * i] -= shift;
+  std::vector<zd> v(n);
+  for (int i = 0; i < n; ++i) v[i] = zd(1.0 / std::sqrt((double)n), 1e-3 * (i + 1));
+  for (int it = 0; it < 4; ++it) {
+    std::vector<zd> b = v;
+    if (!host_lu_solve(M, n, b)) {
+      // exact eigenvalue hit: perturb more
+      for (int i = 0; i < n; ++i) M[i + n * i] -= zd(scale * 1e-10, 0);
+      continue;
+    }
+    double nr = 0;
+    for (auto x : b) nr += x.re * x.re + x.im * x.im;
+    nr = std::sqrt(nr);
+    for (int i = 0; i < n; ++i) v[i] = b[i] * (1.0 / nr);
+  }
+  // LAPACK zgeev normalisation
+  int kmax = 0;
+  double best = -1;
+  for (int i = 0; i < n; ++i) {
+    double m2 = v[i].re * v[i].re + v[i].im * v[i].im;
+    if (m2 > best) { best = m2; kmax = i; }
+  }
+  zd ph = conj(v[kmax]) * (1.0 / std::sqrt(best));
+  for (int i = 0; i < n; ++i) v[i] = v[i] * ph;
+  v[kmax].im = 0.0;
+  return v;
+}
+
+bool host_dgesv(std::vector<double>& A, int n, std::vector<double>& b) {
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = std::fabs(A[k + n * k]);
+    for (int r = k + 1; r < n; ++r)
+      if (std::fabs(A[r + n * k]) > best) { best = std::fabs(A[r + n * k]); p = r; }
+    if (best == 0.0) return false;
+    if (p != k) {
+      for (int c = 0; c < n; ++c) std::swap(A[k + n * c], A[p + n * c]);
+      std::swap(b[k], b[p]);
+    }
+    for (int r = k + 1; r < n; ++r) {
+      double l = A[r + n * k] / A[k + n * k];
+      for (int c = k + 1; c < n; ++c) A[r + n * c] -= l * A[k + n * c];
+      b[r] -= l * b[k];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    double x = b[k];
+    for (int j = k + 1; j < n; ++j) x -= A[k + n * j] * b[j];
+    b[k] = x / A[k + n * k];
+  }
+  return true;
+}
+
+}  // namespace gmod

@@ -1,0 +1,463 @@
+// Library runtime: device context, caching pool, sticky error channel,
+// hipEvent kernel profiling, and the host-side setup entry points of the
+// reference (k-mesh, H(k), band eigenvectors, interaction vertices).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+
+#include "../../include/gmod.h"
+#include "gmod_common.h"
+
+namespace gmod {
+
+// ------------------------------------------------------------------ context
+Context& ctx() {
+  static Context* c = [] {
+    auto* cc = new Context();
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+      throw Error("no HIP device: libgmod needs an AMD GPU (gfx950)");
+    GMOD_HIP(hipSetDevice(0));
+    GMOD_HIP(hipStreamCreateWithFlags(&cc->stream, hipStreamNonBlocking));
+    if (const char* p = getenv("GMOD_PRECISION")) cc->precision = atoi(p);
+    if (const char* v = getenv("GMOD_VERBOSE")) cc->verbose = atoi(v);
+    return cc;
+  }();
+  return *c;
+}
+
+// ------------------------------------------------------------------ errors
+static std::mutex g_err_mu;
+static std::string g_err_msg;
+static int g_err_count = 0;
+
+void set_error(const std::string& msg) {
+  std::lock_guard<std::mutex> g(g_err_mu);
+  g_err_msg = msg;
+  ++g_err_count;
+  fprintf(stderr, "[gmod] ERROR: %s\n", msg.c_str());
+  fflush(stderr);
+  if (getenv("GMOD_ABORT_ON_ERROR")) abort();
+}
+void clear_error() {
+  std::lock_guard<std::mutex> g(g_err_mu);
+  g_err_count = 0;
+  g_err_msg.clear();
+}
+
+// ------------------------------------------------------------------ pool
+void* Pool::get(size_t bytes) {
+  std::lock_guard<std::mutex> g(mu_);
+  size_t want = (bytes + 255) & ~size_t(255);
+  int best = -1;
+  for (size_t i = 0; i < blocks_.size(); ++i) {
+    auto& b = blocks_[i];
+    if (!b.used && b.bytes >= want && b.bytes <= 2 * want + (1 << 20)) {
+      if (best < 0 || b.bytes < blocks_[best].bytes) best = (int)i;
+    }
+  }
+  if (best >= 0) {
+    blocks_[best].used = true;
+    return blocks_[best].p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    for (auto it = blocks_.begin(); it != blocks_.end();) {
+      if (!it->used) { (void)hipFree(it->p); it = blocks_.erase(it); }
+      else ++it;
+    }
+    e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error("device allocation of " + std::to_string(want) + " bytes failed");
+    }
+  }
+  blocks_.push_back({p, want, true});
+  return p;
+}
+void Pool::put(void* p) {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& b : blocks_)
+    if (b.p == p) { b.used = false; return; }
+}
+void Pool::trim() {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto it = blocks_.begin(); it != blocks_.end();) {
+    if (!it->used) { (void)hipFree(it->p); it = blocks_.erase(it); }
+    else ++it;
+  }
+}
+Pool::~Pool() {}
+Pool& pool() {
+  static Pool* p = new Pool();
+  return *p;
+}
+static double pool_bytes() { return 0.0; }
+
+// ------------------------------------------------------------------ profiling
+struct ProfRec { int cls; double bytes; hipEvent_t a, b; };
+static bool g_prof = false;
+static std::vector<ProfRec> g_prof_recs;
+static double g_prof_acc[3 * PROF_NCLASS];
+
+bool prof_enabled() { return g_prof; }
+
+ProfScope::ProfScope(int c, double by, hipStream_t st) : cls(c), bytes(by), s(st) {
+  if (!g_prof) return;
+  GMOD_HIP(hipEventCreate(&e0));
+  GMOD_HIP(hipEventRecord(e0, s));
+}
+ProfScope::~ProfScope() {
+  if (!g_prof || !e0) return;
+  hipEvent_t e1;
+  if (hipEventCreate(&e1) != hipSuccess) return;
+  (void)hipEventRecord(e1, s);
+  g_prof_recs.push_back({cls, bytes, e0, e1});
+}
+
+static void prof_flush() {
+  if (g_prof_recs.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& r : g_prof_recs) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    g_prof_acc[3 * r.cls] += ms;
+    g_prof_acc[3 * r.cls + 1] += 1;
+    g_prof_acc[3 * r.cls + 2] += r.bytes;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  g_prof_recs.clear();
+}
+void prof_reset() {
+  prof_flush();
+  std::fill(g_prof_acc, g_prof_acc + 3 * PROF_NCLASS, 0.0);
+}
+void prof_read(double* out) {
+  prof_flush();
+  std::copy(g_prof_acc, g_prof_acc + 3 * PROF_NCLASS, out);
+}
+
+}  // namespace gmod
+
+using namespace gmod;
+
+// ====================================================================== C ABI: runtime
+extern "C" int gmod_last_error(char* buf, int len) {
+  std::lock_guard<std::mutex> g(g_err_mu);
+  if (buf && len > 0) {
+    strncpy(buf, g_err_msg.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  int c = g_err_count;
+  g_err_count = 0;
+  return c;
+}
+
+extern "C" void gmod_set_precision(int mode) {
+  try { ctx().precision = mode; } catch (std::exception& e) { set_error(e.what()); }
+}
+extern "C" int gmod_get_precision(void) {
+  try { return ctx().precision; } catch (std::exception& e) { set_error(e.what()); return -1; }
+}
+extern "C" void gmod_set_verbose(int level) {
+  try { ctx().verbose = level; } catch (std::exception& e) { set_error(e.what()); }
+}
+extern "C" void gmod_profile(int on) {
+  try {
+    ctx();
+    prof_reset();
+    g_prof = on != 0;
+  } catch (std::exception& e) { set_error(e.what()); }
+}
+extern "C" int gmod_profile_read(double* out, int n) {
+  double tmp[3 * PROF_NCLASS];
+  prof_read(tmp);
+  int m = std::min(n, 3 * (int)PROF_NCLASS);
+  std::copy(tmp, tmp + m, out);
+  return m;
+}
+extern "C" double gmod_pool_bytes(void) { return pool_bytes(); }
+extern "C" void gmod_release_memory(void) {
+  try { pool().trim(); } catch (std::exception& e) { set_error(e.what()); }
+}
+
+// ====================================================================== C ABI: setup (host)
+// These run on the host exactly like the reference's own setup routines: they
+// produce the k-mesh tables and band data once per run.
+
+// finv.f90:102-289: irreducible k-list order (integer coordinates)
+static std::vector<std::array<long, 3>> irr_klist(long Nx, long Ny, long Nz, long Nk) {
+  std::vector<std::array<long, 3>> kl(Nk + 1, {0, 0, 0});
+  auto put = [&](long idx, long i, long j, long k) {
+    if (idx < 1 || idx > Nk) throw Error("k-list index out of range (inconsistent Nk)");
+    kl[idx] = {i, j, k};
+  };
+  long ini;
+  if (Nx % 2 == 0 && Ny % 2 == 0) {
+    for (long j = 0; j <= Ny / 2; ++j)
+      for (long i = 0; i <= Nx / 2; ++i) put(i + 1 + (Nx / 2 + 1) * j, i, j, 0);
+    ini = (Nx / 2 + 1) * (Ny / 2 + 1);
+    for (long j = Ny / 2 + 1; j < Ny; ++j)
+      for (long i = 1; i < Nx / 2; ++i) put(ini + i + (Nx / 2 - 1) * (j - Ny / 2 - 1), i, j, 0);
+    ini = Nx * Ny / 2 + 3;
+  } else if ((Nx * Ny) % 2 == 0) {
+    if (Nx % 2 != 0) throw Error("odd Nx with even Ny is not supported by the reference k-mesh (finv.f90:154)");
+    long it = 1;
+    for (long j = 0; j <= (Ny - 1) / 2; ++j)
+      for (long i = 0; i <= Nx / 2; ++i) put(it++, i, j, 0);
+    for (long j = (Ny + 1) / 2; j < Ny; ++j)
+      for (long i = 1; i < Nx / 2; ++i) put(it++, i, j, 0);
+    ini = it;
+  } else {
+    for (long j = 0; j <= (Ny - 1) / 2; ++j)
+      for (long i = 0; i <= (Nx - 1) / 2; ++i) put(i + 1 + ((Nx + 1) / 2) * j, i, j, 0);
+    ini = (Nx + 1) * (Ny + 1) / 4;
+    for (long j = (Ny + 1) / 2; j < Ny; ++j)
+      for (long i = 1; i <= (Nx - 1) / 2; ++i) put(ini + i + ((Nx - 1) / 2) * (j - (Ny + 1) / 2), i, j, 0);
+    ini = (Nx * Ny + 1) / 2 + 1;
+  }
+  if (Nz > 1) {
+    if (Nz % 2 == 0) {
+      for (long k = 1; k < Nz / 2; ++k)
+        for (long j = 0; j < Ny; ++j)
+          for (long i = 0; i < Nx; ++i) put(ini + i + Nx * j + Nx * Ny * (k - 1), i, j, k);
+      ini += Nx * Ny * (Nz / 2 - 1);
+      const long h = Nz / 2;
+      if (Nx % 2 == 0 && Ny % 2 == 0) {
+        for (long j = 0; j <= Ny / 2; ++j)
+          for (long i = 0; i <= Nx / 2; ++i) put(ini + i + (Nx / 2 + 1) * j, i, j, h);
+        ini += (Nx / 2 + 1) * (Ny / 2 + 1) - 1;
+        for (long j = Ny / 2 + 1; j < Ny; ++j)
+          for (long i = 1; i < Nx / 2; ++i) put(ini + i + (Nx / 2 - 1) * (j - Ny / 2 - 1), i, j, h);
+      } else {
+        long it = ini;
+        if ((Nx * Ny) % 2 == 0) {
+          for (long j = 0; j <= (Ny - 1) / 2; ++j)
+            for (long i = 0; i <= Nx / 2; ++i) put(it++, i, j, h);
+          for (long j = (Ny + 1) / 2; j < Ny; ++j)
+            for (long i = 1; i < Nx / 2; ++i) put(it++, i, j, h);
+        } else {
+          for (long j = 0; j <= (Ny - 1) / 2; ++j)
+            for (long i = 0; i <= (Nx - 1) / 2; ++i) put(it++, i, j, h);
+          for (long j = (Ny + 1) / 2; j < Ny; ++j)
+            for (long i = 1; i <= (Nx - 1) / 2; ++i) put(it++, i, j, h);
+        }
+      }
+    } else {
+      for (long k = 1; k <= (Nz - 1) / 2; ++k)
+        for (long j = 0; j < Ny; ++j)
+          for (long i = 0; i < Nx; ++i) put(ini + i + Nx * j + Nx * Ny * (k - 1), i, j, k);
+    }
+  }
+  return kl;
+}
+
+extern "C" void generate_irr_kpoint_inv(double* klist, int64_t* kmap, int64_t* invk, const int64_t* Nk_,
+                                        const int64_t* Nx_, const int64_t* Ny_, const int64_t* Nz_) {
+  try {
+    const long Nk = *Nk_, Nx = *Nx_, Ny = *Ny_, Nz = *Nz_;
+    const long Nkall = Nx * Ny * Nz;
+    auto kl = irr_klist(Nx, Ny, Nz, Nk);
+    for (long j = 1; j <= Nk; ++j) {
+      klist[3 * (j - 1) + 0] = (double)kl[j][0] / (double)Nx;
+      klist[3 * (j - 1) + 1] = (double)kl[j][1] / (double)Ny;
+      klist[3 * (j - 1) + 2] = (double)kl[j][2] / (double)Nz;
+    }
+    // first irreducible j (list order) matching k directly or as -k (finv.f90:28-53)
+    std::vector<long> first(Nkall, -1);
+    for (long j = Nk; j >= 1; --j) {
+      long c = kl[j][0] + Nx * (kl[j][1] + Ny * kl[j][2]);
+      first[c] = j;
+    }
+    for (long i = 0; i < Nkall; ++i) {
+      long x = i % Nx, y = (i / Nx) % Ny, z = i / (Nx * Ny);
+      kmap[3 * i] = x; kmap[3 * i + 1] = y; kmap[3 * i + 2] = z;
+      long mx = (Nx - x) % Nx, my = (Ny - y) % Ny, mz = (Nz - z) % Nz;
+      long ci = i, cm = mx + Nx * (my + Ny * mz);
+      long jd = first[ci], jn = first[cm];
+      long j;
+      int f;
+      if (jd > 0 && (jn < 0 || jd <= jn)) { j = jd; f = 0; }
+      else if (jn > 0) { j = jn; f = 1; }
+      else { j = 0; f = 0; }
+      invk[3 * i] = j;
+      invk[3 * i + 1] = f;
+      invk[3 * i + 2] = cm + 1;
+    }
+  } catch (std::exception& e) { set_error(e.what()); }
+}
+
+extern "C" void get_kweight(double* w, const int64_t* invk, const int64_t* Nk, const int64_t* Nkall) {
+  for (long i = 0; i < *Nk; ++i) w[i] = 0;
+  for (long i = 0; i < *Nkall; ++i) {
+    long k = invk[3 * i] - 1;
+    if (k >= 0 && k < *Nk) w[k] += 1.0;
+  }
+}
+
+extern "C" void gen_ham(gmod_z* ham_k, const double* klist, const gmod_z* ham_r, const double* rvec,
+                        const int64_t* Nk_, const int64_t* Nr_, const int64_t* Norb_) {
+  const long Nk = *Nk_, Nr = *Nr_, No = *Norb_;
+  const double pi = 3.141592653589793238462643383279;
+  for (long i = 0; i < Nk; ++i) {
+    gmod_z* H = ham_k + No * No * i;
+    for (long e = 0; e < No * No; ++e) H[e] = {0, 0};
+    for (long j = 0; j < Nr; ++j) {
+      double ph = 2 * pi * (klist[3 * i] * rvec[3 * j] + klist[3 * i + 1] * rvec[3 * j + 1] +
+                            klist[3 * i + 2] * rvec[3 * j + 2]);
+      double c = std::cos(ph), s = -std::sin(ph);
+      for (long l = 0; l < No; ++l)
+        for (long m = l; m < No; ++m) {
+          gmod_z h = ham_r[m + No * (l + No * j)];
+          H[m + No * l].re += h.re * c - h.im * s;
+          H[m + No * l].im += h.re * s + h.im * c;
+        }
+    }
+    for (long l = 0; l < No; ++l) {
+      H[l + No * l].im = 0;
+      for (long m = l + 1; m < No; ++m) H[l + No * m] = {H[m + No * l].re, -H[m + No * l].im};
+    }
+  }
+}
+
+// cyclic complex Jacobi for one Hermitian matrix (eigenvalues ascending,
+// eigenvectors in columns) -- same mathematical output as zheev('V','U')
+// (eigenvector phases may differ; every hot-path consumer uses phase-free
+// combinations U_an conj(U_bn)).
+static void herm_jacobi(std::vector<std::complex<double>>& A, int n, std::vector<double>& w,
+                        std::vector<std::complex<double>>& V) {
+  using C = std::complex<double>;
+  V.assign(n * n, C(0));
+  for (int i = 0; i < n; ++i) V[i + n * i] = 1;
+  // use the upper triangle, as zheev('U')
+  for (int j = 0; j < n; ++j)
+    for (int i = j + 1; i < n; ++i) A[i + n * j] = std::conj(A[j + n * i]);
+  for (int i = 0; i < n; ++i) A[i + n * i] = A[i + n * i].real();
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0, tot = 0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        double a = std::norm(A[i + n * j]);
+        tot += a;
+        if (i != j) off += a;
+      }
+    if (off <= 1e-32 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        C apq = A[p + n * q];
+        double mag = std::abs(apq);
+        if (mag < 1e-300) continue;
+        double app = A[p + n * p].real(), aqq = A[q + n * q].real();
+        C ph = apq / mag;  // A_pq = |A_pq| e^{i phi}
+        double theta = 0.5 * std::atan2(2 * mag, aqq - app);
+        double c = std::cos(theta), s = std::sin(theta);
+        // rotation J: columns p,q of identity replaced by [c, -s conj(ph)... ]
+        // J = [[c, s*ph], [-s*conj(ph), c]] acting on (p,q)
+        for (int k = 0; k < n; ++k) {  // A <- A J
+          C akp = A[k + n * p], akq = A[k + n * q];
+          A[k + n * p] = c * akp - s * std::conj(ph) * akq;
+          A[k + n * q] = s * ph * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // A <- J^H A
+          C apk = A[p + n * k], aqk = A[q + n * k];
+          A[p + n * k] = c * apk - s * ph * aqk;
+          A[q + n * k] = s * std::conj(ph) * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          C vkp = V[k + n * p], vkq = V[k + n * q];
+          V[k + n * p] = c * vkp - s * std::conj(ph) * vkq;
+          V[k + n * q] = s * ph * vkp + c * vkq;
+        }
+        A[p + n * q] = A[q + n * p] = 0;
+      }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::vector<double> d(n);
+  for (int i = 0; i < n; ++i) d[i] = A[i + n * i].real();
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+  w.resize(n);
+  std::vector<C> V2(n * n);
+  for (int c = 0; c < n; ++c) {
+    w[c] = d[idx[c]];
+    for (int r = 0; r < n; ++r) V2[r + n * c] = V[r + n * idx[c]];
+  }
+  V = V2;
+}
+
+extern "C" void get_eig(double* eig, gmod_z* uni, const gmod_z* ham_k, const int64_t* Nk_, const int64_t* Norb_) {
+  try {
+    const long Nk = *Nk_;
+    const int No = (int)*Norb_;
+    std::vector<std::complex<double>> A(No * No), V;
+    std::vector<double> w;
+    for (long i = 0; i < Nk; ++i) {
+      for (int e = 0; e < No * No; ++e) A[e] = {ham_k[No * No * i + e].re, ham_k[No * No * i + e].im};
+      herm_jacobi(A, No, w, V);
+      for (int n = 0; n < No; ++n) eig[No * i + n] = w[n];
+      for (int e = 0; e < No * No; ++e) uni[No * No * i + e] = {V[e].real(), V[e].imag()};
+    }
+  } catch (std::exception& e) { set_error(e.what()); }
+}
+
+extern "C" void get_ffermi(double* ffermi, const double* eig, const double* mu, const double* temp,
+                           const int64_t* Nk, const int64_t* Norb) {
+  double ts = std::max(*temp, 1.0e-12);
+  double it = 0.5 / ts;
+  for (long i = 0; i < (*Nk) * (*Norb); ++i) ffermi[i] = 0.5 - 0.5 * std::tanh((eig[i] - *mu) * it);
+}
+
+extern "C" void openmp_params(int64_t* omp_num, bool* omp_check) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *omp_num = n;
+  *omp_check = n > 0;
+}
+
+// fUJ.f90:1-57
+extern "C" void get_scmat(double* S, double* C, const int64_t* ol, const int64_t* site, const double* U,
+                          const double* J, const int64_t* Nchi_) {
+  const long n = *Nchi_;
+  const double Uv = *U, Jv = *J, Up = Uv - 2 * Jv;
+  for (long e = 0; e < n * n; ++e) S[e] = C[e] = 0;
+  for (long i = 0; i < n; ++i)
+    for (long j = 0; j < n; ++j) {
+      if (site[i] != site[j]) continue;
+      long a = ol[i], b = ol[i + n], c = ol[j], d = ol[j + n];
+      double s = 0, cc = 0;
+      bool set = true;
+      if (a == b && c == d) {
+        if (a == c) { s = Uv; cc = Uv; }
+        else { s = Jv; cc = 2 * Up - Jv; }
+      } else if (a == c && b == d) { s = Up; cc = -Up + 2 * Jv; }
+      else if (a == d && b == c) { s = Jv; cc = Jv; }
+      else set = false;
+      if (set) { S[j + n * i] = s; C[j + n * i] = cc; }
+    }
+}
+
+// fUJ.f90:59-107
+extern "C" void get_scmat_orb(double* S, double* C, const int64_t* ol, const int64_t* site, const double* Um,
+                              const double* Jm, const int64_t* Nchi_, const int64_t* Norb_) {
+  const long n = *Nchi_, No = *Norb_;
+  for (long e = 0; e < n * n; ++e) S[e] = C[e] = 0;
+  auto U = [&](long p, long q) { return Um[(p - 1) + No * (q - 1)]; };
+  auto Jf = [&](long p, long q) { return Jm[(p - 1) + No * (q - 1)]; };
+  for (long i = 0; i < n; ++i)
+    for (long j = 0; j < n; ++j) {
+      if (site[i] != site[j]) continue;
+      long a = ol[i], b = ol[i + n], c = ol[j], d = ol[j + n];
+      if (a == b && c == d) {
+        if (a == c) { S[j + n * i] = C[j + n * i] = U(a, a); }
+        else { S[j + n * i] = Jf(a, c); C[j + n * i] = 2 * U(a, c) - Jf(a, c); }
+      } else if (a == c && b == d) {
+        S[j + n * i] = U(a, b);
+        C[j + n * i] = -U(a, b) + 2 * Jf(a, b);
+      } else if (a == d && b == c) {
+        S[j + n * i] = C[j + n * i] = Jf(a, b);
+      }
+    }
+}

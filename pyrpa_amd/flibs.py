@@ -1,0 +1,414 @@
+"""ctypes host layer of libgmod.so, mirroring the reference's libs/flibs wrappers.
+
+Each function keeps the reference wrapper's name, argument order, defaults,
+returned shapes and C-order array layouts (Python shapes are the reversed
+Fortran shapes), so a caller of ``libs.flibs`` can switch by changing the
+import.  Differences: errors raise ``RuntimeError`` instead of terminating the
+process (the reference prints and STOPs), and the GPU library is required.
+"""
+from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int32, c_int64
+
+import numpy as np
+
+from ._loader import check, dbl, i64, lib
+
+_nd = np.ctypeslib.ndpointer
+_z = _nd(dtype=np.complex128)
+_d = _nd(dtype=np.float64)
+_i = _nd(dtype=np.int64)
+_i32 = _nd(dtype=np.int32)
+_pi = POINTER(c_int64)
+_pd = POINTER(c_double)
+_pb = POINTER(c_bool)
+
+_SIGS = {
+    # libs/flibs/_selfenergy.py:7-37
+    "get_vsigma_flex_nosoc_": [_z, _d, _d, _pi, _pi, _pi],
+    "mkself": [_z, _pd, _d, _d, _i, _i, _i, _z, _d, _z, _pd, _pd, _pd, _pi, _pd, _pd, _pi, _pi, _pi, _pi,
+               _pi, _pi, _pi, _pi, _pi, _pb, _pb, _pi, _pb, _pb, _pd],
+    # libs/flibs/_susceptibility.py:7-204
+    "get_chi0_conv_": [_z, _z, _i, _i, _i32, _i32, _i, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi],
+    "get_chi0": [_z, _d, _d, _z, _i, _i, _i, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pd],
+    "get_chi0_tail": [_z, _d, _d, _z, _d, _z, _i, _i, _i, _pd, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi,
+                      _pd],
+    "get_chis": [_z, _z, _d, _pi, _pi],
+    "get_chis_chic": [_z, _z, _z, _d, _d, _pi, _pi, _pi],
+    "get_eig_or_tr_chi": [_z, _z, _i, _pi, _pi, _pi, _pb],
+    # libs/flibs/_green.py:7-27
+    "gen_green0_": [_z, _d, _z, _pd, _pd, _pi, _pi, _pi],
+    "gen_green_inv_": [_z, _z, _z, _pd, _pd, _pi, _pi, _pi],
+    "getinv_": [_z, _pi, _pi, _pi],
+    # libs/flibs/_eliashberg.py:7-143
+    "lin_eliash": [_z, _z, _z, _z, _d, _d, _d, _i, _d, _i, _i, _pd, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi,
+                   _pi, _pi, _pi, _pi, _pd],
+    "mkfk_trs_nsoc_": [_z, _z, _z, _pi, _pi, _pi],
+    "get_initial_delta_": [_z, _d, _z, _i, _i, _pi, _pi, _pi, _pi, _pi],
+    "conv_delta_orb_to_band": [_z, _z, _z, _d, _i, _pi, _pi, _pi, _pi, _pi],
+    "remap_delta_": [_z, _z, _d, _i, _pi, _pi, _pi, _pi, _pi],
+    # libs/flibs/_interaction.py:7-24
+    "get_scmat": [_d, _d, _i, _i, _pd, _pd, _pi],
+    "get_scmat_orb": [_d, _d, _i, _i, _d, _d, _pi, _pi],
+    # libs/flibs/_hamiltonian.py:7-106
+    "openmp_params": [_pi, _pb],
+    "gen_ham": [_z, _d, _z, _d, _pi, _pi, _pi],
+    "get_eig": [_d, _z, _z, _pi, _pi],
+    "get_ffermi": [_d, _d, _pd, _pd, _pi, _pi],
+    "generate_irr_kpoint_inv": [_d, _i, _i, _pi, _pi, _pi, _pi],
+    "get_kweight": [_d, _i, _pi, _pi],
+}
+
+_bound = False
+
+
+def _L():
+    global _bound
+    L = lib()
+    if not _bound:
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
+        L.gmod_last_error.argtypes = [c_char_p, c_int]
+        L.gmod_last_error.restype = c_int
+        L.gmod_set_precision.argtypes = [c_int]
+        L.gmod_get_precision.restype = c_int
+        L.gmod_set_sigma_seed.argtypes = [_z, c_double, c_double, c_int64]
+        L.gmod_flex_stats.argtypes = [_d, c_int]
+        L.gmod_flex_stats.restype = c_int
+        L.gmod_profile.argtypes = [c_int]
+        L.gmod_profile_read.argtypes = [_d, c_int]
+        L.gmod_profile_read.restype = c_int
+        L.gmod_set_verbose.argtypes = [c_int]
+        L.gmod_pool_bytes.restype = c_double
+        _bound = True
+    return L
+
+
+def _call(name, *args):
+    getattr(_L(), name)(*args)
+    check()
+
+
+# =========================================================================== hamiltonian / k-mesh
+def omp_params():
+    """(device count, True if a HIP device is present)  [_hamiltonian.py:122-132]"""
+    n = c_int64()
+    ok = c_bool()
+    _call("openmp_params", byref(n), byref(ok))
+    return n.value, ok.value
+
+
+def gen_ham(klist, ham_r, rvec, Ovl_r=None):
+    """H(k) = sum_R H(R) exp(-2 pi i k.R)  [_hamiltonian.py:134-158]"""
+    if Ovl_r is not None:
+        raise NotImplementedError("overlap matrices (MLO bases) are outside the FLEX/Eliashberg path")
+    klist = np.ascontiguousarray(klist, dtype=np.float64)
+    rvec = np.ascontiguousarray(rvec, dtype=np.float64)
+    ham_r = np.ascontiguousarray(ham_r, dtype=np.complex128)
+    Nk, Nr = len(klist), len(rvec)
+    Norb = ham_r.shape[1]
+    hamk = np.zeros((Nk, Norb, Norb), dtype=np.complex128)
+    _call("gen_ham", hamk, klist, ham_r, rvec, i64(Nk), i64(Nr), i64(Norb))
+    return hamk
+
+
+def get_eig(hamk, Ovlk=None, sw=True):
+    """eig[Nk,No], uni[Nk,No,No]  [_hamiltonian.py:160-184]"""
+    if Ovlk is not None:
+        raise NotImplementedError("generalised eigenproblem is outside the FLEX/Eliashberg path")
+    hamk = np.ascontiguousarray(hamk, dtype=np.complex128)
+    Nk, Norb = len(hamk), hamk.shape[1]
+    eig = np.zeros((Nk, Norb), dtype=np.float64)
+    uni = np.zeros((Nk, Norb, Norb), dtype=np.complex128)
+    _call("get_eig", eig, uni, hamk, i64(Nk), i64(Norb))
+    return (eig, uni) if sw else eig
+
+
+def get_ffermi(eig, mu, temp):
+    """Fermi occupations  [_hamiltonian.py:186-200]"""
+    eig = np.ascontiguousarray(eig, dtype=np.float64)
+    Nk, Norb = eig.shape
+    f = np.zeros((Nk, Norb), dtype=np.float64)
+    _call("get_ffermi", f, eig, dbl(mu), dbl(temp), i64(Nk), i64(Norb))
+    return f
+
+
+def gen_irr_k_TRS(Nx, Ny, Nz):
+    """klist[Nk,3], kmap[Nkall,3], invk[Nkall,3]  [_hamiltonian.py:356-382]"""
+    Nkall = Nx * Ny * Nz
+    if Nkall % 2 == 0:
+        if Nz % 2 == 0:
+            Nk = int(Nkall / 2 + 4) if ((Nx % 2) == 0 and (Ny % 2) == 0) else (
+                int(Nkall / 2 + 2) if ((Nx % 2) == 0 or (Ny % 2) == 0) else int(Nkall / 2 + 1))
+        else:
+            Nk = int(Nkall / 2 + 2) if ((Nx % 2) == 0 and (Ny % 2) == 0) else int(Nkall / 2 + 1)
+    else:
+        Nk = int((Nkall + 1) / 2)
+    klist = np.zeros((Nk, 3), dtype=np.float64)
+    kmap = np.zeros((Nkall, 3), dtype=np.int64)
+    invk = np.zeros((Nkall, 3), dtype=np.int64)
+    _call("generate_irr_kpoint_inv", klist, kmap, invk, i64(Nk), i64(Nx), i64(Ny), i64(Nz))
+    return klist, kmap, invk
+
+
+def gen_kpoint_weight(invk, Nk):
+    """[_hamiltonian.py:384-395]"""
+    w = np.zeros(Nk, dtype=np.float64)
+    _call("get_kweight", w, np.ascontiguousarray(invk, dtype=np.int64), i64(Nk), i64(len(invk)))
+    return w
+
+
+# =========================================================================== interaction
+def gen_SCmatrix(olist, site, U, J):
+    """[_interaction.py:47-62]"""
+    Nchi = len(olist)
+    S = np.zeros((Nchi, Nchi), dtype=np.float64)
+    C = np.zeros((Nchi, Nchi), dtype=np.float64)
+    _call("get_scmat", S, C, olist, np.asarray(site, dtype=np.int64), dbl(U), dbl(J), i64(Nchi))
+    return S, C
+
+
+def gen_SCmatrix_orb(olist, site, Umat, Jmat):
+    """[_interaction.py:64-81]"""
+    Nchi, Norb = len(olist), len(Umat)
+    S = np.zeros((Nchi, Nchi), dtype=np.float64)
+    C = np.zeros((Nchi, Nchi), dtype=np.float64)
+    _call("get_scmat_orb", S, C, olist, np.asarray(site, dtype=np.int64), np.asarray(Umat, dtype=np.float64),
+          np.asarray(Jmat, dtype=np.float64), i64(Nchi), i64(Norb))
+    return S, C
+
+
+# =========================================================================== Green functions
+def gen_Green0(eig, uni, mu, temp, Nw):
+    """G0[No,No,Nw,Nk]  [_green.py:80-97]"""
+    Nk, Norb = np.shape(eig)
+    Gk = np.zeros((Norb, Norb, Nw, Nk), dtype=np.complex128)
+    _call("gen_green0_", Gk, np.ascontiguousarray(eig), np.ascontiguousarray(uni), dbl(mu), dbl(temp), i64(Nk),
+          i64(Nw), i64(Norb))
+    return Gk
+
+
+def gen_green(selfen, hamk, mu, temp):
+    """G = [i w + mu - H - Sigma]^-1  [_green.py:99-117]"""
+    Nk, Norb = len(hamk), hamk.shape[1]
+    Nw = selfen.shape[2]
+    Gk = np.zeros((Norb, Norb, Nw, Nk), dtype=np.complex128)
+    _call("gen_green_inv_", Gk, np.ascontiguousarray(selfen), np.ascontiguousarray(hamk), dbl(mu), dbl(temp),
+          i64(Nk), i64(Nw), i64(Norb))
+    _call("getinv_", Gk, i64(Nk), i64(Nw), i64(Norb))
+    return Gk
+
+
+# =========================================================================== susceptibility
+def _get_chi_map(olist):
+    """chi_map (2,Nchi,Nchi) and irr_chi (2,P) int32 with the memory layout of
+    the Fortran chi_map(Nchi,Nchi,2), irr_chi(P,2)  [_susceptibility.py:206-242,
+    algorithm of fchi.f90:421-481]"""
+    ol = np.asarray(olist, dtype=np.int64)
+    Nchi = len(ol)
+    cmap = np.zeros((2, Nchi, Nchi), dtype=np.int32)
+    irr_flag = np.zeros((Nchi, Nchi), dtype=np.int32)
+    for l1 in range(Nchi):
+        for m1 in range(Nchi):
+            key = (ol[l1, 0], ol[l1, 1], ol[m1, 0], ol[m1, 1])
+            hit = None
+            for l2 in range(Nchi):
+                for m2 in range(Nchi):
+                    if key == (ol[m2, 1], ol[m2, 0], ol[l2, 1], ol[l2, 0]):
+                        hit = (l2, m2)
+                        break
+                if hit:
+                    break
+            if hit:
+                l2, m2 = hit
+                cmap[0, l1, m1] = m2 + 1
+                cmap[1, l1, m1] = l2 + 1
+                if irr_flag[l2, m2] == 0:
+                    irr_flag[l1, m1] = 1
+    irr = np.zeros((2, Nchi * (Nchi + 1) // 2), dtype=np.int32)
+    it = 0
+    for l1 in range(Nchi):
+        for m1 in range(Nchi):
+            if irr_flag[l1, m1] == 1 and it < irr.shape[1]:
+                irr[0, it], irr[1, it] = l1 + 1, m1 + 1
+                it += 1
+    return cmap, irr
+
+
+def get_chi0_conv(Gk, kmap, invk, olist, temp, Nx, Ny, Nz):
+    """chi0 by FFT convolution, sharp cutoff  [_susceptibility.py:244-269]"""
+    Nkall, Nk = len(kmap), Gk.shape[3]
+    Norb, Nchi, Nw = len(Gk), len(olist), Gk.shape[2]
+    chi = np.zeros((Nchi, Nchi, Nw, Nk), dtype=np.complex128)
+    chi_map, irr_chi = _get_chi_map(olist)
+    _call("get_chi0_conv_", chi, np.ascontiguousarray(Gk), kmap, invk, irr_chi, chi_map, olist, dbl(temp),
+          i64(Nx), i64(Ny), i64(Nz), i64(Nw), i64(Nk), i64(Nkall), i64(Norb), i64(Nchi))
+    return chi
+
+
+def get_chi0(Smat, Cmat, Gk, olist, kmap, invk, temp, Nx, Ny, Nz):
+    """(chi0[Nchi,Nchi,Nw,Nk], Stoner factor)  [_susceptibility.py:432-458]"""
+    Norb, Nchi = len(Gk), len(olist)
+    Nkall, Nk, Nw = len(kmap), Gk.shape[3], Gk.shape[2]
+    chi = np.zeros((Nchi, Nchi, Nw, Nk), dtype=np.complex128)
+    st = c_double(0.0)
+    _call("get_chi0", chi, Smat, Cmat, np.ascontiguousarray(Gk), kmap, invk, olist, dbl(temp), i64(Nx), i64(Ny),
+          i64(Nz), i64(Nw), i64(Nk), i64(Nkall), i64(Norb), i64(Nchi), byref(st))
+    return chi, st.value
+
+
+def get_chi0_tail(Smat, Cmat, Gk, eig, uni, olist, kmap, invk, mu, temp, Nx, Ny, Nz):
+    """tail-corrected chi0  [_susceptibility.py:460-494]"""
+    Norb, Nchi = len(Gk), len(olist)
+    Nkall, Nk, Nw = len(kmap), Gk.shape[3], Gk.shape[2]
+    chi = np.zeros((Nchi, Nchi, Nw, Nk), dtype=np.complex128)
+    st = c_double(0.0)
+    _call("get_chi0_tail", chi, Smat, Cmat, np.ascontiguousarray(Gk), np.ascontiguousarray(eig),
+          np.ascontiguousarray(uni), kmap, invk, olist, dbl(mu), dbl(temp), i64(Nx), i64(Ny), i64(Nz), i64(Nw),
+          i64(Nk), i64(Nkall), i64(Norb), i64(Nchi), byref(st))
+    return chi, st.value
+
+
+def get_chis(chi0, Smat):
+    """[_susceptibility.py:419-430]"""
+    Nchi, Nw = len(Smat), len(chi0)
+    chis = np.zeros((Nw, Nchi, Nchi), dtype=np.complex128)
+    _call("get_chis", chis, np.ascontiguousarray(chi0), Smat, i64(Nchi), i64(Nw))
+    return chis
+
+
+def get_chis_chic(chi, Smat, Cmat):
+    """static RPA chi_s, chi_c [Nchi,Nchi,Nk]  [_susceptibility.py:568-583]"""
+    Nk, Nw, Nchi = chi.shape[3], chi.shape[2], len(Smat)
+    chis = np.zeros((Nchi, Nchi, Nk), dtype=np.complex128)
+    chic = np.zeros((Nchi, Nchi, Nk), dtype=np.complex128)
+    _call("get_chis_chic", chis, chic, np.ascontiguousarray(chi), Smat, Cmat, i64(Nk), i64(Nw), i64(Nchi))
+    return chis, chic
+
+
+def get_eig_or_tr_chi(chi, invk, sw_eig):
+    """[_susceptibility.py:615-629]"""
+    Nkall, Nk = len(invk), chi.shape[2]
+    Nchi = chi.shape[0]
+    chiq = np.zeros(Nkall, dtype=np.complex128)
+    _call("get_eig_or_tr_chi", chiq, np.ascontiguousarray(chi), invk, i64(Nkall), i64(Nk), i64(Nchi),
+          byref(c_bool(sw_eig)))
+    return chiq
+
+
+# =========================================================================== self-energy
+def get_Vsigma_nosoc_flex(chi, Smat, Cmat):
+    """V_sigma (chi modified in place, copy returned)  [_selfenergy.py:62-74]"""
+    Nk, Nw, Nchi = chi.shape[3], chi.shape[2], len(Smat)
+    _call("get_vsigma_flex_nosoc_", chi, Smat, Cmat, i64(Nk), i64(Nw), i64(Nchi))
+    return chi.copy()
+
+
+def mkself(Smat, Cmat, kmap, invk, olist, hamk, eig, uni, mu, fill, temp, Nw, Nx, Ny, Nz, sw_out, sw_in,
+           sub_sigma=1, scf_loop=300, eps=1.0e-4, pp=0.3, m_diis=5, sw_rescale=False, sw_tail=False,
+           sigma_scale=1.0):
+    """FLEX self-consistency loop  [_selfenergy.py:76-127]"""
+    Nkall, Nk, Nchi = len(kmap), len(hamk), len(Smat)
+    Norb = hamk.shape[1]
+    mu_self = c_double()
+    sigmak = np.zeros((Norb, Norb, Nw, Nk), dtype=np.complex128)
+    _call("mkself", sigmak, byref(mu_self), Smat, Cmat, kmap, invk, olist, np.ascontiguousarray(hamk),
+          np.ascontiguousarray(eig), np.ascontiguousarray(uni), dbl(mu), dbl(fill), dbl(temp), i64(scf_loop),
+          dbl(pp), dbl(eps), i64(Nkall), i64(Nk), i64(Nw), i64(Norb), i64(Nchi), i64(Nx), i64(Ny), i64(Nz),
+          i64(sub_sigma), byref(c_bool(sw_out)), byref(c_bool(sw_in)), i64(m_diis), byref(c_bool(sw_rescale)),
+          byref(c_bool(sw_tail)), dbl(sigma_scale))
+    return sigmak, mu_self.value
+
+
+# =========================================================================== Eliashberg
+def linearized_eliashberg(chi, Gk, uni, init_delta, Smat, Cmat, olist, plist, kmap, invk, Nx, Ny, Nz, temp,
+                          gap_sym, eps=1.0e-5, itemax=300, arnoldi_m=10):
+    """(delta[No,No,Nw,Nk], lambda)  [_eliashberg.py:145-181]; chi is overwritten with V_Delta"""
+    Norb, Nchi = len(Gk), len(Smat)
+    Nkall, Nk, Nw = len(kmap), Gk.shape[3], Gk.shape[2]
+    delta = np.zeros((Norb, Norb, Nw, Nk), dtype=np.complex128)
+    lam = c_double(0.0)
+    _call("lin_eliash", delta, chi, np.ascontiguousarray(Gk), np.ascontiguousarray(uni),
+          np.ascontiguousarray(init_delta, dtype=np.float64), Smat, Cmat, olist,
+          np.asarray(plist, dtype=np.float64), kmap, invk, dbl(temp), dbl(eps), i64(Nkall), i64(Nk), i64(Nw),
+          i64(Nchi), i64(Norb), i64(Nx), i64(Ny), i64(Nz), i64(itemax), i64(gap_sym), i64(arnoldi_m), byref(lam))
+    return delta, lam.value
+
+
+def gen_Fk(Gk, delta, invk):
+    """F = -G Delta G^dagger  [_eliashberg.py:343-355]"""
+    Nk, Nw, Norb = Gk.shape[3], delta.shape[2], len(delta)
+    Fk = np.zeros((Norb, Norb, Nw, Nk), dtype=np.complex128)
+    _call("mkfk_trs_nsoc_", Fk, np.ascontiguousarray(Gk), np.ascontiguousarray(delta), i64(Nk), i64(Nw), i64(Norb))
+    return Fk
+
+
+def get_initial_delta(init_delta, uni, kmap, invk, Nw, gap_sym):
+    """[_eliashberg.py:378-399]"""
+    Norb, Nk = np.shape(init_delta)
+    delta = np.zeros((Norb, Norb, Nw, Nk), dtype=np.complex128)
+    _call("get_initial_delta_", delta, np.ascontiguousarray(init_delta, dtype=np.float64),
+          np.ascontiguousarray(uni), kmap, invk, i64(len(kmap)), i64(Nk), i64(Nw), i64(Norb), i64(gap_sym))
+    return delta
+
+
+def conv_delta_orb_to_band(delta, uni, invk, plist, gap_sym):
+    """[_eliashberg.py:306-323]"""
+    Nkall, Nk, Nw, Norb = len(invk), len(uni), delta.shape[2], len(delta)
+    deltab = np.zeros((Norb, Norb, Nkall), dtype=np.complex128)
+    _call("conv_delta_orb_to_band", deltab, np.ascontiguousarray(delta), np.ascontiguousarray(uni),
+          np.asarray(plist, dtype=np.float64), invk, i64(Norb), i64(Nkall), i64(Nk), i64(Nw), i64(gap_sym))
+    return deltab
+
+
+def remap_gap(delta0, plist, invk, gap_sym):
+    """[_eliashberg.py:426-441]"""
+    Nkall, Nk, Norb = len(invk), delta0.shape[-1], len(plist)
+    Nw = delta0.shape[2]
+    delta = np.zeros((Norb, Norb, Nw, Nkall), dtype=np.complex128)
+    _call("remap_delta_", delta, np.ascontiguousarray(delta0), np.asarray(plist, dtype=np.float64), invk,
+          i64(Nkall), i64(Nk), i64(Nw), i64(Norb), i64(gap_sym))
+    return delta
+
+
+# =========================================================================== additions
+def set_precision(mode):
+    """0: FP64; 1: FP32 convolutions with FP64 judge (see include/gmod.h)."""
+    _L().gmod_set_precision(int(mode))
+    check()
+
+
+def get_precision():
+    return _L().gmod_get_precision()
+
+
+def set_sigma_seed(sigmak, mu, mu_old):
+    """In-memory warm start consumed by the next mkself(sw_in=True)."""
+    s = np.ascontiguousarray(sigmak, dtype=np.complex128)
+    _L().gmod_set_sigma_seed(s.ravel(), float(mu), float(mu_old), s.size)
+    check()
+
+
+def flex_stats():
+    out = np.zeros(9)
+    _L().gmod_flex_stats(out, 9)
+    keys = ("iterations", "esterr", "stoner", "mu", "seconds", "t_chi0", "t_vertex", "t_sigma", "t_mu_dyson")
+    return dict(zip(keys, out.tolist()))
+
+
+PROFILE_CLASSES = ("fft_pass", "rpa_vertex", "dyson", "scatter", "product", "gather")
+
+
+def profile(on=True):
+    _L().gmod_profile(1 if on else 0)
+
+
+def profile_read():
+    out = np.zeros(3 * len(PROFILE_CLASSES))
+    _L().gmod_profile_read(out, len(out))
+    return {c: dict(ms=out[3 * i], launches=int(out[3 * i + 1]), bytes=out[3 * i + 2])
+            for i, c in enumerate(PROFILE_CLASSES)}
+
+
+def set_verbose(level):
+    _L().gmod_set_verbose(int(level))
