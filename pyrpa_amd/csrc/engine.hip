@@ -161,13 +161,20 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
   for (auto& lm : op.irr) {
     const int l = lm[0], m = lm[1];
     const int a13 = op.ol[l][0], b13 = op.ol[m][0], a42 = op.ol[m][1], b42 = op.ol[l][1];
+    auto pr = op.partner[m + Nchi * l];
+    // A self-partner slice (partner == itself) is conjugated in place after each
+    // of the reference's three accumulation passes (fchi.f90:610, 846); the net
+    // result is conj(conv[G] + ref) - conv[G0].  It equals conj of the plain sum
+    // whenever TRS holds; keep the reference's order so results match it
+    // on any input.
+    const bool self_tail = tail && pr[0] == m && pr[1] == l;
     {
       ProfScope ps(PROF_PRODUCT, (double)N * sizeof(cpx<R>) * (tail ? 7 : 3), s);
       if (!tail) {
         product_rev<R>(work.p, grid(Fg, a13, b13), grid(Fg, a42, b42), (R)-1, true, false, mesh, s);
       } else {
         product_rev<R>(work.p, grid(Fg, a13, b13), grid(Fg, a42, b42), (R)(-w), true, false, mesh, s);
-        product_rev<R>(work.p, grid(F0g, a13, b13), grid(F0g, a42, b42), (R)(w), true, true, mesh, s);
+        if (!self_tail) product_rev<R>(work.p, grid(F0g, a13, b13), grid(F0g, a42, b42), (R)(w), true, true, mesh, s);
         product_rev<R>(work.p, grid(TA, a13, b13), grid(TB, a42, b42), (R)(-wref), false, true, mesh, s);
       }
     }
@@ -182,7 +189,13 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
       ProfScope ps(PROF_GATHER, (double)NkNw * (sizeof(cpx<R>) + sizeof(zd)), s);
       gather<R>(dst, work.p, tail ? 1.0 : w, false, J, tail ? sawc_d.p : nullptr, mesh, s);
     }
-    auto pr = op.partner[m + Nchi * l];
+    if (self_tail) {
+      conj_copy(dst, dst, NkNw, s);
+      product_rev<R>(work.p, grid(F0g, a13, b13), grid(F0g, a42, b42), (R)(w), true, false, mesh, s);
+      fft(work.p, +1, false, s);
+      gather<R>(dst, work.p, 1.0, true, zd(0, 0), nullptr, mesh, s);
+      continue;
+    }
     if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
   threshold(chi, NkNw * Nchi * Nchi, 1e-9, s);

@@ -1,0 +1,160 @@
+"""Host-side model builders and small helpers of the FLEX/Eliashberg path.
+
+Restates the reference's Python helpers that feed the library:
+  * import_hoppings(ftype=1)  - libs/plibs/_lattice.py:34-41 ("rx ry rz Re Im" rows)
+  * get_chi_orb_list          - libs/plibs/_response.py:382-413 (single site)
+  * gap_symms/get_initial_gap - libs/plibs/_response.py:415-462
+  * calc_mu / get_mu          - libs/plibs/_bands.py:63-78, _calc.py:298-309
+and builds the seeded synthetic lattices of the benchmark workloads.  Every
+number-crunching call goes through libgmod (pyrpa_amd.flibs)."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import flibs as F
+
+SEED_BASE = 20261202
+
+
+# --------------------------------------------------------------------------- hoppings
+def import_hoppings(fname: str, ftype: int = 1):
+    """(rvec[Nr,3], ham_r[Nr,No,No], No, Nr) from a ".hop"/".input" text file
+    (ftype=1, libs/plibs/_lattice.py:34-41)."""
+    if ftype != 1:
+        raise NotImplementedError("only the ftype=1 'rx ry rz Re Im' format is supported")
+    data = np.loadtxt(fname, ndmin=2)
+    con = int((data[:, :3] == data[0, :3]).prod(axis=1).sum())
+    no, nr = int(np.sqrt(con)), data[:, 0].size // con
+    rvec = np.array(data[:nr, :3])
+    ham_r = (data[:, 3] + 1j * data[:, 4]).reshape(no * no, nr).T.reshape(nr, no, no).round(6).copy()
+    return rvec, ham_r, no, nr
+
+
+def square_lattice(t=1.0, tp=0.0, tz=0.0):
+    """One-orbital square lattice in the sign convention of inputs/square.hop
+    (matrix element -t for nearest neighbours, -t' for next-nearest, -tz along z)."""
+    rv, hv = [], []
+    for R in [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0)]:
+        rv.append(R); hv.append(-t)
+    if tp != 0.0:
+        for R in [(1, 1, 0), (-1, -1, 0), (1, -1, 0), (-1, 1, 0)]:
+            rv.append(R); hv.append(-tp)
+    if tz != 0.0:
+        for R in [(0, 0, 1), (0, 0, -1)]:
+            rv.append(R); hv.append(-tz)
+    rvec = np.array(rv, dtype=np.float64)
+    ham_r = np.array(hv, dtype=np.complex128).reshape(-1, 1, 1)
+    return rvec, ham_r
+
+
+def synthetic_lattice(norb, in_plane, amp, onsite, seed, inter=(), inter_scale=1.0):
+    """Seeded multi-orbital tight binding with real hoppings (time-reversal
+    symmetric, which the reference's k/-k and Matsubara unfolding assumes):
+    H(R) entries uniform in [-amp, amp], H(-R) = H(R)^T, on-site levels uniform
+    in [-onsite, onsite] and real symmetric on-site mixing."""
+    rng = np.random.default_rng(seed)
+    h0 = np.diag(rng.uniform(-onsite, onsite, norb))
+    off = rng.uniform(-amp, amp, (norb, norb))
+    off = np.triu(off, 1)
+    h0 = h0 + off + off.T
+    rv, hs = [(0, 0, 0)], [h0]
+    for R, scl in [(R, 1.0) for R in in_plane] + [(R, inter_scale) for R in inter]:
+        h = rng.uniform(-amp, amp, (norb, norb)) * scl
+        rv += [R, tuple(-x for x in R)]
+        hs += [h, h.T]
+    return np.array(rv, dtype=np.float64), np.array(hs, dtype=np.complex128)
+
+
+# --------------------------------------------------------------------------- orbital pairs, gaps
+def get_chi_orb_list(Norb, site_prof=None):
+    """olist (Nchi,2), F-contiguous like the reference (_response.py:391-395)."""
+    tmp = np.arange(Norb) + 1
+    o1, o2 = np.meshgrid(tmp, tmp)
+    chiolist = np.array([o1.flatten(), o2.flatten()]).T
+    return chiolist.astype(np.int64), np.ones(len(chiolist), dtype=np.int64)
+
+
+def gap_symms(klist, Norb, gap_sym):
+    A = 2 * np.pi
+    kx, ky, kz = klist[:, 0], klist[:, 1], klist[:, 2]
+    rows = {0: np.ones(len(klist)), 1: np.cos(A * kx) - np.cos(A * ky), 2: 2 * np.cos(A * kx) * np.cos(A * ky),
+            3: 2 * np.sin(A * kx) * np.sin(A * ky), 4: 2 * np.sin(A * kx) * np.sin(A * kz),
+            5: 2 * np.sin(A * ky) * np.sin(A * kz), -1: 2 * np.sin(A * kx), -2: 2 * np.sin(A * ky)}
+    return np.tile(rows.get(gap_sym, np.zeros(len(klist))), (Norb, 1))
+
+
+get_initial_gap = gap_symms
+
+
+# --------------------------------------------------------------------------- chemical potential
+def calc_mu(eig, Nk, fill, temp):
+    """brentq on sum_k f(e_k - mu) = fill * Nk (libs/plibs/_bands.py:63-78)."""
+    from scipy.optimize import brentq
+
+    def func(mu):
+        return fill * Nk - F.get_ffermi(eig, mu, temp).sum()
+    return brentq(func, eig.min(), eig.max(), xtol=1e-14)
+
+
+def get_mu(ham_r, rvec, temp, fill, kmesh=40):
+    """Initial mu on a kmesh^3 full mesh (libs/plibs/_calc.py:298-309)."""
+    g = np.arange(kmesh) / kmesh
+    kx, ky, kz = np.meshgrid(g, g, g, indexing='ij')
+    kl = np.stack([kx.ravel(), ky.ravel(), kz.ravel()], axis=1)
+    eig = F.get_eig(F.gen_ham(kl, ham_r, rvec), sw=False)
+    return calc_mu(eig, len(kl), fill, temp)
+
+
+# --------------------------------------------------------------------------- workloads
+def build(rvec, ham_r, Nx, Ny, Nz, Nw, temp, U, J, fill_per_spin, mu=None):
+    """Everything mkself / get_chi0 / lin_eliash need for one lattice."""
+    Norb = ham_r.shape[1]
+    klist, kmap, invk = F.gen_irr_k_TRS(Nx, Ny, Nz)
+    hamk = F.gen_ham(klist, ham_r, rvec)
+    eig, uni = F.get_eig(hamk)
+    olist, site = get_chi_orb_list(Norb)
+    Smat, Cmat = F.gen_SCmatrix(olist, site, U, J)
+    if mu is None:
+        mu = get_mu(ham_r, rvec, temp, fill_per_spin)
+    return dict(Nx=Nx, Ny=Ny, Nz=Nz, Nw=Nw, temp=temp, U=U, J=J, fill=fill_per_spin, mu=mu, Norb=Norb,
+                rvec=rvec, ham_r=ham_r, klist=klist, kmap=kmap, invk=invk, hamk=hamk, eig=eig, uni=uni,
+                olist=olist, site=site, Smat=Smat, Cmat=Cmat, plist=np.ones(Norb))
+
+
+def config(c, **over):
+    """The five benchmark workloads (c = 1..5 in the order they are listed in
+    the benchmark description; 'filling n' is electrons per site, both spins,
+    so the per-spin rfill of the reference is n*Norb/2 for per-orbital n)."""
+    if c == 1:  # plain FLEX, 1-orbital square, 64x64, T=0.02, Nw=1024, n=0.85, U=3
+        rvec, ham_r = square_lattice(1.0, -0.25)
+        p = dict(Nx=64, Ny=64, Nz=1, Nw=1024, temp=0.02, U=3.0, J=0.0, fill=0.85 / 2)
+    elif c == 2:  # RPA, 5-orbital 2D to 3rd neighbours, 64x64, T=0.01 eV, 512 bosonic frequencies
+        R = [(1, 0, 0), (0, 1, 0), (1, 1, 0), (1, -1, 0), (2, 0, 0), (0, 2, 0)]
+        rvec, ham_r = synthetic_lattice(5, R, 0.5, 0.3, SEED_BASE + 2)
+        p = dict(Nx=64, Ny=64, Nz=1, Nw=512, temp=0.01, U=1.2, J=0.3, fill=2.5)
+    elif c == 3:  # mixed-precision FLEX, 4-orbital 3D, 32x32x16, T=0.015, Nw=1024, U=1, J=0.2, 0.9/orbital
+        R = [(1, 0, 0), (0, 1, 0), (1, 1, 0), (1, -1, 0)]
+        Rz = [(0, 0, 1), (1, 0, 1), (0, 1, 1), (-1, 0, 1), (0, -1, 1)]
+        rvec, ham_r = synthetic_lattice(4, R, 0.4, 0.4, SEED_BASE + 3, inter=Rz, inter_scale=0.2)
+        p = dict(Nx=32, Ny=32, Nz=16, Nw=1024, temp=0.015, U=1.0, J=0.2, fill=0.9 * 4 / 2)
+    elif c == 4:  # Eliashberg on FLEX output, 1-orbital, t'=-0.3, 128x128, T=0.01, Nw=2048, n=0.88, U=3.5
+        rvec, ham_r = square_lattice(1.0, -0.3)
+        p = dict(Nx=128, Ny=128, Nz=1, Nw=2048, temp=0.01, U=3.5, J=0.0, fill=0.88 / 2)
+    elif c == 5:  # surrogate training solve: 32x32, 256 frequencies (parameters per sample)
+        rvec, ham_r = square_lattice(1.0, over.pop("tp", -0.2))
+        p = dict(Nx=32, Ny=32, Nz=1, Nw=256, temp=over.pop("T", 0.03), U=over.pop("U", 3.0), J=0.0,
+                 fill=over.pop("n", 0.85) / 2)
+    else:
+        raise ValueError(c)
+    p.update(over)
+    fill = p.pop("fill")
+    return build(rvec, ham_r, p["Nx"], p["Ny"], p["Nz"], p["Nw"], p["temp"], p["U"], p["J"], fill)
+
+
+def flex(st, eps=1e-8, pp=0.5, m_diis=1, scf_loop=300, sw_tail=False, sw_rescale=True, sub_sigma=1,
+         sw_in=False, sigma_scale=1.0):
+    """mkself with the workload's arguments (libs/plibs/_calc.py:338-352 order)."""
+    return F.mkself(st['Smat'], st['Cmat'], st['kmap'], st['invk'], st['olist'], st['hamk'], st['eig'], st['uni'],
+                    st['mu'], st['fill'], st['temp'], st['Nw'], st['Nx'], st['Ny'], st['Nz'], False, sw_in,
+                    sub_sigma=sub_sigma, scf_loop=scf_loop, eps=eps, pp=pp, m_diis=m_diis, sw_rescale=sw_rescale,
+                    sw_tail=sw_tail, sigma_scale=sigma_scale)
