@@ -27,9 +27,28 @@ class ConvEngine {
   // chi(Nk,Nw,Nchi,Nchi) <- bubble of the transformed Fg (and, if tail, minus
   // the G0 bubble plus the analytic tau reference).  Thresholds at 1e-9.
   void chi0(zd* chi, double temp, bool tail, const double* eig_d, const zd* uni_d, const double* eig_h,
-            const zd* uni_h, const int64_t* invk, double mu, hipStream_t s);
+            const zd* uni_h, const int64_t* invk, double mu, hipStream_t s, bool apply_threshold = true);
   // sigma(Nk,Nw,No,No) <- (T/Nkall) sum_pairs V (*) G, using Fg; thresholds at 1e-9
-  void sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s);
+  void sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s,
+             bool apply_threshold = true);
+
+  // tail constants J (per chi pair, [l + Nchi*m]) at mu and the sawtooth table (built once)
+  void tail_constants(double temp, const double* eig_h, const zd* uni_h, const int64_t* invk, double mu,
+                      std::vector<zd>& J, DBuf<zd>& sawc_d, hipStream_t s) const;
+
+  // ---- increments around a reference state (mixed precision FLEX)
+  // Transforms of the reference G (and, with the tail, G0 and the tau kernels at mu_ref).
+  void set_reference(const zd* Gref, const zd* G0ref, bool tail, const double* eig_d, const zd* uni_d, double mu_ref,
+                     double temp, hipStream_t s);
+  // Transforms of the vertex reference for all Nchi^2 pairs (full grid incl. the nu->inf slot).
+  void set_vertex_reference(const zd* Vref, const double* S_h, const double* C_h, hipStream_t s);
+  // dchi = chi0[G] - chi0[Gref] (+ tail terms at mu vs mu_ref), with the reference's
+  // partner/self-partner conjugation applied to the increment.
+  void chi0_increment(zd* dchi, const zd* G, const zd* Gref, const zd* G0, const zd* G0ref, double temp, bool tail,
+                      const double* eig_d, const zd* uni_d, double mu, double mu_ref, const std::vector<zd>& dJ,
+                      const zd* sawc_d, hipStream_t s);
+  // sig = sig_ref + (Sigma[V, G] - Sigma[Vref, Gref]) (no threshold)
+  void sigma_increment(zd* sig, const zd* sig_ref, const zd* V, const zd* Vref, double temp, hipStream_t s);
   // Eliashberg: cache FFT[V_Delta] of the qualifying pairs (feliash.f90:540)
   void prepare_gap(const zd* Vd, const double* S_h, const double* C_h, bool singlet, hipStream_t s);
   // newdelta (un-weighted, Hermitised) <- sum V_Delta (*) F  (feliash.f90:486-635)
@@ -41,6 +60,7 @@ class ConvEngine {
   const FFTPlan<R>* p4;
   const FFTPlan<R>* p3;
   DBuf<cpx<R>> Fg, F0g, TA, TB, work, tmp, acc, Vft, Ff;
+  DBuf<cpx<R>> Fr, Fd, F0r, F0d, TAr, TBr, TAd, TBd, FVr;  // increment mode
   std::vector<std::array<int, 2>> gap_pairs;
   bool vft_cached = false;
   const zd* Vd_dev = nullptr;

@@ -82,82 +82,109 @@ void ConvEngine<R>::transform_green(const zd* G, bool g0, hipStream_t s) {
   DBuf<cpx<R>>& dst = g0 ? F0g : Fg;
   const int No = op.No;
   if (dst.n != (size_t)(No * No) * N) dst.alloc((size_t)No * No * N);
+  const MeshDev md = mesh_dev(mesh);
   for (int b = 0; b < No; ++b)
     for (int a = 0; a < No; ++a) {
-      {
-        ProfScope ps(PROF_SCATTER, (double)N * sizeof(cpx<R>), s);
-        scatter_green<R>(grid(dst, a, b), G, a, b, No, mesh, s);
-      }
-      fft(grid(dst, a, b), -1, false, s);
+      cpx<R>* g = grid(dst, a, b);
+      p4->exec_fused(GreenLoad<R>{G, a, b, No, md}, GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
     }
+}
+
+// Tail constants of chi0 at chemical potential mu (fchi.f90:731-772):
+// J[l + Nchi*m] = delta_{a13 b13} rhobar_{a42 b42} - delta_{a42 b42} rhobar_{a13 b13}
+// with rhobar the BZ-averaged free density matrix, and the sawtooth correction
+// sawc(nu_j) of the discrete tau reference.
+template <typename R>
+void ConvEngine<R>::tail_constants(double temp, const double* eig_h, const zd* uni_h, const int64_t* invk, double mu,
+                                   std::vector<zd>& J, DBuf<zd>& sawc_d, hipStream_t s) const {
+  const int No = op.No, Nchi = op.Nchi;
+  const long Nk = mesh.Nk, Nw = mesh.Nw, M = mesh.M, Nkall = mesh.Nkall;
+  const double beta = 1.0 / temp;
+  std::vector<zd> rho((size_t)No * No * Nk);
+  for (long k = 0; k < Nk; ++k) {
+    std::vector<double> f(No);
+    for (int n = 0; n < No; ++n) {
+      double xi = eig_h[n + No * k] - mu;
+      f[n] = xi >= 0 ? std::exp(-beta * xi) / (1.0 + std::exp(-beta * xi)) : 1.0 / (1.0 + std::exp(beta * xi));
+    }
+    for (int l = 0; l < No; ++l)
+      for (int m = 0; m < No; ++m) {
+        zd acc(0, 0);
+        for (int n = 0; n < No; ++n) acc += uni_h[m + No * (n + No * k)] * conj(uni_h[l + No * (n + No * k)]) * f[n];
+        rho[m + No * (l + No * k)] = acc;
+      }
+  }
+  std::vector<zd> rb(No * No, zd(0, 0));
+  for (long i = 0; i < Nkall; ++i) {
+    long kk = invk[3 * i] - 1;
+    if (invk[3 * i + 1] == 0) {
+      for (int l = 0; l < No; ++l)
+        for (int m = 0; m < No; ++m) rb[m + No * l] += rho[m + No * (l + No * kk)];
+    } else if (invk[3 * i + 1] == 1) {
+      for (int l = 0; l < No; ++l)
+        for (int m = 0; m < No; ++m) rb[m + No * l] += rho[l + No * (m + No * kk)];
+    }
+  }
+  for (auto& v : rb) v = v * (1.0 / (double)Nkall);
+  J.assign((size_t)Nchi * Nchi, zd(0, 0));
+  for (int l = 0; l < Nchi; ++l)
+    for (int m = 0; m < Nchi; ++m) {
+      const int a13 = op.ol[l][0], b13 = op.ol[m][0], a42 = op.ol[m][1], b42 = op.ol[l][1];
+      zd v(0, 0);
+      if (a13 == b13) v += rb[a42 + No * b42];
+      if (a42 == b42) v -= rb[a13 + No * b13];
+      J[l + Nchi * m] = v;
+    }
+  if (sawc_d.n == (size_t)Nw) return;
+  std::vector<zd> sawc(Nw);
+  const double dtau = beta / (double)M;
+  for (long j = 1; j <= Nw; ++j) {
+    zd v(0, 0);
+    if (j > 1) v = zd(0, 1.0 / (2.0 * PI_E * (double)(j - 1) * temp));
+    for (long sidx = 0; sidx < M; ++sidx) {
+      double ang = 2.0 * PI_E * (double)(j - 1) * (double)sidx / (double)M;
+      zd ph(std::cos(ang), std::sin(ang));
+      v -= ph * (dtau * (0.5 - (double)sidx / (double)M));
+    }
+    sawc[j - 1] = v;
+  }
+  sawc_d.alloc(Nw);
+  sawc_d.upload(sawc.data(), Nw, s);
 }
 
 template <typename R>
 void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, const zd* uni_d,
-                         const double* eig_h, const zd* uni_h, const int64_t* invk, double mu, hipStream_t s) {
+                         const double* eig_h, const zd* uni_h, const int64_t* invk, double mu, hipStream_t s,
+                         bool apply_threshold) {
   const int No = op.No, Nchi = op.Nchi;
   const long Nk = mesh.Nk, Nw = mesh.Nw, M = mesh.M, Nkall = mesh.Nkall;
   const long NkNw = Nk * Nw;
   const double w = temp / (double)Nkall;
-  std::vector<zd> rb(No * No, zd(0, 0));
+  std::vector<zd> rb;
   DBuf<zd> sawc_d;
   if (tail) {
     DBuf<zd> G0((size_t)NkNw * No * No);
     green0(G0.p, eig_d, uni_d, mu, temp, Nk, (int)Nw, No, s);
     transform_green(G0.p, true, s);
     if (TA.n != (size_t)(No * No) * N) { TA.alloc((size_t)No * No * N); TB.alloc((size_t)No * No * N); }
+    const MeshDev md = mesh_dev(mesh);
+    const double beta_ = 1.0 / temp, dtau_ = beta_ / (double)M;
     for (int b = 0; b < No; ++b)
       for (int a = 0; a < No; ++a) {
-        scatter_tau_ref<R>(grid(TA, a, b), eig_d, uni_d, a, b, No, mu, temp, 0, mesh, s);
-        fft(grid(TA, a, b), -1, true, s);
-        scatter_tau_ref<R>(grid(TB, a, b), eig_d, uni_d, a, b, No, mu, temp, 1, mesh, s);
-        fft(grid(TB, a, b), -1, true, s);
-      }
-    // BZ-averaged free density matrix (fchi.f90:731-755)
-    const double beta = 1.0 / temp;
-    std::vector<zd> rho((size_t)No * No * Nk);
-    for (long k = 0; k < Nk; ++k) {
-      std::vector<double> f(No);
-      for (int n = 0; n < No; ++n) {
-        double xi = eig_h[n + No * k] - mu;
-        f[n] = xi >= 0 ? std::exp(-beta * xi) / (1.0 + std::exp(-beta * xi)) : 1.0 / (1.0 + std::exp(beta * xi));
-      }
-      for (int l = 0; l < No; ++l)
-        for (int m = 0; m < No; ++m) {
-          zd acc(0, 0);
-          for (int n = 0; n < No; ++n)
-            acc += uni_h[m + No * (n + No * k)] * conj(uni_h[l + No * (n + No * k)]) * f[n];
-          rho[m + No * (l + No * k)] = acc;
+        for (int which = 0; which < 2; ++which) {
+          cpx<R>* g = grid(which == 0 ? TA : TB, a, b);
+          p3->exec_fused(TauRefLoad<R>{eig_d, uni_d, a, b, No, which, mu, beta_, dtau_, md}, GridStore<R>{g, (unsigned)mesh.Nkall}, g,
+                         tmp.p, -1, (R)1, s);
         }
-    }
-    for (long i = 0; i < Nkall; ++i) {
-      long kk = invk[3 * i] - 1;
-      if (invk[3 * i + 1] == 0) {
-        for (int l = 0; l < No; ++l)
-          for (int m = 0; m < No; ++m) rb[m + No * l] += rho[m + No * (l + No * kk)];
-      } else if (invk[3 * i + 1] == 1) {
-        for (int l = 0; l < No; ++l)
-          for (int m = 0; m < No; ++m) rb[m + No * l] += rho[l + No * (m + No * kk)];
       }
-    }
-    for (auto& v : rb) v = v * (1.0 / (double)Nkall);
-    // sawtooth correction (fchi.f90:757-772)
-    std::vector<zd> sawc(Nw);
-    const double dtau = beta / (double)M;
-    for (long j = 1; j <= Nw; ++j) {
-      zd v(0, 0);
-      if (j > 1) v = zd(0, 1.0 / (2.0 * PI_E * (double)(j - 1) * temp));
-      for (long sidx = 0; sidx < M; ++sidx) {
-        double ang = 2.0 * PI_E * (double)(j - 1) * (double)sidx / (double)M;
-        zd ph(std::cos(ang), std::sin(ang));
-        v -= ph * (dtau * (0.5 - (double)sidx / (double)M));
-      }
-      sawc[j - 1] = v;
-    }
-    sawc_d.alloc(Nw);
-    sawc_d.upload(sawc.data(), Nw, s);
+    std::vector<zd> Jall;
+    tail_constants(temp, eig_h, uni_h, invk, mu, Jall, sawc_d, s);
+    rb = Jall;
   }
   const double wref = 1.0 / (temp * (double)Nkall);
+  const MeshDev md = mesh_dev(mesh);
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
+  if (!mesh.complete) GMOD_HIP(hipMemsetAsync(chi, 0, NkNw * Nchi * Nchi * sizeof(zd), s));
   for (auto& lm : op.irr) {
     const int l = lm[0], m = lm[1];
     const int a13 = op.ol[l][0], b13 = op.ol[m][0], a42 = op.ol[m][1], b42 = op.ol[l][1];
@@ -165,47 +192,50 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
     // A self-partner slice (partner == itself) is conjugated in place after each
     // of the reference's three accumulation passes (fchi.f90:610, 846); the net
     // result is conj(conv[G] + ref) - conv[G0].  It equals conj of the plain sum
-    // whenever TRS holds; keep the reference's order so results match it
-    // on any input.
+    // whenever TRS holds; keep the reference's order so results match it on any
+    // input.
     const bool self_tail = tail && pr[0] == m && pr[1] == l;
-    {
-      ProfScope ps(PROF_PRODUCT, (double)N * sizeof(cpx<R>) * (tail ? 7 : 3), s);
-      if (!tail) {
-        product_rev<R>(work.p, grid(Fg, a13, b13), grid(Fg, a42, b42), (R)-1, true, false, mesh, s);
-      } else {
-        product_rev<R>(work.p, grid(Fg, a13, b13), grid(Fg, a42, b42), (R)(-w), true, false, mesh, s);
-        if (!self_tail) product_rev<R>(work.p, grid(F0g, a13, b13), grid(F0g, a42, b42), (R)(w), true, true, mesh, s);
-        product_rev<R>(work.p, grid(TA, a13, b13), grid(TB, a42, b42), (R)(-wref), false, true, mesh, s);
+    ProductLoad<R> pl{};
+    pl.m = md;
+    if (!tail) {
+      pl.nterm = 1;
+      pl.A[0] = grid(Fg, a13, b13); pl.B[0] = grid(Fg, a42, b42); pl.coef[0] = (R)-1; pl.revt[0] = 1;
+    } else {
+      int t = 0;
+      pl.A[t] = grid(Fg, a13, b13); pl.B[t] = grid(Fg, a42, b42); pl.coef[t] = (R)(-w); pl.revt[t++] = 1;
+      if (!self_tail) {
+        pl.A[t] = grid(F0g, a13, b13); pl.B[t] = grid(F0g, a42, b42); pl.coef[t] = (R)w; pl.revt[t++] = 1;
       }
+      pl.A[t] = grid(TA, a13, b13); pl.B[t] = grid(TB, a42, b42); pl.coef[t] = (R)(-wref); pl.revt[t++] = 0;
+      pl.nterm = t;
     }
-    fft(work.p, +1, false, s);
-    zd J(0, 0);
-    if (tail) {
-      if (a13 == b13) J += rb[a42 + No * b42];
-      if (a42 == b42) J -= rb[a13 + No * b13];
-    }
+    const zd J = tail ? rb[l + Nchi * m] : zd(0, 0);
     zd* dst = chi + NkNw * (m + (long)Nchi * l);
-    {
-      ProfScope ps(PROF_GATHER, (double)NkNw * (sizeof(cpx<R>) + sizeof(zd)), s);
-      gather<R>(dst, work.p, tail ? 1.0 : w, false, J, tail ? sawc_d.p : nullptr, mesh, s);
-    }
+    p4->exec_fused(pl, GatherStore<R>{dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, md}, work.p, tmp.p, +1,
+                   invN, s);
     if (self_tail) {
       conj_copy(dst, dst, NkNw, s);
-      product_rev<R>(work.p, grid(F0g, a13, b13), grid(F0g, a42, b42), (R)(w), true, false, mesh, s);
-      fft(work.p, +1, false, s);
-      gather<R>(dst, work.p, 1.0, true, zd(0, 0), nullptr, mesh, s);
+      ProductLoad<R> p0{};
+      p0.m = md;
+      p0.nterm = 1;
+      p0.A[0] = grid(F0g, a13, b13); p0.B[0] = grid(F0g, a42, b42); p0.coef[0] = (R)w; p0.revt[0] = 1;
+      p4->exec_fused(p0, GatherStore<R>{dst, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1, invN, s);
       continue;
     }
     if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
-  threshold(chi, NkNw * Nchi * Nchi, 1e-9, s);
+  if (apply_threshold) threshold(chi, NkNw * Nchi * Nchi, 1e-9, s);
 }
 
 template <typename R>
-void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s) {
+void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s,
+                          bool apply_threshold) {
   const int No = op.No, Nchi = op.Nchi;
   const long NkNw = mesh.Nk * (long)mesh.Nw;
   if (acc.n != (size_t)N) acc.alloc(N);
+  const MeshDev md = mesh_dev(mesh);
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
+  const long NC = NkNw;
   for (int l1 = 0; l1 < No; ++l1)
     for (int m1 = 0; m1 < No; ++m1) {
       bool first = true;
@@ -213,15 +243,9 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double*
         for (int m = 0; m < Nchi; ++m) {
           if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
           zd stat(1.5 * S_h[m + Nchi * l] - 0.5 * C_h[m + Nchi * l], 0);
-          {
-            ProfScope ps(PROF_SCATTER, (double)N * sizeof(cpx<R>), s);
-            scatter_boson<R>(work.p, V, m, l, Nchi, stat, mesh, s);
-          }
-          fft(work.p, -1, false, s);
-          {
-            ProfScope ps(PROF_PRODUCT, (double)N * sizeof(cpx<R>) * (first ? 3 : 4), s);
-            mul_acc<R>(acc.p, work.p, grid(Fg, op.ol[m][1], op.ol[l][1]), first, N, s);
-          }
+          BosonLoad<R> bl{V, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md};
+          p4->exec_fused(bl, MulAccStore<R>{acc.p, grid(Fg, op.ol[m][1], op.ol[l][1]), first ? 1 : 0, (unsigned)mesh.Nkall}, work.p,
+                         tmp.p, -1, (R)1, s);
           first = false;
         }
       zd* dst = sig + NkNw * (m1 + (long)No * l1);
@@ -229,11 +253,11 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double*
         GMOD_HIP(hipMemsetAsync(dst, 0, NkNw * sizeof(zd), s));
         continue;
       }
-      fft(acc.p, +1, false, s);
-      ProfScope ps(PROF_GATHER, (double)NkNw * (sizeof(cpx<R>) + sizeof(zd)), s);
-      gather<R>(dst, acc.p, temp / (double)mesh.Nkall, false, zd(0, 0), nullptr, mesh, s);
+      if (!mesh.complete) GMOD_HIP(hipMemsetAsync(dst, 0, NkNw * sizeof(zd), s));
+      p4->exec_fused(GridLoad<R>{acc.p, (unsigned)mesh.Nkall}, GatherStore<R>{dst, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md},
+                     acc.p, tmp.p, +1, invN, s);
     }
-  threshold(sig, NkNw * No * No, 1e-9, s);
+  if (apply_threshold) threshold(sig, NkNw * No * No, 1e-9, s);
 }
 
 template <typename R>
@@ -253,11 +277,14 @@ void ConvEngine<R>::prepare_gap(const zd* Vd, const double* S_h, const double* C
   vft_cached = need < 0.5 * (double)free_b;
   if (!vft_cached) return;
   Vft.alloc(gap_pairs.size() * (size_t)N);
+  const MeshDev md = mesh_dev(mesh);
+  const long NC = mesh.Nk * (long)mesh.Nw;
   for (size_t q = 0; q < gap_pairs.size(); ++q) {
     int l = gap_pairs[q][0], m = gap_pairs[q][1];
     zd stat(0.5 * (S_h[m + Nchi * l] + sgn * C_h[m + Nchi * l]), 0);
-    scatter_boson<R>(Vft.p + q * N, Vd, m, l, Nchi, stat, mesh, s);
-    fft(Vft.p + q * N, -1, false, s);
+    cpx<R>* g = Vft.p + q * N;
+    p4->exec_fused(BosonLoad<R>{Vd, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md}, GridStore<R>{g, (unsigned)mesh.Nkall},
+                   g, tmp.p, -1, (R)1, s);
   }
 }
 
@@ -268,10 +295,13 @@ void ConvEngine<R>::gap(zd* nd, const zd* F, const double* prt, bool singlet, hi
   const double sgn = singlet ? 1.0 : -1.0, dprt = sgn;
   if (Ff.n != (size_t)(No * No) * N) Ff.alloc((size_t)No * No * N);
   if (acc.n != (size_t)N) acc.alloc(N);
+  const MeshDev md = mesh_dev(mesh);
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
   for (int b = 0; b < No; ++b)
     for (int a = 0; a < No; ++a) {
-      scatter_anom<R>(grid(Ff, a, b), F, a, b, No, prt[a] * prt[b], sgn, dprt, mesh, s);
-      fft(grid(Ff, a, b), -1, false, s);
+      cpx<R>* g = grid(Ff, a, b);
+      AnomLoad<R> al{F, NkNw * (a + (long)No * b), NkNw * (b + (long)No * a), prt[a] * prt[b], sgn, dprt, md};
+      p4->exec_fused(al, GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
     }
   GMOD_HIP(hipMemsetAsync(nd, 0, NkNw * No * No * sizeof(zd), s));
   for (int m2 = 0; m2 < No; ++m2)
@@ -280,23 +310,180 @@ void ConvEngine<R>::gap(zd* nd, const zd* F, const double* prt, bool singlet, hi
       for (size_t q = 0; q < gap_pairs.size(); ++q) {
         int l = gap_pairs[q][0], m = gap_pairs[q][1];
         if (op.ol[l][0] != l1 || op.ol[m][1] != m2) continue;
-        const cpx<R>* v;
+        const cpx<R>* fg = grid(Ff, op.ol[l][1], op.ol[m][0]);
         if (vft_cached) {
-          v = Vft.p + q * N;
+          mul_acc<R>(acc.p, Vft.p + q * N, fg, first, N, s);
         } else {
           zd stat(0.5 * (S_cache[m + Nchi * l] + sgn * C_cache[m + Nchi * l]), 0);
-          scatter_boson<R>(work.p, Vd_dev, m, l, Nchi, stat, mesh, s);
-          fft(work.p, -1, false, s);
-          v = work.p;
+          BosonLoad<R> bl{Vd_dev, NkNw * (m + (long)Nchi * l), NkNw * (l + (long)Nchi * m), stat, md};
+          p4->exec_fused(bl, MulAccStore<R>{acc.p, fg, first ? 1 : 0, (unsigned)mesh.Nkall}, work.p, tmp.p, -1, (R)1, s);
         }
-        mul_acc<R>(acc.p, v, grid(Ff, op.ol[l][1], op.ol[m][0]), first, N, s);
         first = false;
       }
       if (first) continue;
-      fft(acc.p, +1, false, s);
-      gather<R>(nd + NkNw * (l1 + (long)No * m2), acc.p, 1.0, false, zd(0, 0), nullptr, mesh, s);
+      p4->exec_fused(GridLoad<R>{acc.p, (unsigned)mesh.Nkall}, GatherStore<R>{nd + NkNw * (l1 + (long)No * m2), 1.0, zd(0, 0), nullptr, 0, md},
+                     acc.p, tmp.p, +1, invN, s);
     }
   hermitize_lower(nd, mesh.Nk, mesh.Nw, No, s);
+}
+
+// ====================================================================== increments
+template <typename R>
+void ConvEngine<R>::set_reference(const zd* Gref, const zd* G0ref, bool tail, const double* eig_d, const zd* uni_d,
+                                  double mu_ref, double temp, hipStream_t s) {
+  const int No = op.No;
+  const size_t nb = (size_t)No * No * N;
+  const MeshDev md = mesh_dev(mesh);
+  const unsigned S = (unsigned)mesh.Nkall;
+  auto ensure = [&](DBuf<cpx<R>>& b) { if (b.n != nb) b.alloc(nb); };
+  ensure(Fr); ensure(Fd);
+  for (int b = 0; b < No; ++b)
+    for (int a = 0; a < No; ++a) {
+      cpx<R>* g = grid(Fr, a, b);
+      p4->exec_fused(GreenLoad<R>{Gref, a, b, No, md}, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+    }
+  if (!tail) return;
+  ensure(F0r); ensure(F0d); ensure(TAr); ensure(TBr); ensure(TAd); ensure(TBd);
+  const double beta = 1.0 / temp, dtau = beta / (double)mesh.M;
+  for (int b = 0; b < No; ++b)
+    for (int a = 0; a < No; ++a) {
+      cpx<R>* g = grid(F0r, a, b);
+      p4->exec_fused(GreenLoad<R>{G0ref, a, b, No, md}, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+      for (int which = 0; which < 2; ++which) {
+        cpx<R>* t = grid(which == 0 ? TAr : TBr, a, b);
+        p3->exec_fused(TauRefLoad<R>{eig_d, uni_d, a, b, No, which, mu_ref, beta, dtau, md}, GridStore<R>{t, S}, t,
+                       tmp.p, -1, (R)1, s);
+      }
+    }
+}
+
+template <typename R>
+void ConvEngine<R>::set_vertex_reference(const zd* Vref, const double* S_h, const double* C_h, hipStream_t s) {
+  const int Nchi = op.Nchi;
+  const long NC = mesh.Nk * (long)mesh.Nw;
+  const size_t nb = (size_t)Nchi * Nchi * N;
+  if (FVr.n != nb) FVr.alloc(nb);
+  const MeshDev md = mesh_dev(mesh);
+  const unsigned S = (unsigned)mesh.Nkall;
+  for (int l = 0; l < Nchi; ++l)
+    for (int m = 0; m < Nchi; ++m) {
+      zd stat(1.5 * S_h[m + Nchi * l] - 0.5 * C_h[m + Nchi * l], 0);
+      cpx<R>* g = FVr.p + (size_t)(m + Nchi * l) * N;
+      p4->exec_fused(BosonLoad<R>{Vref, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md},
+                     GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+    }
+}
+
+template <typename R>
+void ConvEngine<R>::chi0_increment(zd* dchi, const zd* G, const zd* Gref, const zd* G0, const zd* G0ref, double temp,
+                                   bool tail, const double* eig_d, const zd* uni_d, double mu, double mu_ref,
+                                   const std::vector<zd>& dJ, const zd* sawc_d, hipStream_t s) {
+  const int No = op.No, Nchi = op.Nchi;
+  const long NkNw = mesh.Nk * (long)mesh.Nw;
+  const double w = temp / (double)mesh.Nkall;
+  const double wref = 1.0 / (temp * (double)mesh.Nkall);
+  const MeshDev md = mesh_dev(mesh);
+  const unsigned S = (unsigned)mesh.Nkall;
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
+  for (int b = 0; b < No; ++b)
+    for (int a = 0; a < No; ++a) {
+      cpx<R>* g = grid(Fd, a, b);
+      GreenDiffLoad<R> dl{GreenLoad<double>{G, a, b, No, md}, GreenLoad<double>{Gref, a, b, No, md}};
+      p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+    }
+  if (tail) {
+    const double beta = 1.0 / temp, dtau = beta / (double)mesh.M;
+    for (int b = 0; b < No; ++b)
+      for (int a = 0; a < No; ++a) {
+        cpx<R>* g = grid(F0d, a, b);
+        GreenDiffLoad<R> dl{GreenLoad<double>{G0, a, b, No, md}, GreenLoad<double>{G0ref, a, b, No, md}};
+        p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+        for (int which = 0; which < 2; ++which) {
+          cpx<R>* t = grid(which == 0 ? TAd : TBd, a, b);
+          TauRefDiffLoad<R> tl{TauRefLoad<double>{eig_d, uni_d, a, b, No, which, mu, beta, dtau, md},
+                               TauRefLoad<double>{eig_d, uni_d, a, b, No, which, mu_ref, beta, dtau, md}};
+          p3->exec_fused(tl, GridStore<R>{t, S}, t, tmp.p, -1, (R)1, s);
+        }
+      }
+  }
+  if (!mesh.complete) GMOD_HIP(hipMemsetAsync(dchi, 0, NkNw * Nchi * Nchi * sizeof(zd), s));
+  for (auto& lm : op.irr) {
+    const int l = lm[0], m = lm[1];
+    const int a13 = op.ol[l][0], b13 = op.ol[m][0], a42 = op.ol[m][1], b42 = op.ol[l][1];
+    auto pr = op.partner[m + Nchi * l];
+    const bool self_tail = tail && pr[0] == m && pr[1] == l;
+    // B(x, y) = x13 * y42(-r,-tau); increment of B(G,G): B(d,r) + B(r,d) + B(d,d)
+    auto add_bilinear = [&](ProductLoad<R>& pl, DBuf<cpx<R>>& Rf, DBuf<cpx<R>>& Df, DBuf<cpx<R>>* Rf2,
+                            DBuf<cpx<R>>* Df2, R c, int revt) {
+      DBuf<cpx<R>>& R2 = Rf2 ? *Rf2 : Rf;
+      DBuf<cpx<R>>& D2 = Df2 ? *Df2 : Df;
+      pl.add(grid(Df, a13, b13), grid(R2, a42, b42), c, revt);
+      pl.add(grid(Rf, a13, b13), grid(D2, a42, b42), c, revt);
+      pl.add(grid(Df, a13, b13), grid(D2, a42, b42), c, revt);
+    };
+    ProductLoad<R> pl{};
+    pl.m = md;
+    pl.nterm = 0;
+    add_bilinear(pl, Fr, Fd, nullptr, nullptr, (R)(tail ? -w : -1.0), 1);
+    if (tail) {
+      if (!self_tail) add_bilinear(pl, F0r, F0d, nullptr, nullptr, (R)w, 1);
+      add_bilinear(pl, TAr, TAd, &TBr, &TBd, (R)(-wref), 0);
+    }
+    zd J(0, 0);
+    if (tail) {
+      J = dJ[l + Nchi * m];
+    }
+    zd* dst = dchi + NkNw * (m + (long)Nchi * l);
+    p4->exec_fused(pl, GatherStore<R>{dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 0, md}, work.p, tmp.p, +1,
+                   invN, s);
+    if (self_tail) {
+      conj_copy(dst, dst, NkNw, s);
+      ProductLoad<R> p0{};
+      p0.m = md;
+      p0.nterm = 0;
+      add_bilinear(p0, F0r, F0d, nullptr, nullptr, (R)w, 1);
+      p4->exec_fused(p0, GatherStore<R>{dst, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1, invN, s);
+      continue;
+    }
+    if (pr[0] >= 0) conj_copy(dchi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
+  }
+}
+
+template <typename R>
+void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_ref, const zd* V, const zd* Vref, double temp,
+                                    hipStream_t s) {
+  const int No = op.No, Nchi = op.Nchi;
+  const long NkNw = mesh.Nk * (long)mesh.Nw;
+  if (acc.n != (size_t)N) acc.alloc(N);
+  const MeshDev md = mesh_dev(mesh);
+  const unsigned S = (unsigned)mesh.Nkall;
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
+  const long NC = NkNw;
+  for (int l1 = 0; l1 < No; ++l1)
+    for (int m1 = 0; m1 < No; ++m1) {
+      bool first = true;
+      for (int l = 0; l < Nchi; ++l)
+        for (int m = 0; m < Nchi; ++m) {
+          if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
+          const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
+          BosonDiffLoad<R> bl{BosonLoad<double>{V, oml, olm, zd(0, 0), md}, BosonLoad<double>{Vref, oml, olm, zd(0, 0), md}};
+          const int a = op.ol[m][1], b = op.ol[l][1];
+          SigmaIncStore<R> st{acc.p, grid(Fr, a, b), grid(Fd, a, b), FVr.p + (size_t)(m + Nchi * l) * N,
+                              first ? 1 : 0, S};
+          p4->exec_fused(bl, st, work.p, tmp.p, -1, (R)1, s);
+          first = false;
+        }
+      zd* dst = sig + NkNw * (m1 + (long)No * l1);
+      const zd* base = sig_ref + NkNw * (m1 + (long)No * l1);
+      if (first) {
+        copy(dst, base, NkNw, s);
+        continue;
+      }
+      if (!mesh.complete) copy(dst, base, NkNw, s);
+      GatherStore<R> gs{dst, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md};
+      gs.base = base;
+      p4->exec_fused(GridLoad<R>{acc.p, S}, gs, acc.p, tmp.p, +1, invN, s);
+    }
 }
 
 template class ConvEngine<double>;

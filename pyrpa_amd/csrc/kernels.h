@@ -21,42 +21,234 @@ struct Mesh {
   DBuf<int> sp;       // spatial grid index of full point i
   DBuf<int> rev;      // spatial index of -r for spatial index s
   DBuf<int> irr_sp;   // spatial index of the flag-0 full point of irreducible k (-1: none)
+  DBuf<int> irr_s;    // per spatial index: irreducible k
+  DBuf<int> flag_s;   // per spatial index: 0 direct / 1 reversed
+  DBuf<int> kgat_s;   // per spatial index: irreducible k if this is its flag-0 point, else -1
   DBuf<double> kw;    // full-BZ multiplicity of irreducible k
   std::vector<double> kw_host;
+  bool complete = true;  // every irreducible k has a direct (flag 0) full-BZ point
   long N() const { return Nkall * (long)M; }
   void build(const int64_t* kmap, const int64_t* invk, long Nkall, long Nk, int Nx, int Ny, int Nz,
              int Nw, hipStream_t s);
 };
 
-// ---- scatters (write the whole full grid)
-template <typename R>
-void scatter_green(cpx<R>* grid, const zd* G, int a, int b, int No, const Mesh& m, hipStream_t s);
-template <typename R>
-void scatter_boson(cpx<R>* grid, const zd* V, int mi, int li, int Nchi, zd stat, const Mesh& m,
-                   hipStream_t s);
-template <typename R>
-void scatter_anom(cpx<R>* grid, const zd* F, int a, int b, int No, double pp, double sgn, double dprt,
-                  const Mesh& m, hipStream_t s);
-// analytic band kernels on the tau grid (fchi.f90:703-729): which=0 -> g(+tau), 1 -> g(-tau)
-template <typename R>
-void scatter_tau_ref(cpx<R>* grid, const double* eig, const zd* uni, int a, int b, int No, double mu,
-                     double temp, int which, const Mesh& m, hipStream_t s);
+// ---- fused load / store functors for FFTPlan::exec_fused
+// Address a = sp + Nkall * t on the full grid (sp: spatial index, t: slot).
+struct MeshDev {
+  const int *irr_s, *flag_s, *kgat_s, *rev;  // per spatial index
+  long Nkall, Nk;
+  int Nw, M;
+};
+MeshDev mesh_dev(const Mesh& m);
 
-// ---- products
-// out = coef * A(r,t) * B(-r,-t)          (rev_t = true)   [fchi.f90:593]
-// out = coef * A(r,t) * B(-r, t)          (rev_t = false)  [fchi.f90:824]
+// G_ab on the full grid (fchi.f90:563-583, fself.f90:226-236):
+//   slot t <  Nw : w_t ;  k direct -> G(k)_ab, k = -k_irr -> G(k_irr)_ba
+//   slot t >= Nw : -w_j with j = M-1-t ; direct -> conj G_ba, reversed -> conj G_ab
 template <typename R>
-void product_rev(cpx<R>* out, const cpx<R>* A, const cpx<R>* B, R coef, bool rev_t, bool accumulate,
-                 const Mesh& m, hipStream_t s);
+struct GreenLoad {
+  const zd* G;
+  int a, b, No;
+  MeshDev m;
+  __device__ cpx<R> operator()(unsigned sp, unsigned tu) const {
+    const int t = (int)tu;
+    const long k = m.irr_s[sp];
+    const int f = m.flag_s[sp];
+    const long NkNw = m.Nk * (long)m.Nw;
+    if (t < m.Nw) {
+      const int p = f == 0 ? a : b, q = f == 0 ? b : a;
+      return cpx<R>(G[k + m.Nk * (long)t + NkNw * (p + (long)No * q)]);
+    }
+    const int j = m.M - 1 - t;
+    const int p = f == 0 ? b : a, q = f == 0 ? a : b;
+    return cpx<R>(conj(G[k + m.Nk * (long)j + NkNw * (p + (long)No * q)]));
+  }
+};
+
+// bosonic V_(m,l) (fself.f90:203-224, feliash.f90:543-564): slot 0 = nu_0,
+// slot Nw = the nu -> infinity value `stat`, slots t and M-t = +-nu_t
+template <typename R>
+struct BosonLoad {
+  const zd* V;
+  long off_ml, off_lm;
+  zd stat;
+  MeshDev m;
+  __device__ cpx<R> operator()(unsigned sp, unsigned tu) const {
+    const int t = (int)tu;
+    const long k = m.irr_s[sp];
+    const int f = m.flag_s[sp];
+    if (t == m.Nw) return cpx<R>(stat);
+    if (t < m.Nw) return cpx<R>(V[k + m.Nk * (long)t + (f == 0 ? off_ml : off_lm)]);
+    const int j = m.M - t;
+    return cpx<R>(conj(V[k + m.Nk * (long)j + (f == 0 ? off_lm : off_ml)]));
+  }
+};
+
+// anomalous F_(a,b) with parity factors (feliash.f90:566-577)
+template <typename R>
+struct AnomLoad {
+  const zd* F;
+  long oab, oba;
+  double pp, sgn, dprt;
+  MeshDev m;
+  __device__ cpx<R> operator()(unsigned sp, unsigned tu) const {
+    const int t = (int)tu;
+    const long k = m.irr_s[sp];
+    const int f = m.flag_s[sp];
+    if (t < m.Nw) {
+      zd v = F[k + m.Nk * (long)t + oab];
+      return cpx<R>(f == 0 ? v : v * (pp * dprt));
+    }
+    const int j = m.M - 1 - t;
+    if (f == 0) return cpx<R>(conj(F[k + m.Nk * (long)j + oab]) * (sgn * dprt * pp));
+    return cpx<R>(F[k + m.Nk * (long)j + oba] * sgn);
+  }
+};
+
+// sum_n U_an conj(U_bn) g_n(k, +-tau_s); -k uses the conjugated coefficient
+// (fchi.f90:703-729, 791-812), overflow-safe forms of the reference
+template <typename R>
+struct TauRefLoad {
+  const double* eig;
+  const zd* uni;
+  int a, b, No, which;
+  double mu, beta, dtau;
+  MeshDev m;
+  __device__ cpx<R> operator()(unsigned sp, unsigned su) const {
+    const int s = (int)su;
+    const long k = m.irr_s[sp];
+    const int f = m.flag_s[sp];
+    const double tau = (double)s * dtau;
+    zd acc(0, 0);
+    for (int n = 0; n < No; ++n) {
+      const double xi = eig[n + (long)No * k] - mu;
+      double gv;
+      if (xi >= 0.0) {
+        const double d = 1.0 + exp(-beta * xi);
+        gv = which == 0 ? -exp(-xi * tau) / d : exp(-xi * (beta - tau)) / d;
+      } else {
+        const double d = 1.0 + exp(beta * xi);
+        gv = which == 0 ? -exp(xi * (beta - tau)) / d : exp(xi * tau) / d;
+      }
+      zd c = uni[a + (long)No * (n + (long)No * k)] * conj(uni[b + (long)No * (n + (long)No * k)]);
+      if (f != 0) c = conj(c);
+      acc += c * gv;
+    }
+    return cpx<R>(acc);
+  }
+};
+
+// sum of up to three products coef * A(r,t) * B(-r, -t)  (revt) or B(-r, t)
+// (fchi.f90:593, 824)
+constexpr int PROD_MAXT = 9;
+template <typename R>
+struct ProductLoad {
+  const cpx<R>* A[PROD_MAXT];
+  const cpx<R>* B[PROD_MAXT];
+  R coef[PROD_MAXT];
+  int revt[PROD_MAXT];
+  int nterm;
+  __host__ void add(const cpx<R>* a, const cpx<R>* b, R c, int rt) {
+    if (nterm >= PROD_MAXT) throw Error("too many product terms");
+    A[nterm] = a; B[nterm] = b; coef[nterm] = c; revt[nterm] = rt; ++nterm;
+  }
+  MeshDev m;
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const {
+    const size_t ad = sp + (size_t)m.Nkall * t;
+    const unsigned rs = (unsigned)m.rev[sp];
+    const unsigned rtt = t == 0 ? 0u : (unsigned)m.M - t;
+    cpx<R> v(0, 0);
+    for (int i = 0; i < nterm; ++i) {
+      const unsigned rt = revt[i] ? rtt : t;
+      v += A[i][ad] * B[i][rs + (size_t)m.Nkall * rt] * coef[i];
+    }
+    return v;
+  }
+};
+
+// G_ab(current) - G_ab(reference) on the full grid, formed in FP64 before
+// the conversion to the grid precision (increments of the bilinear bubble)
+template <typename R>
+struct GreenDiffLoad {
+  GreenLoad<double> cur, ref;
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return cpx<R>(cur(sp, t) - ref(sp, t)); }
+};
+
+// V(current) - V(reference): the static nu->inf slot cancels
+template <typename R>
+struct BosonDiffLoad {
+  BosonLoad<double> cur, ref;
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return cpx<R>(cur(sp, t) - ref(sp, t)); }
+};
+
+template <typename R>
+struct TauRefDiffLoad {
+  TauRefLoad<double> cur, ref;
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return cpx<R>(cur(sp, t) - ref(sp, t)); }
+};
+
+// Sigma increment per vertex pair: acc (+)= dV~ * (Fr + Fd) + FVr * Fd, where
+// dV~ is the transformed vertex increment, Fr/Fd the transformed reference /
+// increment of G_ab and FVr the transformed reference vertex of the pair.
+template <typename R>
+struct SigmaIncStore {
+  cpx<R>* acc;
+  const cpx<R>* Fr;
+  const cpx<R>* Fd;
+  const cpx<R>* FVr;
+  int first;
+  unsigned S;
+  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const {
+    const size_t ad = sp + (size_t)S * t;
+    const cpx<R> fd = Fd[ad];
+    cpx<R> p = v * (Fr[ad] + fd) + FVr[ad] * fd;
+    acc[ad] = first ? p : acc[ad] + p;
+  }
+  __device__ bool skip(unsigned) const { return false; }
+};
+
+// acc = (first ? 0 : acc) + v * F
+template <typename R>
+struct MulAccStore {
+  cpx<R>* acc;
+  const cpx<R>* F;
+  int first;
+  unsigned S;
+  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const {
+    const size_t ad = sp + (size_t)S * t;
+    cpx<R> p = v * F[ad];
+    acc[ad] = first ? p : acc[ad] + p;
+  }
+  __device__ bool skip(unsigned) const { return false; }
+};
+
+// dst(k, t) (+)= w * v + J * sawc(t) for the flag-0 point of irreducible k and
+// slots t < Nw (fchi.f90:601-609, fself.f90:258-267); other slots are dropped
+template <typename R>
+struct GatherStore {
+  zd* dst;
+  double w;
+  zd J;
+  const zd* sawc;
+  int acc;
+  MeshDev m;
+  const zd* base = nullptr;  // optional: dst = base + w v (FP64 reference + increment)
+  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const {
+    if ((int)t >= m.Nw) return;
+    const long k = m.kgat_s[sp];
+    if (k < 0) return;
+    zd x = zd(v) * w;
+    if (sawc) x += J * sawc[t];
+    const long o = k + m.Nk * (long)t;
+    if (base) x += base[o];
+    zd* d = dst + o;
+    *d = acc ? *d + x : x;
+  }
+  __device__ bool skip(unsigned t) const { return (int)t >= m.Nw; }
+};
+
 // acc (+)= A * B
 template <typename R>
 void mul_acc(cpx<R>* acc, const cpx<R>* A, const cpx<R>* B, bool first, long n, hipStream_t s);
-
-// ---- gathers
-// dst(k, j) (+)= weight * grid(sp(k), j) + J * sawc(j)   for j < Nw; dst is a (Nk,Nw) slice
-template <typename R>
-void gather(zd* dst, const cpx<R>* grid, double weight, bool accumulate, zd J, const zd* sawc,
-            const Mesh& m, hipStream_t s);
 
 // ---- elementwise
 void conj_copy(zd* dst, const zd* src, long n, hipStream_t s);

@@ -45,7 +45,21 @@ void Mesh::build(const int64_t* kmap, const int64_t* invk, long Nkall_, long Nk_
         long rx = (Nx - x) % Nx, ry = (Ny - y) % Ny, rz = (Nz - z) % Nz;
         hrev[sp_] = (int)(rx + Nx * (ry + Ny * rz));
       }
+  std::vector<int> hirr_s(Nkall), hflag_s(Nkall), hkgat_s(Nkall, -1);
+  for (long i = 0; i < Nkall; ++i) {
+    hirr_s[hsp[i]] = hirr[i];
+    hflag_s[hsp[i]] = hflag[i];
+  }
+  complete = true;
+  for (long k = 0; k < Nk; ++k) {
+    if (hirrsp[k] >= 0) hkgat_s[hirrsp[k]] = (int)k;
+    else complete = false;
+  }
   irr.alloc(Nkall); flag.alloc(Nkall); sp.alloc(Nkall); rev.alloc(Nkall); irr_sp.alloc(Nk); kw.alloc(Nk);
+  irr_s.alloc(Nkall); flag_s.alloc(Nkall); kgat_s.alloc(Nkall);
+  irr_s.upload(hirr_s.data(), Nkall, s);
+  flag_s.upload(hflag_s.data(), Nkall, s);
+  kgat_s.upload(hkgat_s.data(), Nkall, s);
   irr.upload(hirr.data(), Nkall, s);
   flag.upload(hflag.data(), Nkall, s);
   sp.upload(hsp.data(), Nkall, s);
@@ -55,180 +69,8 @@ void Mesh::build(const int64_t* kmap, const int64_t* invk, long Nkall_, long Nk_
   GMOD_HIP(hipStreamSynchronize(s));
 }
 
-struct MeshView {
-  const int *irr, *flag, *sp, *rev, *irr_sp;
-  long Nkall, Nk;
-  int Nw, M;
-};
-static MeshView view(const Mesh& m) {
-  return {m.irr.p, m.flag.p, m.sp.p, m.rev.p, m.irr_sp.p, m.Nkall, m.Nk, m.Nw, m.M};
-}
-
-// ------------------------------------------------------------------ scatters
-// G_ab on the full grid (fchi.f90:563-583, fself.f90:226-236):
-//   slot t <  Nw : w_t ;  k direct -> G(k)_ab, k = -k_irr -> G(k_irr)_ba
-//   slot t >= Nw : -w_j with j = M-1-t ; direct -> conj G_ba, reversed -> conj G_ab
-template <typename R>
-__global__ void k_scatter_green(cpx<R>* __restrict__ grid, const zd* __restrict__ G, int a, int b, int No,
-                                MeshView m) {
-  const long N = m.Nkall * m.M;
-  const long NkNw = m.Nk * (long)m.Nw;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < N; g += (long)gridDim.x * blockDim.x) {
-    long i = g % m.Nkall;
-    int t = (int)(g / m.Nkall);
-    int k = m.irr[i];
-    int f = m.flag[i];
-    zd v;
-    if (t < m.Nw) {
-      int p = f == 0 ? a : b, q = f == 0 ? b : a;
-      v = G[k + m.Nk * (long)t + NkNw * (p + (long)No * q)];
-    } else {
-      int j = m.M - 1 - t;
-      int p = f == 0 ? b : a, q = f == 0 ? a : b;
-      v = conj(G[k + m.Nk * (long)j + NkNw * (p + (long)No * q)]);
-    }
-    grid[m.sp[i] + m.Nkall * (long)t] = cpx<R>(v);
-  }
-}
-
-template <typename R>
-void scatter_green(cpx<R>* grid, const zd* G, int a, int b, int No, const Mesh& m, hipStream_t s) {
-  hipLaunchKernelGGL((k_scatter_green<R>), dim3(nblocks(m.N())), dim3(TPB), 0, s, grid, G, a, b, No, view(m));
-  GMOD_CHECK_LAUNCH();
-}
-
-// bosonic V_(m,l) (fself.f90:203-224, feliash.f90:543-564): slot 0 = nu_0,
-// slot Nw = nu->inf value `stat`, slots t and M-t = +-nu_t
-template <typename R>
-__global__ void k_scatter_boson(cpx<R>* __restrict__ grid, const zd* __restrict__ V, int mi, int li, int Nchi,
-                                zd stat, MeshView m) {
-  const long N = m.Nkall * m.M;
-  const long NkNw = m.Nk * (long)m.Nw;
-  const long off_ml = NkNw * (mi + (long)Nchi * li);
-  const long off_lm = NkNw * (li + (long)Nchi * mi);
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < N; g += (long)gridDim.x * blockDim.x) {
-    long i = g % m.Nkall;
-    int t = (int)(g / m.Nkall);
-    int k = m.irr[i];
-    int f = m.flag[i];
-    zd v;
-    if (t == m.Nw) {
-      v = stat;
-    } else if (t < m.Nw) {
-      v = V[k + m.Nk * (long)t + (f == 0 ? off_ml : off_lm)];
-    } else {
-      int j = m.M - t;
-      v = conj(V[k + m.Nk * (long)j + (f == 0 ? off_lm : off_ml)]);
-    }
-    grid[m.sp[i] + m.Nkall * (long)t] = cpx<R>(v);
-  }
-}
-
-template <typename R>
-void scatter_boson(cpx<R>* grid, const zd* V, int mi, int li, int Nchi, zd stat, const Mesh& m, hipStream_t s) {
-  hipLaunchKernelGGL((k_scatter_boson<R>), dim3(nblocks(m.N())), dim3(TPB), 0, s, grid, V, mi, li, Nchi, stat,
-                     view(m));
-  GMOD_CHECK_LAUNCH();
-}
-
-// anomalous F_(a,b) with parity (feliash.f90:566-577)
-template <typename R>
-__global__ void k_scatter_anom(cpx<R>* __restrict__ grid, const zd* __restrict__ F, int a, int b, int No,
-                               double pp, double sgn, double dprt, MeshView m) {
-  const long N = m.Nkall * m.M;
-  const long NkNw = m.Nk * (long)m.Nw;
-  const long oab = NkNw * (a + (long)No * b), oba = NkNw * (b + (long)No * a);
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < N; g += (long)gridDim.x * blockDim.x) {
-    long i = g % m.Nkall;
-    int t = (int)(g / m.Nkall);
-    int k = m.irr[i];
-    int f = m.flag[i];
-    zd v;
-    if (t < m.Nw) {
-      v = F[k + m.Nk * (long)t + oab];
-      if (f != 0) v = v * (pp * dprt);
-    } else {
-      int j = m.M - 1 - t;
-      if (f == 0) v = conj(F[k + m.Nk * (long)j + oab]) * (sgn * dprt * pp);
-      else v = F[k + m.Nk * (long)j + oba] * sgn;
-    }
-    grid[m.sp[i] + m.Nkall * (long)t] = cpx<R>(v);
-  }
-}
-
-template <typename R>
-void scatter_anom(cpx<R>* grid, const zd* F, int a, int b, int No, double pp, double sgn, double dprt,
-                  const Mesh& m, hipStream_t s) {
-  hipLaunchKernelGGL((k_scatter_anom<R>), dim3(nblocks(m.N())), dim3(TPB), 0, s, grid, F, a, b, No, pp, sgn, dprt,
-                     view(m));
-  GMOD_CHECK_LAUNCH();
-}
-
-// sum_n U_an conj(U_bn) g_n(k, +-tau_s); -k uses the conjugated coefficient
-// (fchi.f90:703-729, 791-812).  g(+tau) = -e^{-xi tau}(1-f), g(-tau) = e^{xi tau} f,
-// both written in the overflow-safe form of the reference.
-template <typename R>
-__global__ void k_scatter_tau_ref(cpx<R>* __restrict__ grid, const double* __restrict__ eig,
-                                  const zd* __restrict__ uni, int a, int b, int No, double mu, double temp,
-                                  int which, MeshView m) {
-  const long N = m.Nkall * m.M;
-  const double beta = 1.0 / temp;
-  const double dtau = beta / (double)m.M;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < N; g += (long)gridDim.x * blockDim.x) {
-    long i = g % m.Nkall;
-    int s = (int)(g / m.Nkall);
-    int k = m.irr[i];
-    int f = m.flag[i];
-    double tau = (double)s * dtau;
-    zd acc(0, 0);
-    for (int n = 0; n < No; ++n) {
-      double xi = eig[n + (long)No * k] - mu;
-      double gv;
-      if (xi >= 0.0) {
-        double d = 1.0 + exp(-beta * xi);
-        gv = which == 0 ? -exp(-xi * tau) / d : exp(-xi * (beta - tau)) / d;
-      } else {
-        double d = 1.0 + exp(beta * xi);
-        gv = which == 0 ? -exp(xi * (beta - tau)) / d : exp(xi * tau) / d;
-      }
-      zd ua = uni[a + (long)No * (n + (long)No * k)];
-      zd ub = uni[b + (long)No * (n + (long)No * k)];
-      zd c = ua * conj(ub);
-      if (f != 0) c = conj(c);
-      acc += c * gv;
-    }
-    grid[m.sp[i] + m.Nkall * (long)s] = cpx<R>(acc);
-  }
-}
-
-template <typename R>
-void scatter_tau_ref(cpx<R>* grid, const double* eig, const zd* uni, int a, int b, int No, double mu,
-                     double temp, int which, const Mesh& m, hipStream_t s) {
-  hipLaunchKernelGGL((k_scatter_tau_ref<R>), dim3(nblocks(m.N())), dim3(TPB), 0, s, grid, eig, uni, a, b, No, mu,
-                     temp, which, view(m));
-  GMOD_CHECK_LAUNCH();
-}
-
-// ------------------------------------------------------------------ products
-template <typename R>
-__global__ void k_product_rev(cpx<R>* __restrict__ out, const cpx<R>* __restrict__ A, const cpx<R>* __restrict__ B,
-                              R coef, int rev_t, int acc, MeshView m) {
-  const long N = m.Nkall * m.M;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < N; g += (long)gridDim.x * blockDim.x) {
-    long sp_ = g % m.Nkall;
-    long t = g / m.Nkall;
-    long rt = rev_t ? (t == 0 ? 0 : m.M - t) : t;
-    cpx<R> v = A[g] * B[m.rev[sp_] + m.Nkall * rt] * coef;
-    out[g] = acc ? out[g] + v : v;
-  }
-}
-
-template <typename R>
-void product_rev(cpx<R>* out, const cpx<R>* A, const cpx<R>* B, R coef, bool rev_t, bool accumulate,
-                 const Mesh& m, hipStream_t s) {
-  hipLaunchKernelGGL((k_product_rev<R>), dim3(nblocks(m.N())), dim3(TPB), 0, s, out, A, B, coef, (int)rev_t,
-                     (int)accumulate, view(m));
-  GMOD_CHECK_LAUNCH();
+MeshDev mesh_dev(const Mesh& m) {
+  return {m.irr_s.p, m.flag_s.p, m.kgat_s.p, m.rev.p, m.Nkall, m.Nk, m.Nw, m.M};
 }
 
 template <typename R>
@@ -243,29 +85,6 @@ __global__ void k_mul_acc(cpx<R>* __restrict__ acc, const cpx<R>* __restrict__ A
 template <typename R>
 void mul_acc(cpx<R>* acc, const cpx<R>* A, const cpx<R>* B, bool first, long n, hipStream_t s) {
   hipLaunchKernelGGL((k_mul_acc<R>), dim3(nblocks(n)), dim3(TPB), 0, s, acc, A, B, (int)first, n);
-  GMOD_CHECK_LAUNCH();
-}
-
-// ------------------------------------------------------------------ gather
-template <typename R>
-__global__ void k_gather(zd* __restrict__ dst, const cpx<R>* __restrict__ grid, double w, int acc, zd J,
-                         const zd* __restrict__ sawc, MeshView m) {
-  const long n = m.Nk * (long)m.Nw;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    long k = e % m.Nk;
-    long j = e / m.Nk;
-    int sp_ = m.irr_sp[k];
-    zd v = sp_ >= 0 ? zd(grid[sp_ + m.Nkall * j]) * w : zd(0, 0);
-    if (sawc) v += J * sawc[j];
-    dst[e] = acc ? dst[e] + v : v;
-  }
-}
-
-template <typename R>
-void gather(zd* dst, const cpx<R>* grid, double weight, bool accumulate, zd J, const zd* sawc, const Mesh& m,
-            hipStream_t s) {
-  hipLaunchKernelGGL((k_gather<R>), dim3(nblocks(m.Nk * (long)m.Nw)), dim3(TPB), 0, s, dst, grid, weight,
-                     (int)accumulate, J, sawc, view(m));
   GMOD_CHECK_LAUNCH();
 }
 
@@ -444,17 +263,8 @@ double min_real(const zd* a, long n, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ instantiations
-#define INST(R)                                                                                          \
-  template void scatter_green<R>(cpx<R>*, const zd*, int, int, int, const Mesh&, hipStream_t);           \
-  template void scatter_boson<R>(cpx<R>*, const zd*, int, int, int, zd, const Mesh&, hipStream_t);       \
-  template void scatter_anom<R>(cpx<R>*, const zd*, int, int, int, double, double, double, const Mesh&,  \
-                                hipStream_t);                                                            \
-  template void scatter_tau_ref<R>(cpx<R>*, const double*, const zd*, int, int, int, double, double, int, \
-                                   const Mesh&, hipStream_t);                                            \
-  template void product_rev<R>(cpx<R>*, const cpx<R>*, const cpx<R>*, R, bool, bool, const Mesh&,        \
-                               hipStream_t);                                                             \
-  template void mul_acc<R>(cpx<R>*, const cpx<R>*, const cpx<R>*, bool, long, hipStream_t);              \
-  template void gather<R>(zd*, const cpx<R>*, double, bool, zd, const zd*, const Mesh&, hipStream_t);
+#define INST(R) \
+  template void mul_acc<R>(cpx<R>*, const cpx<R>*, const cpx<R>*, bool, long, hipStream_t);
 INST(double)
 INST(float)
 

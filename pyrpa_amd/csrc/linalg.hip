@@ -141,6 +141,201 @@ k_rpa_vertex(zd* chi, BatchLayout L, long batch, int n, const double* S, const d
   }
 }
 
+// ---------------------------------------------------------------------- register LU
+// Column-per-lane variant for Nchi = N in {1,4,9,16,25}.  Each solve uses 2N
+// lanes: lane c < N holds column c of M = I -+ chi0 K (K = S or C), lane
+// N + c holds column c of the right-hand side chi0 K.  The LU with partial
+// pivoting runs on the M lanes (multipliers broadcast through LDS); L, U and
+// the pivots are then published in LDS and every RHS lane solves its column
+// locally.  When 4N <= 64 both solves of one (q, i nu) run side by side in a
+// wavefront (several matrices per wavefront for small N); otherwise the two
+// solves run one after the other.  Matrices of consecutive (q, i nu) are
+// staged through LDS so the global loads and stores are coalesced.
+template <int N>
+struct VCfg {
+  static constexpr bool PAR = 4 * N <= 64;
+  static constexpr int LPM = PAR ? 4 * N : 2 * N;     // lanes per matrix
+  static constexpr int MPW = PAR ? 64 / (4 * N) : 1;  // matrices per wavefront
+  static constexpr int WAVES = PAR ? 4 : 2;
+  static constexpr int MPB = MPW * WAVES;             // matrices per block
+  static constexpr int THREADS = 64 * WAVES;
+};
+
+template <int N>
+__global__ void __launch_bounds__(VCfg<N>::THREADS, 2)
+k_vertex_reg(zd* __restrict__ chi, long batch, long es, const double* __restrict__ S, const double* __restrict__ C,
+             int mode, int* err) {
+  using Cf = VCfg<N>;
+  constexpr int NN = N * N;
+  constexpr int MPB = Cf::MPB;
+  __shared__ double2 Abuf_[MPB * NN];
+  __shared__ double2 Vbuf_[MPB * NN];
+  __shared__ double2 LUb_[MPB * 2 * NN];
+  __shared__ double2 bc_[MPB * 2 * N];
+  __shared__ double2 rinv_[MPB * 2 * N];
+  __shared__ int piv[MPB * 2 * N];
+  __shared__ double Ks[3 * NN];
+  zd* rinv = reinterpret_cast<zd*>(rinv_);
+  zd* Abuf = reinterpret_cast<zd*>(Abuf_);
+  zd* Vbuf = reinterpret_cast<zd*>(Vbuf_);
+  zd* LUb = reinterpret_cast<zd*>(LUb_);
+  zd* bc = reinterpret_cast<zd*>(bc_);
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int mw = lane / Cf::LPM;  // matrix within the wavefront
+  const int lm = lane % Cf::LPM;
+  const int mb = wave * Cf::MPW + mw;  // matrix within the block
+  const bool inmat = mw < Cf::MPW;
+  const double* Sg = Ks;
+  const double* Cg = Ks + NN;
+  const double* Wg = Ks + 2 * NN;  // S + C
+  for (int e = tid; e < NN; e += Cf::THREADS) { Ks[e] = S[e]; Ks[NN + e] = C[e]; Ks[2 * NN + e] = S[e] + C[e]; }
+  __syncthreads();
+  const double a1 = mode == 0 ? 1.5 : mode == 1 ? 1.5 : -0.5;  // coefficient of S X_s
+  const double a2 = mode == 0 ? 0.5 : -0.5;                     // coefficient of C X_c
+  for (long b0 = (long)blockIdx.x * MPB; b0 < batch; b0 += (long)gridDim.x * MPB) {
+    for (int idx = tid; idx < MPB * NN; idx += Cf::THREADS) {
+      int m = idx % MPB, e = idx / MPB;
+      long b = b0 + m;
+      Abuf[m * NN + e] = b < batch ? chi[b + (long)e * es] : zd(0, 0);
+    }
+    __syncthreads();
+    // static part, and for V_sigma -0.25 (S+C) chi0 (S+C), block-wide
+    if (mode == 0) {
+      for (int idx = tid; idx < MPB * NN; idx += Cf::THREADS) {
+        int m = idx / NN, e = idx % NN, p = e % N, q = e / N;
+        zd acc(0, 0);
+        for (int r = 0; r < N; ++r) acc += Abuf[m * NN + p + N * r] * Wg[r + N * q];
+        LUb[m * 2 * NN + e] = acc;
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < MPB * NN; idx += Cf::THREADS) {
+      int m = idx / NN, e = idx % NN, p = e % N, q = e / N;
+      double st = mode == 0 ? 1.5 * Sg[e] - 0.5 * Cg[e] : mode == 1 ? 0.5 * (Sg[e] + Cg[e]) : 0.5 * (Sg[e] - Cg[e]);
+      zd v(st, 0);
+      if (mode == 0) {
+        zd acc(0, 0);
+        for (int r = 0; r < N; ++r) acc += LUb[m * 2 * NN + r + N * q] * Wg[p + N * r];
+        v -= acc * 0.25;
+      }
+      Vbuf[idx] = v;
+    }
+    __syncthreads();
+    const zd* A = Abuf + mb * NN;
+    for (int pass = 0; pass < (Cf::PAR ? 1 : 2); ++pass) {
+      const int sv = Cf::PAR ? lm / (2 * N) : pass;
+      const int c = Cf::PAR ? lm % (2 * N) : lm;
+      const bool act = inmat && (Cf::PAR || lm < 2 * N);
+      const int slot = mb * 2 + sv;
+      const double* K = sv == 0 ? Sg : Cg;
+      const int j = c % N;
+      zd col[N];
+#pragma unroll
+      for (int p = 0; p < N; ++p) col[p] = zd(0, 0);
+      if (act) {
+        for (int r = 0; r < N; ++r) {
+          const double kr = K[r + N * j];
+#pragma unroll
+          for (int p = 0; p < N; ++p) col[p] += A[p + N * r] * kr;
+        }
+        if (c < N) {
+#pragma unroll
+          for (int p = 0; p < N; ++p) {
+            if (sv == 0) col[p] = zd(0, 0) - col[p];
+            if (p == j) col[p] += zd(1.0, 0);
+          }
+        }
+      }
+      // ---- LU with partial pivoting; the RHS lanes are eliminated alongside
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        if (act && c == k) {
+          int p = k;
+          double best = abs1(col[k]);
+#pragma unroll
+          for (int r = k + 1; r < N; ++r) {
+            double v = abs1(col[r]);
+            if (v > best) { best = v; p = r; }
+          }
+          if (best == 0.0) atomicOr(err, 1);
+          piv[slot * N + k] = p;
+        }
+        __syncthreads();
+        if (act && c >= k) {
+          const int p = piv[slot * N + k];
+#pragma unroll
+          for (int r = k + 1; r < N; ++r)
+            if (r == p) { zd t = col[k]; col[k] = col[r]; col[r] = t; }
+          if (c == k) {
+            const zd d = col[k];
+            const zd inv = (d.re == 0.0 && d.im == 0.0) ? zd(0, 0) : zd(1.0, 0) / d;
+            rinv[slot * N + k] = inv;
+#pragma unroll
+            for (int r = k + 1; r < N; ++r) {
+              col[r] = col[r] * inv;
+              bc[slot * N + r] = col[r];
+            }
+          }
+        }
+        __syncthreads();
+        if (act && c > k) {
+#pragma unroll
+          for (int r = k + 1; r < N; ++r) col[r] -= bc[slot * N + r] * col[k];
+        }
+      }
+      if (act && c < N) {
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+          if (r <= c) LUb[slot * NN + r + N * c] = col[r];
+      }
+      __syncthreads();
+      // ---- RHS lanes: back substitution with U (reciprocal pivots from LDS)
+      if (act && c >= N) {
+        const zd* LU = LUb + slot * NN;
+        const zd* ri = rinv + slot * N;
+#pragma unroll
+        for (int k = N - 1; k >= 0; --k) {
+          col[k] = col[k] * ri[k];
+#pragma unroll
+          for (int r = 0; r < k; ++r) col[r] -= LU[r + N * k] * col[k];
+        }
+      }
+      // ---- add this solve's share K X to column q of the vertex (solves in turn)
+      for (int turn = 0; turn < 2; ++turn) {
+        if (act && c >= N && sv == turn) {
+          const int q = c - N;
+          const double ax = sv == 0 ? a1 : a2;
+          zd* vb = Vbuf + mb * NN + N * q;
+          for (int p = 0; p < N; ++p) {
+            zd acc(0, 0);
+#pragma unroll
+            for (int r = 0; r < N; ++r) acc += col[r] * K[p + N * r];
+            vb[p] += acc * ax;
+          }
+        }
+        if (Cf::PAR) __syncthreads();
+      }
+      if (!Cf::PAR) __syncthreads();
+    }
+    for (int idx = tid; idx < MPB * NN; idx += Cf::THREADS) {
+      int m = idx % MPB, e = idx / MPB;
+      long b = b0 + m;
+      if (b < batch) chi[b + (long)e * es] = Vbuf[m * NN + e];
+    }
+    __syncthreads();
+  }
+}
+
+template <int N>
+static void launch_vertex_reg(zd* chi, long batch, long es, const double* S, const double* C, int mode, int* err,
+                              hipStream_t s) {
+  long nb = (batch + VCfg<N>::MPB - 1) / VCfg<N>::MPB;
+  unsigned grid = (unsigned)std::min<long>(nb, 256L * 32);
+  hipLaunchKernelGGL((k_vertex_reg<N>), dim3(grid), dim3(VCfg<N>::THREADS), 0, s, chi, batch, es, S, C, mode, err);
+  GMOD_CHECK_LAUNCH();
+}
+
 static void check_lds(size_t bytes) {
   if (bytes > 160 * 1024) throw Error("Nchi too large for the LDS-resident solver");
 }
@@ -148,6 +343,15 @@ static void check_lds(size_t bytes) {
 void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, const double* C, int mode, int* err,
                 hipStream_t s) {
   if (batch <= 0) return;
+  if (L.bs == 1 && !getenv("GMOD_VERTEX_LDS")) {
+    switch (n) {
+      case 1: launch_vertex_reg<1>(chi, batch, L.es, S, C, mode, err, s); return;
+      case 4: launch_vertex_reg<4>(chi, batch, L.es, S, C, mode, err, s); return;
+      case 9: launch_vertex_reg<9>(chi, batch, L.es, S, C, mode, err, s); return;
+      case 16: launch_vertex_reg<16>(chi, batch, L.es, S, C, mode, err, s); return;
+      default: break;
+    }
+  }
   size_t shm = 2 * n * n * sizeof(double) + 5 * n * n * sizeof(zd);
   check_lds(shm);
   unsigned grid = (unsigned)std::min<long>(batch, 256L * 64);

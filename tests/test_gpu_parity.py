@@ -76,9 +76,10 @@ def test_chi0_tail_exact_at_q0(F):
     assert abs(cn[0, 0, 0, 0] - ex0) < 1e-10
 
 
-def test_rpa_vertices_and_susceptibilities(F):
-    st = MODELS["3orb_4x4x2_nw6"]()
-    chi = O.get_chi0_conv(st['Gk'], st['kmap'], st['invk'], st['olist'], st['temp'], 4, 4, 2)
+@pytest.mark.parametrize("name", ["1orb_4x4_nw16", "2orb_4x4_nw8", "3orb_4x4x2_nw6", "4orb_2x2x4_nw4"])
+def test_rpa_vertices_and_susceptibilities(F, name):
+    st = MODELS[name]()
+    chi = O.get_chi0_conv(st['Gk'], st['kmap'], st['invk'], st['olist'], st['temp'], st['Nx'], st['Ny'], st['Nz'])
     V = F.get_Vsigma_nosoc_flex(chi.copy(), st['Smat'], st['Cmat'])
     assert rel(V, O.get_Vsigma_nosoc_flex(chi, st['Smat'], st['Cmat'])) < TOL
     a = F.get_chis_chic(chi, st['Smat'], st['Cmat'])
@@ -168,15 +169,18 @@ def test_lin_eliash_matches_oracle(F, arnoldi_m):
     assert rel(ca, O.get_V_delta(chi, st['Smat'], st['Cmat'])) < TOL  # chi leaves as V_Delta
 
 
-def test_lin_eliash_multiorbital_arnoldi(F):
+@pytest.mark.parametrize("gap_sym", [1, -1])
+def test_lin_eliash_multiorbital_arnoldi(F, gap_sym):
     st = MD.two_orbital(Nx=4, Ny=4, Nw=4, temp=0.1, mu=0.0, U=1.0, J=0.2)
     chi, _ = O.get_chi0(st['Smat'], st['Cmat'], st['Gk'], st['olist'], st['kmap'], st['invk'], st['temp'], 4, 4, 1)
-    init = O.gap_symms(st['klist'], 2, 1)
+    init = O.gap_symms(st['klist'], 2, gap_sym)
     args = (st['Gk'], st['uni'], init, st['Smat'], st['Cmat'], st['olist'], st['plist'], st['kmap'], st['invk'],
-            4, 4, 1, st['temp'], 1)
-    _, la = F.linearized_eliashberg(chi.copy(), *args, arnoldi_m=6)
+            4, 4, 1, st['temp'], gap_sym)
+    ca = chi.copy()
+    _, la = F.linearized_eliashberg(ca, *args, arnoldi_m=6)
     _, lb = O.linearized_eliashberg(chi.copy(), *args, arnoldi_m=6)
     assert abs(la - lb) < 1e-8 * max(1, abs(lb))
+    assert rel(ca, O.get_V_delta(chi, st['Smat'], st['Cmat'], singlet=gap_sym >= 0)) < TOL
 
 
 def test_lin_eliash_zero_interaction_has_zero_lambda(F):
