@@ -143,12 +143,14 @@ int gmod_get_precision(void);
 void gmod_set_sigma_seed(const gmod_z* sigma, double mu, double mu_old, int64_t count);
 /* Statistics of the last mkself call: [iterations, esterr, stoner, mu,
  * seconds, seconds in chi0, seconds in vertex, seconds in sigma, seconds in
- * mu search + Dyson]; returns the number of values written. */
+ * mu search + Dyson, FP32-increment iterations, device-resident solve seconds
+ * (inputs in HBM to converged Sigma and mu, no PCIe copies)]; returns the
+ * number of values written. */
 int gmod_flex_stats(double* out, int n);
 /* Kernel profiling with hipEvents on the library stream.  gmod_profile(1)
  * resets and enables; gmod_profile_read returns, per kernel class
- * [fft_pass, rpa_vertex, dyson, scatter, product, gather]: ms, launches,
- * algorithmic bytes; 3 values per class. */
+ * [fft_pass (FP64), rpa_vertex, dyson, scatter, product, gather,
+ * fft_pass_fp32]: ms, launches, algorithmic bytes; 3 values per class. */
 void gmod_profile(int on);
 int gmod_profile_read(double* out, int n);
 void gmod_set_verbose(int level);

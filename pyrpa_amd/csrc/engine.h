@@ -36,19 +36,24 @@ class ConvEngine {
   void tail_constants(double temp, const double* eig_h, const zd* uni_h, const int64_t* invk, double mu,
                       std::vector<zd>& J, DBuf<zd>& sawc_d, hipStream_t s) const;
 
-  // ---- increments around a reference state (mixed precision FLEX)
-  // Transforms of the reference G (and, with the tail, G0 and the tau kernels at mu_ref).
+  // ---- rolling increments (mixed-precision FLEX): the transforms of the
+  // previous iterate are kept at grid precision and advanced by increments.
+  // Transforms of G (and, with the tail, G0 and the tau kernels at mu).
   void set_reference(const zd* Gref, const zd* G0ref, bool tail, const double* eig_d, const zd* uni_d, double mu_ref,
                      double temp, hipStream_t s);
-  // Transforms of the vertex reference for all Nchi^2 pairs (full grid incl. the nu->inf slot).
+  // Transforms of the vertex for all Nchi^2 pairs (full grid incl. the nu->inf slot).
   void set_vertex_reference(const zd* Vref, const double* S_h, const double* C_h, hipStream_t s);
-  // dchi = chi0[G] - chi0[Gref] (+ tail terms at mu vs mu_ref), with the reference's
-  // partner/self-partner conjugation applied to the increment.
-  void chi0_increment(zd* dchi, const zd* G, const zd* Gref, const zd* G0, const zd* G0ref, double temp, bool tail,
-                      const double* eig_d, const zd* uni_d, double mu, double mu_ref, const std::vector<zd>& dJ,
-                      const zd* sawc_d, hipStream_t s);
-  // sig = sig_ref + (Sigma[V, G] - Sigma[Vref, Gref]) (no threshold)
-  void sigma_increment(zd* sig, const zd* sig_ref, const zd* V, const zd* Vref, double temp, hipStream_t s);
+  // chi_acc += chi0[G] - chi0[Gprev] (tail: + the G0 and tau-reference terms at mu vs
+  // mu_prev and dJ sawc), with the reference's partner / self-partner conjugation.
+  // Leaves the transformed increments in Fd (F0d, TAd, TBd).
+  void chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, const zd* G0, const zd* G0prev, double temp,
+                      bool tail, const double* eig_d, const zd* uni_d, double mu, double mu_prev,
+                      const std::vector<zd>& dJ, const zd* sawc_d, hipStream_t s);
+  // sig_acc += Sigma[V, G] - Sigma[Vprev, Gprev] (unthresholded); uses Fr/Fd of the
+  // preceding chi0_increment and advances FVr to V.
+  void sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, double temp, hipStream_t s);
+  // Fr += Fd (and the tail grids): the current iterate becomes the reference
+  void advance(bool tail, hipStream_t s);
   // Eliashberg: cache FFT[V_Delta] of the qualifying pairs (feliash.f90:540)
   void prepare_gap(const zd* Vd, const double* S_h, const double* C_h, bool singlet, hipStream_t s);
   // newdelta (un-weighted, Hermitised) <- sum V_Delta (*) F  (feliash.f90:486-635)

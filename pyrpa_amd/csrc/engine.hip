@@ -328,6 +328,15 @@ void ConvEngine<R>::gap(zd* nd, const zd* F, const double* prt, bool singlet, hi
 }
 
 // ====================================================================== increments
+// Rolling increments for the mixed-precision FLEX loop.  Between iterations
+// n-1 and n the bubble and the self-energy change by
+//   dB = B(Gr + dG) - B(Gr)  = dG13 (Gr42 + dG42) + Gr13 dG42      (per product)
+//   dS = S(Vr + dV, Gr + dG) - S(Vr, Gr) = dV (Gr + dG) + Vr dG
+// with Gr, Vr the previous iterate.  dG and dV are formed in FP64 in k-space
+// and only the increments are transformed at the grid precision, so the
+// rounding error of an update is proportional to the update itself and
+// vanishes as the loop converges; the transforms of the previous iterate
+// (Fr, F0r, TAr, TBr, FVr) advance by the transformed increments.
 template <typename R>
 void ConvEngine<R>::set_reference(const zd* Gref, const zd* G0ref, bool tail, const double* eig_d, const zd* uni_d,
                                   double mu_ref, double temp, hipStream_t s) {
@@ -375,9 +384,9 @@ void ConvEngine<R>::set_vertex_reference(const zd* Vref, const double* S_h, cons
 }
 
 template <typename R>
-void ConvEngine<R>::chi0_increment(zd* dchi, const zd* G, const zd* Gref, const zd* G0, const zd* G0ref, double temp,
-                                   bool tail, const double* eig_d, const zd* uni_d, double mu, double mu_ref,
-                                   const std::vector<zd>& dJ, const zd* sawc_d, hipStream_t s) {
+void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, const zd* G0, const zd* G0prev,
+                                   double temp, bool tail, const double* eig_d, const zd* uni_d, double mu,
+                                   double mu_prev, const std::vector<zd>& dJ, const zd* sawc_d, hipStream_t s) {
   const int No = op.No, Nchi = op.Nchi;
   const long NkNw = mesh.Nk * (long)mesh.Nw;
   const double w = temp / (double)mesh.Nkall;
@@ -388,70 +397,75 @@ void ConvEngine<R>::chi0_increment(zd* dchi, const zd* G, const zd* Gref, const 
   for (int b = 0; b < No; ++b)
     for (int a = 0; a < No; ++a) {
       cpx<R>* g = grid(Fd, a, b);
-      GreenDiffLoad<R> dl{GreenLoad<double>{G, a, b, No, md}, GreenLoad<double>{Gref, a, b, No, md}};
+      GreenDiffLoad<R> dl{GreenLoad<double>{G, a, b, No, md}, GreenLoad<double>{Gprev, a, b, No, md}};
       p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
     }
+  DBuf<zd> slice;
   if (tail) {
     const double beta = 1.0 / temp, dtau = beta / (double)mesh.M;
     for (int b = 0; b < No; ++b)
       for (int a = 0; a < No; ++a) {
         cpx<R>* g = grid(F0d, a, b);
-        GreenDiffLoad<R> dl{GreenLoad<double>{G0, a, b, No, md}, GreenLoad<double>{G0ref, a, b, No, md}};
+        GreenDiffLoad<R> dl{GreenLoad<double>{G0, a, b, No, md}, GreenLoad<double>{G0prev, a, b, No, md}};
         p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
         for (int which = 0; which < 2; ++which) {
           cpx<R>* t = grid(which == 0 ? TAd : TBd, a, b);
           TauRefDiffLoad<R> tl{TauRefLoad<double>{eig_d, uni_d, a, b, No, which, mu, beta, dtau, md},
-                               TauRefLoad<double>{eig_d, uni_d, a, b, No, which, mu_ref, beta, dtau, md}};
+                               TauRefLoad<double>{eig_d, uni_d, a, b, No, which, mu_prev, beta, dtau, md}};
           p3->exec_fused(tl, GridStore<R>{t, S}, t, tmp.p, -1, (R)1, s);
         }
       }
   }
-  if (!mesh.complete) GMOD_HIP(hipMemsetAsync(dchi, 0, NkNw * Nchi * Nchi * sizeof(zd), s));
+  for (auto& lm : op.irr) {
+    auto pr = op.partner[lm[1] + Nchi * lm[0]];
+    if (pr[0] == lm[1] && pr[1] == lm[0] && slice.n != (size_t)NkNw) slice.alloc(NkNw);
+  }
   for (auto& lm : op.irr) {
     const int l = lm[0], m = lm[1];
     const int a13 = op.ol[l][0], b13 = op.ol[m][0], a42 = op.ol[m][1], b42 = op.ol[l][1];
     auto pr = op.partner[m + Nchi * l];
-    const bool self_tail = tail && pr[0] == m && pr[1] == l;
-    // B(x, y) = x13 * y42(-r,-tau); increment of B(G,G): B(d,r) + B(r,d) + B(d,d)
-    auto add_bilinear = [&](ProductLoad<R>& pl, DBuf<cpx<R>>& Rf, DBuf<cpx<R>>& Df, DBuf<cpx<R>>* Rf2,
-                            DBuf<cpx<R>>* Df2, R c, int revt) {
-      DBuf<cpx<R>>& R2 = Rf2 ? *Rf2 : Rf;
-      DBuf<cpx<R>>& D2 = Df2 ? *Df2 : Df;
-      pl.add(grid(Df, a13, b13), grid(R2, a42, b42), c, revt);
-      pl.add(grid(Rf, a13, b13), grid(D2, a42, b42), c, revt);
-      pl.add(grid(Df, a13, b13), grid(D2, a42, b42), c, revt);
-    };
-    ProductLoad<R> pl{};
+    const bool self = pr[0] == m && pr[1] == l;
+    const bool self_tail = tail && self;
+    IncProductLoad<R> pl{};
     pl.m = md;
-    pl.nterm = 0;
-    add_bilinear(pl, Fr, Fd, nullptr, nullptr, (R)(tail ? -w : -1.0), 1);
+    pl.add(grid(Fr, a13, b13), grid(Fd, a13, b13), grid(Fr, a42, b42), grid(Fd, a42, b42), (R)(tail ? -w : -1.0), 1);
     if (tail) {
-      if (!self_tail) add_bilinear(pl, F0r, F0d, nullptr, nullptr, (R)w, 1);
-      add_bilinear(pl, TAr, TAd, &TBr, &TBd, (R)(-wref), 0);
+      if (!self_tail)
+        pl.add(grid(F0r, a13, b13), grid(F0d, a13, b13), grid(F0r, a42, b42), grid(F0d, a42, b42), (R)w, 1);
+      pl.add(grid(TAr, a13, b13), grid(TAd, a13, b13), grid(TBr, a42, b42), grid(TBd, a42, b42), (R)(-wref), 0);
     }
-    zd J(0, 0);
-    if (tail) {
-      J = dJ[l + Nchi * m];
-    }
-    zd* dst = dchi + NkNw * (m + (long)Nchi * l);
-    p4->exec_fused(pl, GatherStore<R>{dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 0, md}, work.p, tmp.p, +1,
-                   invN, s);
+    const zd J = tail ? dJ[l + Nchi * m] : zd(0, 0);
+    zd* dst = chi_acc + NkNw * (m + (long)Nchi * l);
     if (self_tail) {
-      conj_copy(dst, dst, NkNw, s);
-      ProductLoad<R> p0{};
+      // increment of conj(conv[G] + ref) - conv[G0], formed apart and then added
+      if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
+      p4->exec_fused(pl, GatherStore<R>{slice.p, 1.0, J, sawc_d, 0, md}, work.p, tmp.p, +1, invN, s);
+      conj_copy(slice.p, slice.p, NkNw, s);
+      IncProductLoad<R> p0{};
       p0.m = md;
-      p0.nterm = 0;
-      add_bilinear(p0, F0r, F0d, nullptr, nullptr, (R)w, 1);
-      p4->exec_fused(p0, GatherStore<R>{dst, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1, invN, s);
+      p0.add(grid(F0r, a13, b13), grid(F0d, a13, b13), grid(F0r, a42, b42), grid(F0d, a42, b42), (R)w, 1);
+      p4->exec_fused(p0, GatherStore<R>{slice.p, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1, invN, s);
+      axpy(dst, slice.p, zd(1, 0), NkNw, s);
       continue;
     }
-    if (pr[0] >= 0) conj_copy(dchi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
+    if (self) {
+      // the reference conjugates a self-partner pair in place after the
+      // accumulation (fchi.f90:610): the stored value is conj(conv[G]), so the
+      // increment is conjugated before it is added
+      if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
+      p4->exec_fused(pl, GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p, +1, invN, s);
+      conj_copy(slice.p, slice.p, NkNw, s);
+      axpy(dst, slice.p, zd(1, 0), NkNw, s);
+      continue;
+    }
+    p4->exec_fused(pl, GatherStore<R>{dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, md}, work.p, tmp.p, +1,
+                   invN, s);
+    if (pr[0] >= 0) conj_copy(chi_acc + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
 }
 
 template <typename R>
-void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_ref, const zd* V, const zd* Vref, double temp,
-                                    hipStream_t s) {
+void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, double temp, hipStream_t s) {
   const int No = op.No, Nchi = op.Nchi;
   const long NkNw = mesh.Nk * (long)mesh.Nw;
   if (acc.n != (size_t)N) acc.alloc(N);
@@ -466,24 +480,29 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_ref, const zd* V, con
         for (int m = 0; m < Nchi; ++m) {
           if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
           const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
-          BosonDiffLoad<R> bl{BosonLoad<double>{V, oml, olm, zd(0, 0), md}, BosonLoad<double>{Vref, oml, olm, zd(0, 0), md}};
+          BosonDiffLoad<R> bl{BosonLoad<double>{V, oml, olm, zd(0, 0), md},
+                              BosonLoad<double>{Vprev, oml, olm, zd(0, 0), md}};
           const int a = op.ol[m][1], b = op.ol[l][1];
           SigmaIncStore<R> st{acc.p, grid(Fr, a, b), grid(Fd, a, b), FVr.p + (size_t)(m + Nchi * l) * N,
                               first ? 1 : 0, S};
           p4->exec_fused(bl, st, work.p, tmp.p, -1, (R)1, s);
           first = false;
         }
-      zd* dst = sig + NkNw * (m1 + (long)No * l1);
-      const zd* base = sig_ref + NkNw * (m1 + (long)No * l1);
-      if (first) {
-        copy(dst, base, NkNw, s);
-        continue;
-      }
-      if (!mesh.complete) copy(dst, base, NkNw, s);
-      GatherStore<R> gs{dst, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md};
-      gs.base = base;
-      p4->exec_fused(GridLoad<R>{acc.p, S}, gs, acc.p, tmp.p, +1, invN, s);
+      if (first) continue;
+      zd* dst = sig_acc + NkNw * (m1 + (long)No * l1);
+      p4->exec_fused(GridLoad<R>{acc.p, S}, GatherStore<R>{dst, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 1, md},
+                     acc.p, tmp.p, +1, invN, s);
     }
+}
+
+template <typename R>
+void ConvEngine<R>::advance(bool tail, hipStream_t s) {
+  const long nb = (long)op.No * op.No * N;
+  add_inplace(Fr.p, Fd.p, nb, s);
+  if (!tail) return;
+  add_inplace(F0r.p, F0d.p, nb, s);
+  add_inplace(TAr.p, TAd.p, nb, s);
+  add_inplace(TBr.p, TBd.p, nb, s);
 }
 
 template class ConvEngine<double>;

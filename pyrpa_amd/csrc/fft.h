@@ -251,7 +251,7 @@ void FFTPlan<R>::launch(const FFTPassDesc& p, LD ld, ST st, const cpx<R>* tw, R 
   a.loglines = fft_is_pow2(p.lines) ? fft_ilog2(p.lines) : -1;
   const long blocks = (a.total_lines + p.lines - 1) / p.lines;
   const size_t shm = (size_t)p.lines * a.pitch * sizeof(cpx<R>);
-  ProfScope ps(PROF_FFT, 2.0 * (double)size() * sizeof(cpx<R>), s);
+  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, 2.0 * (double)size() * sizeof(cpx<R>), s);
   if (fft_is_pow2(p.n))
     hipLaunchKernelGGL((fft_pass_kernel<R, true, LD, ST>), dim3((unsigned)blocks), dim3(FFT_THREADS), shm, s, ld, st,
                        a, tw + p.tw_off, tw + p.tw4_off, sc);

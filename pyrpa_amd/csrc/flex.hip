@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <fstream>
+#include <memory>
 
 #include "../../include/gmod.h"
 #include "engine.h"
@@ -129,8 +130,14 @@ struct FlexIn {
   bool sw_out, sw_in, sw_rescale, sw_tail;
 };
 
+// Precision modes: R = double, mixed = false: FP64 throughout (the reference's
+// arithmetic).  R = float, mixed = false: FP32 transforms of the full fields.
+// R = double, mixed = true: the first `head` iterations in FP64, then FP32
+// transforms of rolling increments (ConvEngine::chi0_increment /
+// sigma_increment) around FP64 accumulators of chi0 and Sigma; vertex, Dyson,
+// mu search, mixing and the convergence test stay FP64.
 template <typename R>
-static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out) {
+static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mixed) {
   auto t_start = clk::now();
   hipStream_t s = ctx().stream;
   const int verbose = ctx().verbose;
@@ -156,9 +163,28 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out) {
   err.zero(s);
   std::vector<DBuf<zd>> xout(A.m_diis), res(A.m_diis);
   for (long i = 0; i < A.m_diis; ++i) { xout[i].alloc(nS); res[i].alloc(nS); }
-  ConvEngine<R> eng(mesh, op);
+  std::unique_ptr<ConvEngine<R>> eng(new ConvEngine<R>(mesh, op));
+  std::unique_ptr<ConvEngine<float>> e32;
+  // mixed-precision state: FP64 accumulators and the previous iterate
+  DBuf<zd> chi_acc, Vprev, sig_acc, Gprev, G0, G0prev, sawc_d;
+  std::vector<zd> Jprev, Jcur, dJ;
+  double mu_prev = 0.0;
+  long head = 1;
+  if (const char* e = getenv("GMOD_MIXED_HEAD")) head = std::max(1L, atol(e));
+  if (mixed) {
+    chi_acc.alloc(nC);
+    Vprev.alloc(nC);
+    sig_acc.alloc(nS);
+    Gprev.alloc(nS);
+    if (A.sw_tail) { G0.alloc(nS); G0prev.alloc(nS); }
+  }
+  int n_inc = 0;
 
   double mu = A.mu_init, mu_old = 0.0;
+  // device-resident solve time: from the inputs resident in HBM to the
+  // converged Sigma and mu (excludes the host<->device copies of the ABI)
+  GMOD_HIP(hipStreamSynchronize(s));
+  auto t_dev = clk::now();
   if (A.sw_in) {
     std::vector<zd> h;
     if (g_seed.valid) {
@@ -186,6 +212,8 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out) {
       }
     }
     sig.upload(h.data(), nS, s);
+    GMOD_HIP(hipStreamSynchronize(s));
+    t_dev = clk::now();
     if (A.sigma_scale != 1.0) scale(sig.p, nS, A.sigma_scale, s);
     copy(sig0.p, sig.p, nS, s);
     dyson(G.p, sig.p, ham_d.p, mu, T, Nk, (int)Nw, No, err.p, s);
@@ -294,8 +322,22 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out) {
     iters = (int)it;
     if (verbose) printf("iter=%5ld\n", it);
     auto t0 = clk::now();
-    eng.transform_green(G.p, false, s);
-    eng.chi0(chi.p, T, A.sw_tail, eig_d.p, uni_d.p, A.eig, A.uni, A.invk, mu, s);
+    const bool inc = mixed && e32 != nullptr;
+    if (!inc) {
+      eng->transform_green(G.p, false, s);
+      eng->chi0(mixed ? chi_acc.p : chi.p, T, A.sw_tail, eig_d.p, uni_d.p, A.eig, A.uni, A.invk, mu, s, !mixed);
+      if (mixed) threshold_copy(chi.p, chi_acc.p, nC, 1e-9, s);
+    } else {
+      if (A.sw_tail) {
+        green0(G0.p, eig_d.p, uni_d.p, mu, T, Nk, (int)Nw, No, s);
+        e32->tail_constants(T, A.eig, A.uni, A.invk, mu, Jcur, sawc_d, s);
+        dJ.resize(Jcur.size());
+        for (size_t i = 0; i < Jcur.size(); ++i) dJ[i] = Jcur[i] - Jprev[i];
+      }
+      e32->chi0_increment(chi_acc.p, G.p, Gprev.p, G0.p, G0prev.p, T, A.sw_tail, eig_d.p, uni_d.p, mu, mu_prev, dJ,
+                          sawc_d.p, s);
+      threshold_copy(chi.p, chi_acc.p, nC, 1e-9, s);
+    }
     stoner = stoner_factor(chi.p, Nk, Nw, Nchi, S_d.p, s);
     t_chi += secs(t0);
     if (verbose) printf("SDW max eig(chi0 S) = %12.8f\n", stoner);
@@ -314,7 +356,38 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out) {
     if (verbose)
       printf("Re V_sigma: max:%12.4E min:%12.4E\n", max_real(chi.p, nC, s), min_real(chi.p, nC, s));
     t0 = clk::now();
-    eng.sigma(sig.p, chi.p, A.Smat, A.Cmat, T, s);
+    if (!inc) {
+      eng->sigma(mixed ? sig_acc.p : sig.p, chi.p, A.Smat, A.Cmat, T, s, !mixed);
+      if (mixed) {
+        threshold_copy(sig.p, sig_acc.p, nS, 1e-9, s);
+        copy(Vprev.p, chi.p, nC, s);
+        copy(Gprev.p, G.p, nS, s);
+        mu_prev = mu;
+        if (A.sw_tail) {
+          green0(G0prev.p, eig_d.p, uni_d.p, mu, T, Nk, (int)Nw, No, s);
+          eng->tail_constants(T, A.eig, A.uni, A.invk, mu, Jprev, sawc_d, s);
+        }
+        if (it >= head) {
+          if (verbose) printf("[FLEX] switching to FP32 increments\n");
+          eng.reset();  // FP64 grids are no longer needed
+          e32.reset(new ConvEngine<float>(mesh, op));
+          e32->set_reference(Gprev.p, G0prev.p, A.sw_tail, eig_d.p, uni_d.p, mu_prev, T, s);
+          e32->set_vertex_reference(Vprev.p, A.Smat, A.Cmat, s);
+        }
+      }
+    } else {
+      e32->sigma_increment(sig_acc.p, chi.p, Vprev.p, T, s);
+      e32->advance(A.sw_tail, s);
+      threshold_copy(sig.p, sig_acc.p, nS, 1e-9, s);
+      std::swap(Vprev, chi);
+      std::swap(Gprev, G);  // G is rewritten by the Dyson step below
+      if (A.sw_tail) {
+        std::swap(G0prev, G0);
+        Jprev = Jcur;
+      }
+      mu_prev = mu;
+      ++n_inc;
+    }
     if (A.sub_sigma > 0) subtract_static(sig.p, Nk, (int)Nw, No, (int)A.sub_sigma, s);
     // ---- compare_sigma
     i_hist = i_hist % m + 1;
@@ -364,6 +437,7 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out) {
     t_mu += secs(t0);
   }
   check_err(err, s, "mkself");
+  const double t_solve = secs(t_dev);
   if (stoner >= 1.0 && verbose) printf("[FLEX] WARNING: Stoner factor >= 1 at the final iteration\n");
   std::vector<zd> h(nS);
   sig.download(h.data(), nS, s);
@@ -378,8 +452,9 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out) {
     double meta[4] = {T, (double)Nw, (double)No, (double)Nk};
     write_record(f, meta, 32);
   }
-  double st[9] = {(double)iters, esterr, stoner, mu, secs(t_start), t_chi, t_vert, t_sig, t_mu};
-  std::copy(st, st + 9, g_stats);
+  double st[11] = {(double)iters, esterr, stoner,  mu,       secs(t_start),  t_chi,
+                   t_vert,        t_sig,  t_mu,    (double)n_inc, t_solve};
+  std::copy(st, st + 11, g_stats);
 }
 
 // ---------------------------------------------------------------- Eliashberg
@@ -629,7 +704,7 @@ extern "C" void get_chi0(gmod_z* chi, const double* Smat, const double* Cmat, co
   (void)Cmat;
   OrbPairs op;
   op.from_olist(olist, (int)*Nchi, (int)*Norb);
-  if (ctx().precision == 1)
+  if (ctx().precision >= 1)
     chi0_driver<float>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
                        false, nullptr, nullptr, 0.0, Smat, maxchi0s_out);
   else
@@ -648,7 +723,7 @@ extern "C" void get_chi0_tail(gmod_z* chi, const double* Smat, const double* Cma
   (void)Cmat;
   OrbPairs op;
   op.from_olist(olist, (int)*Nchi, (int)*Norb);
-  if (ctx().precision == 1)
+  if (ctx().precision >= 1)
     chi0_driver<float>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
                        true, eig, (const zd*)uni, *mu, Smat, maxchi0s_out);
   else
@@ -667,7 +742,7 @@ extern "C" void get_chi0_conv_(gmod_z* chi, const gmod_z* Gk, const int64_t* kma
   OrbPairs op;
   op.from_olist(olist, (int)*Nchi, (int)*Norb);
   op.from_maps(irr_chi, chi_map);
-  if (ctx().precision == 1)
+  if (ctx().precision >= 1)
     chi0_driver<float>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, (int)*Norb,
                        false, nullptr, nullptr, 0.0, nullptr, nullptr);
   else
@@ -861,8 +936,9 @@ extern "C" void mkself(gmod_z* sigmak, double* mu, const double* Smat, const dou
   A.sub_sigma = *sub_sigma; A.m_diis = *m_diis;
   A.No = (int)*Norb; A.Nchi = (int)*Nchi;
   A.sw_out = *sw_out; A.sw_in = *sw_in; A.sw_rescale = *sw_rescale; A.sw_tail = *sw_tail;
-  if (ctx().precision == 1) mkself_driver<float>(A, (zd*)sigmak, mu);
-  else mkself_driver<double>(A, (zd*)sigmak, mu);
+  if (ctx().precision == 1) mkself_driver<double>(A, (zd*)sigmak, mu, true);
+  else if (ctx().precision == 2) mkself_driver<float>(A, (zd*)sigmak, mu, false);
+  else mkself_driver<double>(A, (zd*)sigmak, mu, false);
   GMOD_CATCH
 }
 
@@ -876,7 +952,7 @@ extern "C" void gmod_set_sigma_seed(const gmod_z* sigma, double mu, double mu_ol
 }
 
 extern "C" int gmod_flex_stats(double* out, int n) {
-  int m = std::min(n, 9);
+  int m = std::min(n, 11);
   std::copy(g_stats, g_stats + m, out);
   return m;
 }
@@ -900,7 +976,7 @@ extern "C" void lin_eliash(gmod_z* delta, gmod_z* chi, const gmod_z* Gk, const g
   A.itemax = *itemax; A.gap_sym = *gap_sym; A.arnoldi_m = *arnoldi_m;
   A.No = (int)*Norb; A.Nchi = (int)*Nchi;
   *lambda_out = 0.0;
-  if (ctx().precision == 1) *lambda_out = eliash_driver<float>(A, (zd*)delta);
+  if (ctx().precision >= 1) *lambda_out = eliash_driver<float>(A, (zd*)delta);
   else *lambda_out = eliash_driver<double>(A, (zd*)delta);
   GMOD_CATCH
 }

@@ -134,7 +134,9 @@ Context& ctx();
 // ----------------------------------------------------------------- profiling
 // hipEvent pairs around kernel launches on the library stream, enabled by
 // gmod_profile(1); per class: elapsed ms, launches, algorithmic bytes.
-enum ProfClass { PROF_FFT = 0, PROF_VERTEX, PROF_DYSON, PROF_SCATTER, PROF_PRODUCT, PROF_GATHER, PROF_NCLASS };
+enum ProfClass {
+  PROF_FFT = 0, PROF_VERTEX, PROF_DYSON, PROF_SCATTER, PROF_PRODUCT, PROF_GATHER, PROF_FFT32, PROF_NCLASS
+};
 bool prof_enabled();
 struct ProfScope {
   int cls;

@@ -186,22 +186,58 @@ struct TauRefDiffLoad {
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return cpx<R>(cur(sp, t) - ref(sp, t)); }
 };
 
-// Sigma increment per vertex pair: acc (+)= dV~ * (Fr + Fd) + FVr * Fd, where
-// dV~ is the transformed vertex increment, Fr/Fd the transformed reference /
-// increment of G_ab and FVr the transformed reference vertex of the pair.
+// Increment of a bilinear bubble B(X) = X13(r,t) X42(-r, -t or t) between a
+// reference X = Xr and X + Xd, summed over up to three terms:
+//   v = sum_i coef_i [ Ad13 (Br42 + Bd42) + Ar13 Bd42 ]
+// (the products of fchi.f90:593, 824 expanded to first + second order in Xd)
+constexpr int INC_MAXT = 3;
+template <typename R>
+struct IncProductLoad {
+  const cpx<R>* Ar[INC_MAXT];
+  const cpx<R>* Ad[INC_MAXT];
+  const cpx<R>* Br[INC_MAXT];
+  const cpx<R>* Bd[INC_MAXT];
+  R coef[INC_MAXT];
+  int revt[INC_MAXT];
+  int nterm;
+  MeshDev m;
+  __host__ void add(const cpx<R>* ar, const cpx<R>* ad, const cpx<R>* br, const cpx<R>* bd, R c, int rt) {
+    if (nterm >= INC_MAXT) throw Error("too many increment terms");
+    Ar[nterm] = ar; Ad[nterm] = ad; Br[nterm] = br; Bd[nterm] = bd; coef[nterm] = c; revt[nterm] = rt; ++nterm;
+  }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const {
+    const size_t ad = sp + (size_t)m.Nkall * t;
+    const unsigned rs = (unsigned)m.rev[sp];
+    const unsigned rtt = t == 0 ? 0u : (unsigned)m.M - t;
+    cpx<R> v(0, 0);
+    for (int i = 0; i < nterm; ++i) {
+      const size_t bd = rs + (size_t)m.Nkall * (revt[i] ? rtt : t);
+      const cpx<R> bdv = Bd[i][bd];
+      v += (Ad[i][ad] * (Br[i][bd] + bdv) + Ar[i][ad] * bdv) * coef[i];
+    }
+    return v;
+  }
+};
+
+// Sigma increment per vertex pair: acc (+)= dV~ (Fr + Fd) + FVr Fd, where dV~
+// is the transformed vertex increment, Fr / Fd the transformed reference /
+// increment of G_ab and FVr the transformed reference vertex of the pair;
+// FVr then advances to the current vertex (FVr += dV~).
 template <typename R>
 struct SigmaIncStore {
   cpx<R>* acc;
   const cpx<R>* Fr;
   const cpx<R>* Fd;
-  const cpx<R>* FVr;
+  cpx<R>* FVr;
   int first;
   unsigned S;
   __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const {
     const size_t ad = sp + (size_t)S * t;
     const cpx<R> fd = Fd[ad];
-    cpx<R> p = v * (Fr[ad] + fd) + FVr[ad] * fd;
+    const cpx<R> fv = FVr[ad];
+    cpx<R> p = v * (Fr[ad] + fd) + fv * fd;
     acc[ad] = first ? p : acc[ad] + p;
+    FVr[ad] = fv + v;
   }
   __device__ bool skip(unsigned) const { return false; }
 };
@@ -257,6 +293,9 @@ void scale(zd* a, long n, double f, hipStream_t s);
 void axpy(zd* y, const zd* x, zd alpha, long n, hipStream_t s);      // y += alpha x
 void copy(zd* dst, const zd* src, long n, hipStream_t s);
 void lincomb(zd* y, const zd* a, double ca, const zd* b, double cb, long n, hipStream_t s);  // y = ca a + cb b
+void threshold_copy(zd* dst, const zd* src, long n, double eps, hipStream_t s);              // dst = thr(src)
+template <typename R>
+void add_inplace(cpx<R>* y, const cpx<R>* x, long n, hipStream_t s);  // y += x
 
 // ---- deterministic reductions (two-stage, fixed order)
 // sum over k-space arrays laid out (Nk, rest): weights kw[k] (or 1)

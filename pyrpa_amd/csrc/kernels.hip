@@ -134,6 +134,30 @@ void copy(zd* dst, const zd* src, long n, hipStream_t s) {
   GMOD_HIP(hipMemcpyAsync(dst, src, n * sizeof(zd), hipMemcpyDeviceToDevice, s));
 }
 
+__global__ void k_threshold_copy(zd* dst, const zd* src, long n, double eps) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    zd v = src[e];
+    if (fabs(v.re) < eps) v.re = 0.0;
+    if (fabs(v.im) < eps) v.im = 0.0;
+    dst[e] = v;
+  }
+}
+void threshold_copy(zd* dst, const zd* src, long n, double eps, hipStream_t s) {
+  hipLaunchKernelGGL(k_threshold_copy, dim3(nblocks(n)), dim3(TPB), 0, s, dst, src, n, eps);
+  GMOD_CHECK_LAUNCH();
+}
+
+template <typename R>
+__global__ void k_add_inplace(cpx<R>* y, const cpx<R>* x, long n) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    y[e] = y[e] + x[e];
+}
+template <typename R>
+void add_inplace(cpx<R>* y, const cpx<R>* x, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_inplace<R>, dim3(nblocks(n)), dim3(TPB), 0, s, y, x, n);
+  GMOD_CHECK_LAUNCH();
+}
+
 __global__ void k_lincomb(zd* y, const zd* a, double ca, const zd* b, double cb, long n) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
     y[e] = a[e] * ca + b[e] * cb;
@@ -264,7 +288,8 @@ double min_real(const zd* a, long n, hipStream_t s) {
 
 // ------------------------------------------------------------------ instantiations
 #define INST(R) \
-  template void mul_acc<R>(cpx<R>*, const cpx<R>*, const cpx<R>*, bool, long, hipStream_t);
+  template void mul_acc<R>(cpx<R>*, const cpx<R>*, const cpx<R>*, bool, long, hipStream_t); \
+  template void add_inplace<R>(cpx<R>*, const cpx<R>*, long, hipStream_t);
 INST(double)
 INST(float)
 

@@ -101,18 +101,30 @@ struct ProfRec { int cls; double bytes; hipEvent_t a, b; };
 static bool g_prof = false;
 static std::vector<ProfRec> g_prof_recs;
 static double g_prof_acc[3 * PROF_NCLASS];
+static std::vector<hipEvent_t> g_event_pool;  // recycled events (creation is not free)
 
 bool prof_enabled() { return g_prof; }
 
+static hipEvent_t prof_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  GMOD_HIP(hipEventCreate(&e));
+  return e;
+}
+
 ProfScope::ProfScope(int c, double by, hipStream_t st) : cls(c), bytes(by), s(st) {
   if (!g_prof) return;
-  GMOD_HIP(hipEventCreate(&e0));
+  e0 = prof_event();
   GMOD_HIP(hipEventRecord(e0, s));
 }
 ProfScope::~ProfScope() {
   if (!g_prof || !e0) return;
-  hipEvent_t e1;
-  if (hipEventCreate(&e1) != hipSuccess) return;
+  hipEvent_t e1 = nullptr;
+  try { e1 = prof_event(); } catch (...) { return; }
   (void)hipEventRecord(e1, s);
   g_prof_recs.push_back({cls, bytes, e0, e1});
 }
@@ -126,8 +138,8 @@ static void prof_flush() {
     g_prof_acc[3 * r.cls] += ms;
     g_prof_acc[3 * r.cls + 1] += 1;
     g_prof_acc[3 * r.cls + 2] += r.bytes;
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
+    g_event_pool.push_back(r.a);
+    g_event_pool.push_back(r.b);
   }
   g_prof_recs.clear();
 }

@@ -390,13 +390,14 @@ def set_sigma_seed(sigmak, mu, mu_old):
 
 
 def flex_stats():
-    out = np.zeros(9)
-    _L().gmod_flex_stats(out, 9)
-    keys = ("iterations", "esterr", "stoner", "mu", "seconds", "t_chi0", "t_vertex", "t_sigma", "t_mu_dyson")
+    out = np.zeros(11)
+    _L().gmod_flex_stats(out, 11)
+    keys = ("iterations", "esterr", "stoner", "mu", "seconds", "t_chi0", "t_vertex", "t_sigma", "t_mu_dyson",
+            "increment_iterations", "solve_seconds")
     return dict(zip(keys, out.tolist()))
 
 
-PROFILE_CLASSES = ("fft_pass", "rpa_vertex", "dyson", "scatter", "product", "gather")
+PROFILE_CLASSES = ("fft_pass", "rpa_vertex", "dyson", "scatter", "product", "gather", "fft_pass_fp32")
 
 
 def profile(on=True):
@@ -412,3 +413,14 @@ def profile_read():
 
 def set_verbose(level):
     _L().gmod_set_verbose(int(level))
+
+
+def pool_bytes():
+    """Device memory held by the library's caching pool."""
+    return float(_L().gmod_pool_bytes())
+
+
+def release_memory():
+    """Return the unused blocks of the library's device pool to HIP."""
+    _L().gmod_release_memory()
+    check()

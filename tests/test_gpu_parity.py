@@ -134,6 +134,45 @@ def test_mkself_matches_oracle(F, case):
     assert abs(ma - mb) < 1e-8
 
 
+def _flex_case(case):
+    if case == "1orb_conv":
+        st, U, J, fill, kw = MD.one_orbital(Nx=8, Ny=8, Nw=32, temp=0.05), 1.5, 0.0, 0.4, {}
+    elif case == "1orb_tail":
+        st, U, J, fill, kw = MD.one_orbital(Nx=8, Ny=8, Nw=32, temp=0.05), 1.5, 0.0, 0.4, dict(sw_tail=True)
+    elif case == "2orb_diis_sub2":
+        st, U, J, fill, kw = MD.two_orbital(Nx=8, Ny=8, Nw=16, temp=0.05), 0.8, 0.1, 0.9, dict(m_diis=3, sub_sigma=2)
+    else:
+        st = MD.random_model(3, 4, 4, 2, 16, 0.1, 0.0, 0, 0, seed=7)
+        U, J, fill, kw = 0.6, 0.1, 1.2, dict(sw_tail=True, sw_rescale=True)
+    S, C = O.get_scmat(st['olist'], st['site'], U, J)
+    args = (S, C, st['kmap'], st['invk'], st['olist'], st['hamk'], st['eig'], st['uni'], 0.0, fill, st['temp'],
+            st['Nw'], st['Nx'], st['Ny'], st['Nz'], False, False)
+    return args, kw
+
+
+@pytest.mark.parametrize("case", ["1orb_conv", "1orb_tail", "2orb_diis_sub2", "3orb_3d_tail"])
+def test_mkself_mixed_precision_matches_fp64(F, case):
+    """FP32 rolling increments around FP64 accumulators: converges to the
+    reference criterion esterr < 1e-8 and lands on the FP64 fixed point
+    (max|dSigma| / max|Sigma| < 1e-6, |dmu| < 1e-6)."""
+    args, kw = _flex_case(case)
+    opts = dict(eps=1e-8, pp=0.5, m_diis=1, scf_loop=300)
+    opts.update(kw)
+    s64, m64 = F.mkself(*args, **opts)
+    n64 = F.flex_stats()['iterations']
+    try:
+        F.set_precision(1)
+        s32, m32 = F.mkself(*args, **opts)
+        st = F.flex_stats()
+    finally:
+        F.set_precision(0)
+    assert st['esterr'] < 1e-8
+    assert st['increment_iterations'] >= 1
+    assert st['iterations'] <= n64 + 3
+    assert rel(s32, s64) < 1e-6
+    assert abs(m32 - m64) < 1e-6
+
+
 def test_mkself_warm_start_from_memory_and_sigma_bin(F, tmp_path, monkeypatch):
     st = MD.one_orbital(Nx=4, Ny=4, Nw=16, temp=0.1)
     S, C = O.get_scmat(st['olist'], st['site'], 1.2, 0.0)
