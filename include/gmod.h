@@ -154,6 +154,16 @@ int gmod_flex_stats(double* out, int n);
 void gmod_profile(int on);
 int gmod_profile_read(double* out, int n);
 void gmod_set_verbose(int level);
+/* Host test hook (no reference counterpart; used by the CPU test suite only):
+ * runs the box-pass FFT plan of an (nx,ny,nz,m) grid (x fastest, interleaved
+ * complex, in place) on the host with the kernel's own load / sub-pass /
+ * store code and bounds checks.  mask selects the axes (bit 0: x .. bit 3:
+ * t), sign -1 forward / +1 normalised backward, precision 0 FP64 / 1 FP32,
+ * lds_bytes the per-work-group LDS limit to plan for; npasses receives the
+ * number of box passes.  Returns 0, 1 if the box engine does not cover the
+ * shape, 2 on an indexing error. */
+int gmod_box_fft_host(double* data, int nx, int ny, int nz, int m, int mask, int sign, int precision,
+                      long lds_bytes, int* npasses);
 /* Device memory held by the library's caching pool, in bytes. */
 double gmod_pool_bytes(void);
 void gmod_release_memory(void);

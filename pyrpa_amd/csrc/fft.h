@@ -23,6 +23,7 @@
 // the last backward pass works on lines of one fixed t and skips the slots a
 // gather does not keep.
 #pragma once
+#include "fftbox.h"
 #include "gmod_common.h"
 
 namespace gmod {
@@ -62,6 +63,9 @@ template <typename R> struct GridLoad {
 template <typename R> struct GridStore {
   cpx<R>* p;
   unsigned S;
+  struct Pre {};
+  __device__ Pre prep(unsigned, unsigned) const { return Pre{}; }
+  __device__ void commit(unsigned sp, unsigned t, const Pre&, cpx<R> v) const { p[sp + (size_t)S * t] = v; }
   __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { p[sp + (size_t)S * t] = v; }
   __device__ bool skip(unsigned) const { return false; }
 };
@@ -217,17 +221,22 @@ class FFTPlan {
   long size() const { return (long)dims_[0] * dims_[1] * dims_[2] * dims_[3]; }
   unsigned spatial() const { return S_; }
   long transformed_points() const;
-  int num_passes() const { return (int)fwd_.size(); }
+  int num_passes() const { return box_.ok ? (int)box_.fwd.size() : (int)fwd_.size(); }
+  bool uses_box() const { return box_.ok; }
 
  private:
   template <class LD, class ST>
   void launch(const FFTPassDesc& p, LD ld, ST st, const cpx<R>* tw, R sc, hipStream_t s) const;
+  template <class LD, class ST>
+  void launch_box(const BoxArgs& a, LD ld, ST st, const cpx<R>* tw, R sc, hipStream_t s) const;
   int dims_[4];
   int mask_;
   unsigned S_ = 1;
   std::vector<FFTPassDesc> fwd_, bwd_;
   cpx<R>* tw_[2] = {nullptr, nullptr};  // [0]: sign -1 tables, [1]: sign +1 tables
   long tw_len_ = 0;
+  BoxPlan<R> box_;                        // box passes (preferred when box_.ok)
+  cpx<R>* twb_[2] = {nullptr, nullptr};   // their twiddle pools
 };
 
 inline bool fft_is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -263,7 +272,37 @@ void FFTPlan<R>::launch(const FFTPassDesc& p, LD ld, ST st, const cpx<R>* tw, R 
 
 template <typename R>
 template <class LD, class ST>
+void FFTPlan<R>::launch_box(const BoxArgs& a, LD ld, ST st, const cpx<R>* tw, R sc, hipStream_t s) const {
+  const size_t shm = (size_t)a.lds_elems * sizeof(cpx<R>);
+  static bool attr_done = false;  // one per kernel instantiation: allow the device's whole LDS
+  if (!attr_done) {
+    GMOD_HIP(hipFuncSetAttribute((const void*)&box_fft_kernel<R, LD, ST>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)box_.lds_limit));
+    attr_done = true;
+  }
+  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, 2.0 * (double)size() * sizeof(cpx<R>), s);
+  hipLaunchKernelGGL((box_fft_kernel<R, LD, ST>), dim3((unsigned)a.blocks()), dim3(BOX_THREADS), shm, s, ld, st, a,
+                     tw, sc);
+  GMOD_CHECK_LAUNCH();
+}
+
+template <typename R>
+template <class LD, class ST>
 void FFTPlan<R>::exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R scale, hipStream_t s) const {
+  if (box_.ok) {
+    const cpx<R>* twb = twb_[sign < 0 ? 0 : 1];
+    const std::vector<BoxArgs>& bp = sign < 0 ? box_.fwd : box_.bwd;
+    const int nb = (int)bp.size();
+    for (int i = 0; i < nb; ++i) {
+      const R sc = (i == nb - 1) ? scale : (R)1;
+      const bool first = i == 0, last = i == nb - 1;
+      if (first && last) launch_box(bp[i], ld, st, twb, sc, s);
+      else if (first) launch_box(bp[i], ld, GridStore<R>{buf, S_}, twb, sc, s);
+      else if (last) launch_box(bp[i], GridLoad<R>{buf, S_}, st, twb, sc, s);
+      else launch_box(bp[i], GridLoad<R>{buf, S_}, GridStore<R>{buf, S_}, twb, sc, s);
+    }
+    return;
+  }
   const cpx<R>* tw = tw_[sign < 0 ? 0 : 1];
   const std::vector<FFTPassDesc>& ps = sign < 0 ? fwd_ : bwd_;
   const int np = (int)ps.size();

@@ -1,6 +1,8 @@
 // Hand-written batched FFT for gfx950: plan construction (pass lists for the
 // forward x,y,z,t and backward t,z,y,x orders), twiddle tables, plan cache.
 // The pass kernel and the fused-exec template live in fft.h.
+#include <algorithm>
+#include <cmath>
 #include <map>
 #include <tuple>
 
@@ -13,6 +15,196 @@ constexpr int MAX_SINGLE = 256;
 template <typename R> struct LdsBudget;
 template <> struct LdsBudget<double> { static constexpr int elems = 4096; };
 template <> struct LdsBudget<float> { static constexpr int elems = 8192; };
+
+// ---------------------------------------------------------------- box plans
+// radices of one dimension: powers of two up to maxr (balanced), then 7, 5, 3
+static bool box_factor(int n, int maxr, std::vector<int>& out) {
+  out.clear();
+  if (n <= 1) return true;
+  int e = 0, o = n;
+  while ((o & 1) == 0) { o >>= 1; ++e; }
+  std::vector<int> odd;
+  for (int p : {7, 5, 3})
+    while (o % p == 0) { odd.push_back(p); o /= p; }
+  if (o != 1) return false;
+  const int lg = fft_ilog2(maxr);
+  if (e > 0) {
+    const int k = (e + lg - 1) / lg;
+    const int base = e / k, extra = e % k;
+    for (int i = 0; i < k; ++i) out.push_back(1 << (base + (i < extra ? 1 : 0)));
+  }
+  for (int p : odd) out.push_back(p);
+  return (int)out.size() <= BOX_MAX_RAD;
+}
+
+static int largest_divisor_le(int n, long cap) {
+  if (cap < 1) return 0;
+  for (int d = (int)std::min<long>(n, cap); d >= 1; --d)
+    if (n % d == 0) return d;
+  return 1;
+}
+
+template <typename R>
+void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, size_t lds_bytes) {
+  ok = false;
+  fwd.clear();
+  bwd.clear();
+  const int n[4] = {nx, ny, nz, m};
+  const int maxr = sizeof(R) == 4 ? 32 : 16;
+  const long b = (long)sizeof(cpx<R>);
+  std::vector<int> tables;
+  auto add_table = [&](int len) -> int {
+    int off = 0;
+    for (int t : tables) {
+      if (t == len) return off;
+      off += t;
+    }
+    tables.push_back(len);
+    return off;
+  };
+  max_lds_bytes = 0;
+  lds_limit = lds_bytes;
+  auto make_box = [&](const int ext[4], const std::vector<int>& tdims, BoxArgs& a) -> bool {
+    a = BoxArgs{};
+    long elems = 1;
+    for (int d = 0; d < 4; ++d) {
+      if (ext[d] < 1 || n[d] % ext[d] != 0) return false;
+      a.box[d] = ext[d];
+      a.nblk[d] = n[d] / ext[d];
+      elems *= ext[d];
+    }
+    const int px = ext[0] + ((ext[0] > 1 && elems > ext[0]) ? 1 : 0);
+    a.lds_stride[0] = 1;
+    a.lds_stride[1] = px;
+    a.lds_stride[2] = px * ext[1];
+    a.lds_stride[3] = px * ext[1] * ext[2];
+    const long lds_elems = (long)px * ext[1] * ext[2] * ext[3];
+    if (lds_elems * b > (long)lds_bytes || elems > budget) return false;
+    a.lds_elems = (int)lds_elems;
+    a.elems = (int)elems;
+    a.nsub = 0;
+    for (int d : tdims) {
+      std::vector<int> rads;
+      if (!box_factor(n[d], maxr, rads)) return false;
+      a.nrad[d] = (int)rads.size();
+      int L = n[d], w = n[d];
+      for (size_t i = 0; i < rads.size(); ++i) {
+        const int r = rads[i];
+        w /= r;
+        a.rad[d][i] = r;
+        a.radw[d][i] = w;
+        if (a.nsub >= BOX_MAX_SUB) return false;
+        BoxSub sb;
+        sb.dim = d;
+        sb.radix = r;
+        sb.L = L;
+        sb.tw_off = L / r > 1 ? add_table(L) : 0;
+        sb.wr_off = add_table(r);
+        a.sub[a.nsub++] = sb;
+        L /= r;
+      }
+    }
+    a.Nx = (unsigned)nx;
+    a.Ny = (unsigned)ny;
+    a.S = (unsigned)((long)nx * ny * nz);
+    a.M = (unsigned)m;
+    // bytes contiguous in the grid per box row; boxes sharing a 128 B line
+    // are kept on one XCD
+    long seg = ext[0] * b;
+    if (ext[0] == nx) {
+      seg = (long)nx * ext[1] * b;
+      if (ext[1] == ny) seg = (long)nx * ny * ext[2] * b;
+    }
+    a.xcd_group = (seg < 128 && 128 % seg == 0) ? (int)(128 / seg) : 1;
+    a.src = a.dst = 0;
+    max_lds_bytes = std::max(max_lds_bytes, (size_t)(lds_elems * b));
+    return true;
+  };
+  std::vector<int> tx;
+  for (int d = 0; d < 4; ++d)
+    if (((mask >> d) & 1) && n[d] > 1) tx.push_back(d);
+  if (tx.empty()) return;
+  std::vector<int> sp;
+  for (int d : tx)
+    if (d < 3) sp.push_back(d);
+  std::vector<std::vector<int>> groups;
+  if (!sp.empty()) {
+    long vol = 1;
+    for (int d : sp) vol *= n[d];
+    if (vol <= budget) {
+      groups.push_back(sp);
+    } else {
+      std::vector<int> xy, rest;
+      for (int d : sp) (d < 2 ? xy : rest).push_back(d);
+      long vxy = 1;
+      for (int d : xy) vxy *= n[d];
+      if (xy.size() == 2 && vxy <= budget) {
+        groups.push_back(xy);
+        if (!rest.empty()) groups.push_back(rest);
+      } else {
+        for (int d : sp) groups.push_back({d});
+      }
+    }
+  }
+  std::vector<BoxArgs> spatial;
+  for (auto& gdims : groups) {
+    int ext[4] = {1, 1, 1, 1};
+    long vol = 1;
+    for (int d : gdims) { ext[d] = n[d]; vol *= n[d]; }
+    if (vol > budget) return;
+    if (ext[0] == 1) {  // keep rows contiguous: adjacent x columns in the box
+      ext[0] = largest_divisor_le(nx, std::min<long>(256 / b, budget / vol));
+      vol *= ext[0];
+    }
+    ext[3] = largest_divisor_le(m, budget / vol);
+    BoxArgs a;
+    if (!make_box(ext, gdims, a)) return;
+    spatial.push_back(a);
+  }
+  const bool do_t = std::find(tx.begin(), tx.end(), 3) != tx.end();
+  BoxArgs tb;
+  if (do_t) {
+    if (m > budget) return;
+    int ext[4] = {1, 1, 1, m};
+    ext[0] = largest_divisor_le(nx, budget / m);
+    if (ext[0] == nx) ext[1] = largest_divisor_le(ny, budget / ((long)m * nx));
+    if (ext[0] == nx && ext[1] == ny) ext[2] = largest_divisor_le(nz, budget / ((long)m * nx * ny));
+    if (!make_box(ext, {3}, tb)) return;
+  }
+  for (auto& a : spatial) fwd.push_back(a);
+  if (do_t) fwd.push_back(tb);
+  if (do_t) bwd.push_back(tb);
+  for (auto it = spatial.rbegin(); it != spatial.rend(); ++it) bwd.push_back(*it);
+  // twiddle pools: tw[k] = exp(sign 2 pi i k / len), computed in double
+  for (int sg = 0; sg < 2; ++sg) {
+    tw_host[sg].clear();
+    const double sign = sg == 0 ? -1.0 : 1.0;
+    for (int t : tables)
+      for (int k = 0; k < t; ++k) {
+        double c, si;
+        if (2 * k == t) { c = -1.0; si = 0.0; }
+        else if (4 * k == t) { c = 0.0; si = 1.0; }
+        else if (4 * k == 3 * t) { c = 0.0; si = -1.0; }
+        else { const double ang = 2.0 * M_PI * (double)k / (double)t; c = std::cos(ang); si = std::sin(ang); }
+        tw_host[sg].push_back(cpx<R>((R)c, (R)(sign * si)));
+      }
+  }
+  ok = true;
+}
+
+// box budget from the device's LDS limit: the largest power-of-two element
+// count whose padded box fits
+template <typename R>
+static void box_budget(long& budget, size_t& lds_bytes) {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+    v = 65536;
+  lds_bytes = (size_t)v;
+  long e = 1;
+  while ((e * 2) * (long)sizeof(cpx<R>) * 33 / 32 <= (long)v) e *= 2;
+  budget = e;
+}
 
 template <typename R>
 FFTPlan<R>::FFTPlan(int nx, int ny, int nz, int m, int axes_mask) {
@@ -101,6 +293,22 @@ FFTPlan<R>::FFTPlan(int nx, int ny, int nz, int m, int axes_mask) {
       if (lines > nl) lines = (int)nl;
       p.lines = lines;
     }
+  // box passes (two or three HBM round trips per transform) unless disabled
+  const char* eb = getenv("GMOD_FFT_BOX");
+  if (!eb || atoi(eb) != 0) {
+    long budget = 0;
+    size_t lds = 0;
+    box_budget<R>(budget, lds);
+    box_.build(nx, ny, nz, m, axes_mask, budget, lds);
+    if (box_.ok) {
+      for (int sg = 0; sg < 2; ++sg) {
+        const size_t nt = box_.tw_host[sg].size();
+        GMOD_HIP(hipMalloc(&twb_[sg], std::max<size_t>(nt, 1) * sizeof(cpx<R>)));
+        if (nt)
+          GMOD_HIP(hipMemcpy(twb_[sg], box_.tw_host[sg].data(), nt * sizeof(cpx<R>), hipMemcpyHostToDevice));
+      }
+    }
+  }
   // twiddle tables: tw[k] = exp(sign * 2 pi i k / len), computed in double
   long len = 0;
   for (int t : tables) len += t;
@@ -134,6 +342,8 @@ template <typename R>
 FFTPlan<R>::~FFTPlan() {
   for (auto p : tw_)
     if (p) (void)hipFree(p);
+  for (auto p : twb_)
+    if (p) (void)hipFree(p);
 }
 
 template <typename R>
@@ -157,9 +367,94 @@ const FFTPlan<R>& fft_plan(int nx, int ny, int nz, int m, int axes_mask) {
   return *p;
 }
 
+template struct BoxPlan<double>;
+template struct BoxPlan<float>;
 template class FFTPlan<double>;
 template class FFTPlan<float>;
 template const FFTPlan<double>& fft_plan<double>(int, int, int, int, int);
 template const FFTPlan<float>& fft_plan<float>(int, int, int, int, int);
 
+
+// ---------------------------------------------------------------- host test hook
+// Runs the box passes of a plan on the host (the same load / sub-pass / store
+// code as the kernel, one emulated thread, with bounds checks), so the CPU
+// test suite can check plans and indexing against a reference FFT without a
+// device.  Not used by any computation of the library.
+namespace {
+template <typename R>
+struct HostGridLoad {
+  const cpx<R>* p;
+  unsigned S;
+  cpx<R> operator()(unsigned sp, unsigned t) const { return p[sp + (size_t)S * t]; }
+};
+template <typename R>
+struct HostGridStore {
+  cpx<R>* p;
+  unsigned S;
+  struct Pre {};
+  Pre prep(unsigned, unsigned) const { return Pre{}; }
+  void commit(unsigned sp, unsigned t, const Pre&, cpx<R> v) const { p[sp + (size_t)S * t] = v; }
+  bool skip(unsigned) const { return false; }
+};
+
+template <typename R>
+int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign, long budget, size_t lds_bytes,
+                 int* npasses) {
+  BoxPlan<R> plan;
+  plan.build(nx, ny, nz, m, mask, budget, lds_bytes);
+  if (!plan.ok) return 1;
+  const long N = (long)nx * ny * nz * m;
+  const unsigned S = (unsigned)((long)nx * ny * nz);
+  std::vector<cpx<R>> g(N);
+  for (long i = 0; i < N; ++i) g[i] = cpx<R>((R)data[2 * i], (R)data[2 * i + 1]);
+  const std::vector<BoxArgs>& ps = sign < 0 ? plan.fwd : plan.bwd;
+  const cpx<R>* tw = plan.tw_host[sign < 0 ? 0 : 1].data();
+  long tpts = 1;
+  const int n[4] = {nx, ny, nz, m};
+  for (int d = 0; d < 4; ++d)
+    if ((mask >> d) & 1) tpts *= n[d];
+  for (size_t i = 0; i < ps.size(); ++i) {
+    const BoxArgs& a = ps[i];
+    const R scale = (sign > 0 && i + 1 == ps.size()) ? (R)(1.0 / (double)tpts) : (R)1;
+    std::vector<cpx<R>> lds(a.lds_elems);
+    const long nb = a.blocks();
+    std::vector<char> seen(nb, 0);
+    for (long blk = 0; blk < nb; ++blk) {
+      const unsigned lb = box_remap((unsigned)blk, (unsigned)nb, a.xcd_group);
+      if (lb >= (unsigned)nb || seen[lb]) throw Error("box remap is not a permutation");
+      seen[lb] = 1;
+      const BoxOrigin O = box_origin(a, lb);
+      box_load<R>(lds.data(), HostGridLoad<R>{g.data(), S}, a, O, 0, 1);
+      for (int j = 0; j < a.nsub; ++j) box_dispatch<R>(lds.data(), a, a.sub[j], tw, 0, 1);
+      box_store<R>(lds.data(), HostGridStore<R>{g.data(), S}, a, O, scale, 0, 1);
+    }
+  }
+  for (long i = 0; i < N; ++i) { data[2 * i] = (double)g[i].re; data[2 * i + 1] = (double)g[i].im; }
+  if (npasses) *npasses = (int)ps.size();
+  return 0;
+}
+}  // namespace
+
 }  // namespace gmod
+
+// Host emulation of the box-pass FFT of an (nx,ny,nz,m) grid (x fastest),
+// interleaved complex in place; precision 0: FP64, 1: FP32 arithmetic.
+// lds_bytes sets the box budget as the device's LDS limit would.  Returns 0,
+// 1 when the box engine does not cover the shape, 2 on an indexing error.
+extern "C" int gmod_box_fft_host(double* data, int nx, int ny, int nz, int m, int mask, int sign, int precision,
+                                 long lds_bytes, int* npasses) {
+  using namespace gmod;
+  try {
+    if (precision == 1) {
+      long e = 1;
+      while ((e * 2) * 8 * 33 / 32 <= lds_bytes) e *= 2;
+      return box_host_run<float>(data, nx, ny, nz, m, mask, sign, e, (size_t)lds_bytes, npasses);
+    }
+    long e = 1;
+    while ((e * 2) * 16 * 33 / 32 <= lds_bytes) e *= 2;
+    return box_host_run<double>(data, nx, ny, nz, m, mask, sign, e, (size_t)lds_bytes, npasses);
+  } catch (std::exception& ex) {
+    set_error(ex.what());
+    return 2;
+  }
+}

@@ -1,0 +1,322 @@
+// Box passes of the batched 4D FFT: one work-group loads a whole 4D box of
+// the grid into LDS, transforms every dimension of the box that spans the
+// grid (a sequence of in-place mixed-radix sub-passes with the butterflies in
+// registers), and stores the box.  A 4D transform is two box passes when the
+// spatial volume fits in LDS (a box of all Nx*Ny*Nz points x bt slots, then a
+// box of c adjacent spatial columns x all M slots), three when it does not
+// (x-y planes, z columns, t columns).  Every element crosses HBM once per
+// pass.  The per-axis passes of fft.h remain for shapes this engine does not
+// cover.
+//
+// In-place mixed-radix recursion along one dimension of length n = r1 r2 ...:
+// sub-pass i works on blocks of length L_i = n / (r1 ... r_{i-1}) with stride
+// s_i = L_i / r_i: a radix-r_i DFT over the elements j' + s_i j of a block,
+// then the twiddle w_{L_i}^(j' k), stored back in place at j' + s_i k.  The
+// output index k = k1 + r1 (k2 + r2 (k3 + ...)) then sits at position
+// sum_i k_i n / (r1 ... r_i), which only the store undoes.
+//
+// The three phases (load, sub-passes, store) are __host__ __device__ with an
+// explicit (thread, thread count) so that the host test hook runs the same
+// code, with bounds checks, on the CPU.
+#pragma once
+#include <vector>
+
+#include "gmod_common.h"
+
+namespace gmod {
+
+constexpr int BOX_THREADS = 512;
+constexpr int BOX_MAX_SUB = 10;
+constexpr int BOX_MAX_RAD = 4;  // sub-passes per dimension
+
+struct BoxSub {
+  int dim;      // 0:x 1:y 2:z 3:t
+  int radix;
+  int L;        // block length; stride s = L / radix
+  int tw_off;   // w_L^m table (sign-specific pool)
+  int wr_off;   // w_radix^m table
+};
+
+struct BoxArgs {
+  int box[4];      // box extent per dimension
+  int nblk[4];     // boxes per dimension
+  int lds_stride[4];
+  int lds_elems;   // LDS elements (with the x-row padding)
+  int elems;       // grid elements per box
+  int nsub;
+  BoxSub sub[BOX_MAX_SUB];
+  int nrad[4];                  // radices of a transformed dimension (0: not transformed)
+  int rad[4][BOX_MAX_RAD];
+  int radw[4][BOX_MAX_RAD];     // position weight n / (r1 ... r_i)
+  unsigned Nx, Ny, S, M;
+  int xcd_group;                // consecutive boxes that share cache lines (1: no remap)
+  int src, dst;                 // intermediate buffers (0: caller buffer, 1: scratch)
+  long blocks() const { return (long)nblk[0] * nblk[1] * nblk[2] * nblk[3]; }
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GMOD_BOX_CHECK(cond, what) ((void)0)
+#else
+#define GMOD_BOX_CHECK(cond, what) \
+  do {                             \
+    if (!(cond)) throw ::gmod::Error(std::string("box FFT index out of range: ") + (what)); \
+  } while (0)
+#endif
+
+__host__ __device__ inline unsigned box_remap(unsigned a, unsigned nb, int g) {
+  // consecutive logical boxes 0..g-1 of a group land on one XCD (the
+  // dispatcher deals work-groups round-robin over the 8 XCDs)
+  if (g <= 1 || (nb % (8u * (unsigned)g)) != 0) return a;
+  const unsigned xcd = a & 7u, slot = a >> 3;
+  return (unsigned)g * (8u * (slot / (unsigned)g) + xcd) + slot % (unsigned)g;
+}
+
+template <int N>
+struct ConstLog2 { static constexpr int v = 1 + ConstLog2<N / 2>::v; };
+template <>
+struct ConstLog2<1> { static constexpr int v = 0; };
+
+template <int N>
+__host__ __device__ constexpr int cbrev(int i) {
+  int r = 0;
+  for (int b = 0; b < ConstLog2<N>::v; ++b)
+    if (i & (1 << b)) r |= 1 << (ConstLog2<N>::v - 1 - b);
+  return r;
+}
+
+// In-register DFT of length RAD, natural order in and out; wr[m] = w_RAD^m.
+template <typename R, int RAD>
+__host__ __device__ __forceinline__ void box_dft(cpx<R> (&v)[RAD], const cpx<R>* __restrict__ wr) {
+  if constexpr ((RAD & (RAD - 1)) == 0) {
+    cpx<R> u[RAD];
+#pragma unroll
+    for (int i = 0; i < RAD; ++i) u[cbrev<RAD>(i)] = v[i];
+#pragma unroll
+    for (int len = 2; len <= RAD; len <<= 1) {
+#pragma unroll
+      for (int i = 0; i < RAD; i += len) {
+#pragma unroll
+        for (int j = 0; j < len / 2; ++j) {
+          cpx<R> a = u[i + j], b = u[i + j + len / 2];
+          if (j != 0) b = b * wr[j * (RAD / len)];
+          u[i + j] = a + b;
+          u[i + j + len / 2] = a - b;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RAD; ++i) v[i] = u[i];
+  } else {
+    cpx<R> u[RAD];
+#pragma unroll
+    for (int k = 0; k < RAD; ++k) {
+      cpx<R> acc = v[0];
+#pragma unroll
+      for (int j = 1; j < RAD; ++j) acc += v[j] * wr[(j * k) % RAD];
+      u[k] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < RAD; ++i) v[i] = u[i];
+  }
+}
+
+struct BoxOrigin {
+  int o[4];
+};
+
+__host__ __device__ inline BoxOrigin box_origin(const BoxArgs& a, unsigned b) {
+  BoxOrigin O;
+  unsigned r = b;
+  for (int d = 0; d < 3; ++d) {
+    O.o[d] = (int)(r % (unsigned)a.nblk[d]) * a.box[d];
+    r /= (unsigned)a.nblk[d];
+  }
+  O.o[3] = (int)r * a.box[3];
+  return O;
+}
+
+// ---- phase 1: grid -> LDS (x fastest: coalesced along the grid's contiguous axis)
+template <typename R, class LD>
+__host__ __device__ __forceinline__ void box_load(cpx<R>* lds, const LD& ld, const BoxArgs& a, const BoxOrigin& O,
+                                                  int tid, int nthr) {
+  // U loads in flight per thread before the first LDS store: one work-group
+  // per CU has to keep ~64 KB of HBM reads outstanding
+  constexpr int U = sizeof(R) == 4 ? 16 : 8;
+  const int ex = a.box[0], ey = a.box[1], ez = a.box[2];
+  for (int e0 = tid; e0 < a.elems; e0 += nthr * U) {
+    cpx<R> v[U];
+    int off[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * nthr;
+      off[u] = -1;
+      if (e < a.elems) {
+        int q = e;
+        const int ix = q % ex; q /= ex;
+        const int iy = q % ey; q /= ey;
+        const int iz = q % ez;
+        const int it = q / ez;
+        const unsigned x = (unsigned)(O.o[0] + ix), y = (unsigned)(O.o[1] + iy), z = (unsigned)(O.o[2] + iz);
+        const unsigned t = (unsigned)(O.o[3] + it);
+        const unsigned sp = x + a.Nx * (y + a.Ny * z);
+        GMOD_BOX_CHECK(sp < a.S && t < a.M, "load grid");
+        v[u] = ld(sp, t);
+        off[u] = ix * a.lds_stride[0] + iy * a.lds_stride[1] + iz * a.lds_stride[2] + it * a.lds_stride[3];
+        GMOD_BOX_CHECK(off[u] < a.lds_elems, "load lds");
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (off[u] >= 0) lds[off[u]] = v[u];
+  }
+}
+
+// ---- phase 2: one radix-RAD sub-pass over the whole box (in place in LDS)
+template <typename R, int RAD>
+__host__ __device__ __forceinline__ void box_subpass(cpx<R>* lds, const BoxArgs& a, const BoxSub& sp,
+                                                     const cpx<R>* __restrict__ tw, int tid, int nthr) {
+  const int d = sp.dim;
+  const int n = a.box[d];
+  const int L = sp.L, s = L / RAD;
+  const int lines = a.elems / n;
+  const int groups = a.elems / RAD;
+  const int ld = a.lds_stride[d];
+  // the other three dimensions of a line, x first (selects, not a local
+  // array with a run-time index, which would live in scratch memory)
+  const int q0 = d == 0 ? 1 : 0, q1 = d <= 1 ? 2 : 1, q2 = d <= 2 ? 3 : 2;
+  auto pick = [](const int* v, int q) { return q == 0 ? v[0] : q == 1 ? v[1] : q == 2 ? v[2] : v[3]; };
+  const int oe[3] = {pick(a.box, q0), pick(a.box, q1), pick(a.box, q2)};
+  const int os[3] = {pick(a.lds_stride, q0), pick(a.lds_stride, q1), pick(a.lds_stride, q2)};
+  const cpx<R>* wr = tw + sp.wr_off;
+  const cpx<R>* wl = tw + sp.tw_off;
+  for (int g = tid; g < groups; g += nthr) {
+    const int line = g % lines, gl = g / lines;
+    int rem = line;
+    const int c0 = rem % oe[0];
+    rem /= oe[0];
+    const int c1 = rem % oe[1];
+    const int c2 = rem / oe[1];
+    const int base = c0 * os[0] + c1 * os[1] + c2 * os[2];
+    const int blk = gl / s, jp = gl - blk * s;
+    const int p0 = blk * L + jp;
+    GMOD_BOX_CHECK(c2 < oe[2] && p0 + s * (RAD - 1) < n, "sub-pass position");
+    GMOD_BOX_CHECK(base + (p0 + s * (RAD - 1)) * ld < a.lds_elems, "sub-pass lds");
+    cpx<R> v[RAD];
+#pragma unroll
+    for (int j = 0; j < RAD; ++j) v[j] = lds[base + (p0 + s * j) * ld];
+    box_dft<R, RAD>(v, wr);
+    if (s > 1) {
+      GMOD_BOX_CHECK(jp * (RAD - 1) < L, "sub-pass twiddle");
+#pragma unroll
+      for (int k = 1; k < RAD; ++k) v[k] = v[k] * wl[jp * k];
+    }
+#pragma unroll
+    for (int k = 0; k < RAD; ++k) lds[base + (p0 + s * k) * ld] = v[k];
+  }
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void box_dispatch(cpx<R>* lds, const BoxArgs& a, const BoxSub& sp,
+                                                      const cpx<R>* __restrict__ tw, int tid, int nthr) {
+  switch (sp.radix) {
+    case 2: box_subpass<R, 2>(lds, a, sp, tw, tid, nthr); break;
+    case 3: box_subpass<R, 3>(lds, a, sp, tw, tid, nthr); break;
+    case 4: box_subpass<R, 4>(lds, a, sp, tw, tid, nthr); break;
+    case 5: box_subpass<R, 5>(lds, a, sp, tw, tid, nthr); break;
+    case 7: box_subpass<R, 7>(lds, a, sp, tw, tid, nthr); break;
+    case 8: box_subpass<R, 8>(lds, a, sp, tw, tid, nthr); break;
+    case 16: box_subpass<R, 16>(lds, a, sp, tw, tid, nthr); break;
+    case 32:
+      if constexpr (sizeof(R) == 4) box_subpass<R, 32>(lds, a, sp, tw, tid, nthr);
+      break;
+    default: break;
+  }
+}
+
+// position of output index k of a transformed dimension (undoes the in-place
+// digit order of the sub-passes)
+__host__ __device__ __forceinline__ int box_outpos(const BoxArgs& a, int d, int k) {
+  const int nr = a.nrad[d];
+  if (nr <= 1) return k;
+  int p = 0;
+  for (int i = 0; i < nr; ++i) {
+    const int r = a.rad[d][i];
+    p += (k % r) * a.radw[d][i];
+    k /= r;
+  }
+  return p;
+}
+
+// ---- phase 3: LDS -> grid through the store functor
+template <typename R, class ST>
+__host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& st, const BoxArgs& a,
+                                                   const BoxOrigin& O, R scale, int tid, int nthr) {
+  // batches of U elements: every load a store functor needs (accumulators,
+  // operands, gather indices) is issued before the batch's first store
+  constexpr int U = 8;
+  const int ex = a.box[0], ey = a.box[1], ez = a.box[2];
+  for (int e0 = tid; e0 < a.elems; e0 += nthr * U) {
+    unsigned spv[U], tv[U];
+    int offv[U];
+    typename ST::Pre pre[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * nthr;
+      offv[u] = -1;
+      if (e < a.elems) {
+        int q = e;
+        const int kx = q % ex; q /= ex;
+        const int ky = q % ey; q /= ey;
+        const int kz = q % ez;
+        const int kt = q / ez;
+        const unsigned t = (unsigned)(O.o[3] + kt);
+        if (!st.skip(t)) {
+          offv[u] = box_outpos(a, 0, kx) * a.lds_stride[0] + box_outpos(a, 1, ky) * a.lds_stride[1] +
+                    box_outpos(a, 2, kz) * a.lds_stride[2] + box_outpos(a, 3, kt) * a.lds_stride[3];
+          GMOD_BOX_CHECK(offv[u] < a.lds_elems, "store lds");
+          const unsigned x = (unsigned)(O.o[0] + kx), y = (unsigned)(O.o[1] + ky), z = (unsigned)(O.o[2] + kz);
+          spv[u] = x + a.Nx * (y + a.Ny * z);
+          tv[u] = t;
+          GMOD_BOX_CHECK(spv[u] < a.S && t < a.M, "store grid");
+          pre[u] = st.prep(spv[u], t);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (offv[u] >= 0) st.commit(spv[u], tv[u], pre[u], lds[offv[u]] * scale);
+  }
+}
+
+template <typename R, class LD, class ST>
+__global__ void __launch_bounds__(BOX_THREADS)
+box_fft_kernel(LD ld, ST st, BoxArgs a, const cpx<R>* __restrict__ tw, R scale) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const BoxOrigin O = box_origin(a, box_remap(blockIdx.x, gridDim.x, a.xcd_group));
+  // a box whose slots the store keeps none of (the gather keeps t < Nw)
+  if (st.skip((unsigned)O.o[3]) && st.skip((unsigned)(O.o[3] + a.box[3] - 1))) return;
+  const int tid = threadIdx.x;
+  box_load<R>(lds, ld, a, O, tid, BOX_THREADS);
+  __syncthreads();
+  for (int i = 0; i < a.nsub; ++i) {
+    box_dispatch<R>(lds, a, a.sub[i], tw, tid, BOX_THREADS);
+    __syncthreads();
+  }
+  box_store<R>(lds, st, a, O, scale, tid, BOX_THREADS);
+}
+
+// Host-side plan of box passes (built without a device).
+template <typename R>
+struct BoxPlan {
+  bool ok = false;
+  std::vector<BoxArgs> fwd, bwd;
+  std::vector<cpx<R>> tw_host[2];  // [0]: sign -1, [1]: sign +1
+  size_t max_lds_bytes = 0;  // largest box of this plan
+  size_t lds_limit = 0;      // the device's per-work-group LDS limit the plan was built for
+  // plan the transform of the axes in `mask` of an (nx,ny,nz,m) grid with
+  // boxes of at most `budget` elements and `lds_bytes` of LDS
+  void build(int nx, int ny, int nz, int m, int mask, long budget, size_t lds_bytes);
+};
+
+}  // namespace gmod

@@ -231,14 +231,24 @@ struct SigmaIncStore {
   cpx<R>* FVr;
   int first;
   unsigned S;
-  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const {
+  // prep (all loads) / commit (arithmetic and stores): the box FFT issues the
+  // loads of a batch of elements before any of their stores
+  struct Pre { cpx<R> acc, fr, fd, fv; };
+  __device__ Pre prep(unsigned sp, unsigned t) const {
     const size_t ad = sp + (size_t)S * t;
-    const cpx<R> fd = Fd[ad];
-    const cpx<R> fv = FVr[ad];
-    cpx<R> p = v * (Fr[ad] + fd) + fv * fd;
-    acc[ad] = first ? p : acc[ad] + p;
-    FVr[ad] = fv + v;
+    Pre q;
+    q.acc = first ? cpx<R>(0, 0) : acc[ad];
+    q.fr = Fr[ad];
+    q.fd = Fd[ad];
+    q.fv = FVr[ad];
+    return q;
   }
+  __device__ void commit(unsigned sp, unsigned t, const Pre& q, cpx<R> v) const {
+    const size_t ad = sp + (size_t)S * t;
+    acc[ad] = q.acc + v * (q.fr + q.fd) + q.fv * q.fd;
+    FVr[ad] = q.fv + v;
+  }
+  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
   __device__ bool skip(unsigned) const { return false; }
 };
 
@@ -249,11 +259,18 @@ struct MulAccStore {
   const cpx<R>* F;
   int first;
   unsigned S;
-  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const {
+  struct Pre { cpx<R> acc, f; };
+  __device__ Pre prep(unsigned sp, unsigned t) const {
     const size_t ad = sp + (size_t)S * t;
-    cpx<R> p = v * F[ad];
-    acc[ad] = first ? p : acc[ad] + p;
+    Pre q;
+    q.acc = first ? cpx<R>(0, 0) : acc[ad];
+    q.f = F[ad];
+    return q;
   }
+  __device__ void commit(unsigned sp, unsigned t, const Pre& q, cpx<R> v) const {
+    acc[sp + (size_t)S * t] = q.acc + v * q.f;
+  }
+  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
   __device__ bool skip(unsigned) const { return false; }
 };
 
@@ -268,17 +285,27 @@ struct GatherStore {
   int acc;
   MeshDev m;
   const zd* base = nullptr;  // optional: dst = base + w v (FP64 reference + increment)
-  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const {
-    if ((int)t >= m.Nw) return;
+  struct Pre { long o; zd old, bv, js; };  // o < 0: slot or point not kept
+  __device__ Pre prep(unsigned sp, unsigned t) const {
+    Pre q;
+    q.o = -1;
+    if ((int)t >= m.Nw) return q;
     const long k = m.kgat_s[sp];
-    if (k < 0) return;
-    zd x = zd(v) * w;
-    if (sawc) x += J * sawc[t];
-    const long o = k + m.Nk * (long)t;
-    if (base) x += base[o];
-    zd* d = dst + o;
-    *d = acc ? *d + x : x;
+    if (k < 0) return q;
+    q.o = k + m.Nk * (long)t;
+    if (acc) q.old = dst[q.o];
+    if (base) q.bv = base[q.o];
+    if (sawc) q.js = J * sawc[t];
+    return q;
   }
+  __device__ void commit(unsigned, unsigned, const Pre& q, cpx<R> v) const {
+    if (q.o < 0) return;
+    zd x = zd(v) * w;
+    if (sawc) x += q.js;
+    if (base) x += q.bv;
+    dst[q.o] = acc ? q.old + x : x;
+  }
+  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
   __device__ bool skip(unsigned t) const { return (int)t >= m.Nw; }
 };
 

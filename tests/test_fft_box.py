@@ -1,0 +1,64 @@
+"""The box-pass FFT of libgmod.so, checked on the CPU through its host hook
+(gmod_box_fft_host runs the kernel's own load / sub-pass / store code on the
+host with bounds checks) against numpy's FFT.  Covers the plan shapes of the
+benchmark workloads (one spatial box + one t box in FP32, x-y / z / t boxes
+in FP64), mixed radices 2-32 with 3, 5 and the four-step splits of long t
+axes, and a small LDS budget that forces the split plans.
+
+Conventions (libs/src/fmod.f90:67-160): forward sign -1 unnormalised,
+backward sign +1 divided by the number of transformed points."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from pyrpa_amd import _loader
+
+
+def _hook():
+    f = _loader.lib().gmod_box_fft_host
+    f.argtypes = [np.ctypeslib.ndpointer(np.complex128), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_int)]
+    f.restype = ctypes.c_int
+    return f
+
+
+SHAPES = [(4, 4, 1, 32), (8, 8, 1, 320), (3, 3, 1, 10), (4, 3, 1, 8), (32, 32, 16, 16), (64, 64, 1, 128),
+          (4, 4, 4, 8), (2, 2, 4, 8), (128, 128, 1, 8), (8, 4, 1, 2048), (16, 16, 1, 4096)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_box_fft_matches_numpy(shape, prec):
+    f = _hook()
+    nx, ny, nz, m = shape
+    rng = np.random.default_rng(sum(shape) + prec)
+    a = rng.standard_normal((m, nz, ny, nx)) + 1j * rng.standard_normal((m, nz, ny, nx))
+    tol = 1e-13 if prec == 0 else 3e-6
+    covered = 0
+    for mask in (15, 7):
+        axes = [3 - i for i in range(4) if (mask >> i) & 1]
+        for sign in (-1, 1):
+            ref = np.fft.fftn(a, axes=axes) if sign < 0 else np.fft.ifftn(a, axes=axes)
+            for lds in (163840, 16384):
+                d = a.ravel().copy()
+                npass = ctypes.c_int(0)
+                rc = f(d, nx, ny, nz, m, mask, sign, prec, lds, ctypes.byref(npass))
+                assert rc in (0, 1), (shape, mask, sign, lds, rc)
+                if rc == 1:
+                    continue
+                covered += 1
+                err = np.abs(d.reshape(a.shape) - ref).max() / np.abs(ref).max()
+                assert err < tol, (shape, prec, mask, sign, lds, err)
+    assert covered >= 4
+
+
+@pytest.mark.parametrize("prec,passes", [(1, 2), (0, 3)])
+def test_box_plan_pass_count_of_the_4orb_3d_mesh(prec, passes):
+    """32x32x16 spatial volume: one spatial box + one t box in FP32 (128 KB),
+    x-y plane + z column + t boxes in FP64, with the 160 KB LDS of gfx950."""
+    f = _hook()
+    d = np.zeros(32 * 32 * 16 * 8, np.complex128)
+    npass = ctypes.c_int(0)
+    assert f(d, 32, 32, 16, 8, 15, -1, prec, 163840, ctypes.byref(npass)) == 0
+    assert npass.value == passes
