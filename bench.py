@@ -118,6 +118,7 @@ def main():
     ap.add_argument("--config", type=int, default=3, help="workload (1..4); the metric is quoted on 3")
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--warm", default="coarse", choices=["coarse", "none"], help="warm start of the mixed arm")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -138,6 +139,7 @@ def main():
     for w in range(args.warmup):
         solve(F, L, st, 0, 0.0, scf_loop=2)
         solve(F, L, st, 1, 0.0, scf_loop=3)
+        W.warm_start(st, args.warm)
         log(f"warmup {w + 1}/{args.warmup} done")
 
     steps = []
@@ -146,8 +148,12 @@ def main():
         log(f"step {k + 1}: plain FP64 cold: {int(s64['iterations'])} iterations, {s64['solve_seconds']:.2f} s "
             f"(wall {s64['wall_seconds']:.2f} s), esterr {s64['esterr']:.2e}")
         tw = time.perf_counter()
-        seed, winfo = W.warm_start(st)
+        seed, winfo = W.warm_start(st, args.warm)
         t_warm = time.perf_counter() - tw
+        if seed is not None:
+            winfo["seed_rel_err_vs_converged"] = rel(seed[0], sig64)
+            log(f"step {k + 1}: warm start {winfo['kind']}: {t_warm:.2f} s, seed max|dSigma|/max|Sigma| vs the "
+                f"converged FP64 Sigma {winfo['seed_rel_err_vs_converged']:.2e}")
         sigm, mum, sm, pm = solve(F, L, st, 1, args.eps, seed=seed, profile=True)
         t_mixed = t_warm + sm["solve_seconds"]
         log(f"step {k + 1}: mixed + warm start ({winfo['kind']}, {t_warm:.2f} s): {int(sm['iterations'])} "
@@ -199,6 +205,7 @@ def main():
         "t_plain_fp64_s": round(t_plain, 3),
         "t_mixed_warm_s": round(t_mix, 3),
         "t_warm_start_s": round(best["t_warm"], 3),
+        "warm_start": best["winfo"],
         "iterations_plain": int(best["s64"]["iterations"]),
         "iterations_mixed": int(best["sm"]["iterations"]),
         "ms_per_iteration_plain": round(1e3 * best["s64"]["solve_seconds"] / best["s64"]["iterations"], 1),

@@ -154,6 +154,19 @@ int gmod_flex_stats(double* out, int n);
 void gmod_profile(int on);
 int gmod_profile_read(double* out, int n);
 void gmod_set_verbose(int level);
+/* Warm-start re-gridding (no C counterpart in the reference; its Python
+ * re-gridder libs/plibs/_calc.py:52-182 covers the frequency part): Sigma
+ * (Nk_c,Nw_c,Norb,Norb) of a solve on a coarse mesh onto Sigma
+ * (Nk_f,Nw_f,Norb,Norb) of a finer mesh at the same temperature.  k: Fourier
+ * interpolation over the full BZ with the TRS unfolding Sigma(-k) =
+ * Sigma(k)^T; frequency: shared Matsubara frequencies n < Nw_c - 1 kept (all
+ * of them when Nw_c == Nw_f), the tail c0 + c1/(i w) fitted on
+ * [0.55, 0.85] Nw_c beyond, diagonal Im Sigma
+ * clipped to <= 0.  kmap/invk are the (3, Nkall) tables of each mesh. */
+void gmod_regrid_sigma(gmod_z* sig_f, const gmod_z* sig_c, const int64_t* kmap_c, const int64_t* invk_c,
+                       int64_t Nk_c, int64_t Nx_c, int64_t Ny_c, int64_t Nz_c, int64_t Nw_c,
+                       const int64_t* kmap_f, const int64_t* invk_f, int64_t Nk_f, int64_t Nx_f, int64_t Ny_f,
+                       int64_t Nz_f, int64_t Nw_f, int64_t Norb, double temp);
 /* Host test hook (no reference counterpart; used by the CPU test suite only):
  * runs the box-pass FFT plan of an (nx,ny,nz,m) grid (x fastest, interleaved
  * complex, in place) on the host with the kernel's own load / sub-pass /

@@ -75,6 +75,9 @@ def _L():
         L.gmod_set_sigma_seed.argtypes = [_z, c_double, c_double, c_int64]
         L.gmod_flex_stats.argtypes = [_d, c_int]
         L.gmod_flex_stats.restype = c_int
+        L.gmod_regrid_sigma.argtypes = [_z, _z, _i, _i, c_int64, c_int64, c_int64, c_int64, c_int64, _i, _i, c_int64,
+                                        c_int64, c_int64, c_int64, c_int64, c_int64, c_double]
+        L.gmod_regrid_sigma.restype = None
         L.gmod_profile.argtypes = [c_int]
         L.gmod_profile_read.argtypes = [_d, c_int]
         L.gmod_profile_read.restype = c_int
@@ -424,3 +427,24 @@ def release_memory():
     """Return the unused blocks of the library's device pool to HIP."""
     _L().gmod_release_memory()
     check()
+
+
+def regrid_sigma(sigmak_c, kmap_c, invk_c, mesh_c, kmap_f, invk_f, mesh_f, temp):
+    """Sigma[No,No,Nw_c,Nk_c] of a coarse solve onto the fine mesh (same T):
+    Fourier interpolation in k, shared Matsubara frequencies kept, tail
+    c0 + c1/(i w) beyond (gmod_regrid_sigma).  mesh = (Nx, Ny, Nz, Nw).
+    Returns Sigma[No,No,Nw_f,Nk_f]."""
+    s = np.ascontiguousarray(sigmak_c, dtype=np.complex128)
+    No, _, Nwc, Nkc = s.shape
+    Nxc, Nyc, Nzc, Nwc2 = mesh_c
+    Nxf, Nyf, Nzf, Nwf = mesh_f
+    if Nwc2 != Nwc:
+        raise ValueError("mesh_c Nw does not match sigmak_c")
+    Nkf = int(invk_f[:, 0].max())
+    out = np.zeros((No, No, Nwf, Nkf), dtype=np.complex128)
+    _L().gmod_regrid_sigma(out, s, np.ascontiguousarray(kmap_c, dtype=np.int64),
+                           np.ascontiguousarray(invk_c, dtype=np.int64), Nkc, Nxc, Nyc, Nzc, Nwc,
+                           np.ascontiguousarray(kmap_f, dtype=np.int64), np.ascontiguousarray(invk_f, dtype=np.int64),
+                           Nkf, Nxf, Nyf, Nzf, Nwf, No, float(temp))
+    check()
+    return out

@@ -1,0 +1,100 @@
+"""Warm-start re-gridding on the device (gmod_regrid_sigma), checked against
+what the mathematics fixes: Fourier interpolation reproduces a
+band-limited Sigma(k) exactly, the tail fit reproduces an exact
+c0 + c1/(i w) tail, and a Sigma on the same mesh comes back unchanged
+(up to the dropped outermost frequency, which the tail replaces)."""
+import numpy as np
+import pytest
+
+from tests import models as MD
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def F():
+    import pyrpa_amd.flibs as F
+    F.set_precision(0)
+    return F
+
+
+def _mesh(F, Nx, Ny, Nz):
+    kl, kmap, invk = F.gen_irr_k_TRS(Nx, Ny, Nz)
+    return kl, kmap, invk
+
+
+def _bandlimited(kl, No, Nw, temp, R, coef):
+    """Sigma_ab(k, w_n) = sum_R c_R,ab(n) exp(-2 pi i k.R) with the R set
+    closed under R -> -R and c_-R,ab = c_R,ba (time reversal: Sigma(-k) =
+    Sigma(k)^T).  Returns Python layout [No,No,Nw,Nk]."""
+    ph = np.exp(-2j * np.pi * kl @ R.T)                   # (Nk, nR)
+    sig = np.einsum('kr,rabn->abnk', ph, coef)
+    return np.ascontiguousarray(sig)
+
+
+def _coefs(rng, R, No, Nw):
+    nR = len(R)
+    coef = np.zeros((nR, No, No, Nw), complex)
+    idx = {tuple(r): i for i, r in enumerate(R.astype(int))}
+    for i, r in enumerate(R.astype(int)):
+        j = idx[tuple(-r)]
+        if j < i:
+            continue
+        c = rng.standard_normal((No, No, Nw)) + 1j * rng.standard_normal((No, No, Nw))
+        coef[i] = c
+        coef[j] = np.swapaxes(c, 0, 1) if j != i else 0.5 * (c + np.swapaxes(c, 0, 1))
+    return coef
+
+
+@pytest.mark.parametrize("coarse,fine", [((4, 4, 1), (8, 8, 1)), ((4, 4, 4), (8, 8, 8)), ((6, 6, 1), (12, 12, 1))])
+def test_fourier_interpolation_is_exact_for_band_limited_sigma(F, coarse, fine):
+    rng = np.random.default_rng(3)
+    No, Nw, temp = 2, 16, 0.1
+    R = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (1, 1, 0), (-1, -1, 0)]
+    if coarse[2] > 1:
+        R += [(0, 0, 1), (0, 0, -1)]
+    R = np.array(R, dtype=float)
+    coef = _coefs(rng, R, No, Nw)
+    klc, kmc, ivc = _mesh(F, *coarse)
+    klf, kmf, ivf = _mesh(F, *fine)
+    sc = _bandlimited(klc, No, Nw, temp, R, coef)
+    sf = _bandlimited(klf, No, Nw, temp, R, coef)
+    out = F.regrid_sigma(sc, kmc, ivc, (*coarse, Nw), kmf, ivf, (*fine, Nw), temp)
+    # equal cutoffs: every frequency is kept (no tail); causality clipping
+    # of the diagonal is masked below
+    keep = Nw
+    d = np.abs(out[:, :, :keep] - sf[:, :, :keep])
+    # clipping touches only diagonal elements with Im > 0; exclude them
+    mask = np.ones_like(d, dtype=bool)
+    for a in range(No):
+        mask[a, a] = sf[a, a, :keep].imag <= 0
+    assert d[mask].max() < 1e-11 * np.abs(sf).max()
+
+
+def test_tail_fit_reproduces_an_exact_tail_and_clips_causality(F):
+    No, Nwc, Nwf, temp = 2, 64, 256, 0.05
+    kl, km, iv = _mesh(F, 4, 4, 1)
+    Nk = len(kl)
+    w = (2 * np.arange(Nwf) + 1) * np.pi * temp
+    rng = np.random.default_rng(5)
+    c0 = rng.standard_normal((No, No, Nk)) + 1j * rng.standard_normal((No, No, Nk))
+    c1 = rng.standard_normal((No, No, Nk)) + 1j * rng.standard_normal((No, No, Nk))
+    c0 = 0.5 * (c0 + np.swapaxes(c0, 0, 1))   # symmetric so that the k-unfolding is consistent
+    c1 = 0.5 * (c1 + np.swapaxes(c1, 0, 1))
+    full = c0[:, :, None, :] + c1[:, :, None, :] / (1j * w)[None, None, :, None]
+    out = F.regrid_sigma(np.ascontiguousarray(full[:, :, :Nwc]), km, iv, (4, 4, 1, Nwc), km, iv, (4, 4, 1, Nwf), temp)
+    ref = full.copy()
+    for a in range(No):
+        ref[a, a] = ref[a, a].real + 1j * np.minimum(ref[a, a].imag, 0.0)
+    assert np.abs(out - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+def test_same_mesh_identity(F):
+    st = MD.two_orbital(Nx=4, Ny=4, Nw=16, temp=0.1, mu=0.0)
+    G = st['Gk']
+    kl, km, iv = st['klist'], st['kmap'], st['invk']
+    out = F.regrid_sigma(G, km, iv, (4, 4, 1, 16), km, iv, (4, 4, 1, 16), 0.1)
+    ref = G.copy()
+    for a in range(2):
+        ref[a, a] = ref[a, a].real + 1j * np.minimum(ref[a, a].imag, 0.0)
+    assert np.abs(out - ref).max() < 1e-12
