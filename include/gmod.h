@@ -173,8 +173,8 @@ void gmod_regrid_sigma(gmod_z* sig_f, const gmod_z* sig_c, const int64_t* kmap_c
  * store code and bounds checks.  mask selects the axes (bit 0: x .. bit 3:
  * t), sign -1 forward / +1 normalised backward, precision 0 FP64 / 1 FP32,
  * lds_bytes the per-work-group LDS limit to plan for; npasses receives the
- * number of box passes.  Returns 0, 1 if the box engine does not cover the
- * shape, 2 on an indexing error. */
+ * number of box passes (data == NULL: plan only).  Returns 0, 1 if the box
+ * engine does not cover the shape, 2 on an indexing error. */
 int gmod_box_fft_host(double* data, int nx, int ny, int nz, int m, int mask, int sign, int precision,
                       long lds_bytes, int* npasses);
 /* Device memory held by the library's caching pool, in bytes. */

@@ -293,13 +293,15 @@ void FFTPlan<R>::exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R 
     const cpx<R>* twb = twb_[sign < 0 ? 0 : 1];
     const std::vector<BoxArgs>& bp = sign < 0 ? box_.fwd : box_.bwd;
     const int nb = (int)bp.size();
+    auto bo = [&](int which) { return which ? tmp : buf; };
     for (int i = 0; i < nb; ++i) {
+      const BoxArgs& a = bp[i];
       const R sc = (i == nb - 1) ? scale : (R)1;
       const bool first = i == 0, last = i == nb - 1;
-      if (first && last) launch_box(bp[i], ld, st, twb, sc, s);
-      else if (first) launch_box(bp[i], ld, GridStore<R>{buf, S_}, twb, sc, s);
-      else if (last) launch_box(bp[i], GridLoad<R>{buf, S_}, st, twb, sc, s);
-      else launch_box(bp[i], GridLoad<R>{buf, S_}, GridStore<R>{buf, S_}, twb, sc, s);
+      if (first && last) launch_box(a, ld, st, twb, sc, s);
+      else if (first) launch_box(a, ld, GridStore<R>{bo(a.dst), S_}, twb, sc, s);
+      else if (last) launch_box(a, GridLoad<R>{bo(a.src), S_}, st, twb, sc, s);
+      else launch_box(a, GridLoad<R>{bo(a.src), S_}, GridStore<R>{bo(a.dst), S_}, twb, sc, s);
     }
     return;
   }

@@ -83,11 +83,14 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
     a.lds_elems = (int)lds_elems;
     a.elems = (int)elems;
     a.nsub = 0;
+    a.tl_mul = a.ts_mul = ext[3];
+    a.tl_str = a.ts_str = 1;
+    a.tw4_n = a.tw4_off = 0;
     for (int d : tdims) {
       std::vector<int> rads;
-      if (!box_factor(n[d], maxr, rads)) return false;
+      if (!box_factor(ext[d], maxr, rads)) return false;
       a.nrad[d] = (int)rads.size();
-      int L = n[d], w = n[d];
+      int L = ext[d], w = ext[d];
       for (size_t i = 0; i < rads.size(); ++i) {
         const int r = rads[i];
         w /= r;
@@ -162,7 +165,8 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
     spatial.push_back(a);
   }
   const bool do_t = std::find(tx.begin(), tx.end(), 3) != tx.end();
-  BoxArgs tb;
+  BoxArgs tb, tA, tB;
+  bool split = false;
   if (do_t) {
     if (m > budget) return;
     int ext[4] = {1, 1, 1, m};
@@ -170,11 +174,65 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
     if (ext[0] == nx) ext[1] = largest_divisor_le(ny, budget / ((long)m * nx));
     if (ext[0] == nx && ext[1] == ny) ext[2] = largest_divisor_le(nz, budget / ((long)m * nx * ny));
     if (!make_box(ext, {3}, tb)) return;
+    // A t box whose rows are shorter than a 128 B cache line (few columns x a
+    // long t line) reads and writes scattered half lines.  Split t = t2 + M2 t1
+    // (four-step): pass A transforms t1 for each t2 (twiddle w_M^(t2 k1)),
+    // pass B transforms t2 for each k1 and stores k1 + M1 k2 in natural order;
+    // both boxes hold whole x rows.
+    const long row = (ext[0] < nx ? (long)ext[0] : (long)nx * ext[1]) * b;
+    if (row < 128 && (long)nx * b >= 128 && !getenv("GMOD_FFT_NO_TSPLIT")) {
+      int M1 = 1;
+      while ((long)M1 * M1 < m) M1 *= 2;
+      while (M1 > 1 && m % M1 != 0) M1 /= 2;
+      const int M2 = m / M1;
+      std::vector<int> r1, r2;
+      if (M1 > 1 && M2 > 1 && box_factor(M1, maxr, r1) && box_factor(M2, maxr, r2)) {
+        // as many whole x rows per box as the budget allows: each slot row of
+        // a box is then one contiguous run of the grid (2 KB and more at the
+        // benchmark shapes), which DRAM reads and writes as whole pages
+        int eA[4] = {nx, largest_divisor_le(ny, std::max<long>(1, budget / ((long)nx * M1))), 1, M1};
+        int eB[4] = {nx, largest_divisor_le(ny, std::max<long>(1, budget / ((long)nx * M2))), 1, M2};
+        if (eA[1] == ny) eA[2] = largest_divisor_le(nz, std::max<long>(1, budget / ((long)nx * ny * M1)));
+        if (eB[1] == ny) eB[2] = largest_divisor_le(nz, std::max<long>(1, budget / ((long)nx * ny * M2)));
+        if (make_box(eA, {3}, tA) && make_box(eB, {3}, tB)) {
+          tA.tl_mul = tA.ts_mul = 1;            // bt = t2
+          tA.tl_str = tA.ts_str = M2;
+          tA.tw4_n = m;
+          tA.tw4_off = add_table(m);
+          tB.tl_mul = M2;                       // bt = k1: loads t = k1 M2 + t2
+          tB.tl_str = 1;
+          tB.ts_mul = 1;                        // stores t = k1 + M1 k2
+          tB.ts_str = M1;
+          split = true;
+        }
+      }
+    }
   }
-  for (auto& a : spatial) fwd.push_back(a);
-  if (do_t) fwd.push_back(tb);
-  if (do_t) bwd.push_back(tb);
-  for (auto it = spatial.rbegin(); it != spatial.rend(); ++it) bwd.push_back(*it);
+  // buffers (0: caller's, 1: scratch): a pass that moves data between slots
+  // (pass B) never writes the buffer it reads
+  if (!split) {
+    for (auto& a : spatial) fwd.push_back(a);
+    if (do_t) fwd.push_back(tb);
+    if (do_t) bwd.push_back(tb);
+    for (auto it = spatial.rbegin(); it != spatial.rend(); ++it) bwd.push_back(*it);
+  } else {
+    for (auto& a : spatial) { a.src = a.dst = 0; fwd.push_back(a); }
+    tA.src = 0; tA.dst = 1;
+    tB.src = 1; tB.dst = 0;
+    fwd.push_back(tA);
+    fwd.push_back(tB);
+    // backward: A then B, then the spatial passes in place on the scratch
+    BoxArgs bA = tA, bB = tB;
+    if (spatial.empty()) { bA.src = 0; bA.dst = 1; bB.src = 1; bB.dst = 0; }
+    else { bA.src = 0; bA.dst = 0; bB.src = 0; bB.dst = 1; }
+    bwd.push_back(bA);
+    bwd.push_back(bB);
+    for (auto it = spatial.rbegin(); it != spatial.rend(); ++it) {
+      BoxArgs a = *it;
+      a.src = a.dst = 1;
+      bwd.push_back(a);
+    }
+  }
   // twiddle pools: tw[k] = exp(sign 2 pi i k / len), computed in double
   for (int sg = 0; sg < 2; ++sg) {
     tw_host[sg].clear();
@@ -403,10 +461,18 @@ int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign
   BoxPlan<R> plan;
   plan.build(nx, ny, nz, m, mask, budget, lds_bytes);
   if (!plan.ok) return 1;
+  if (!data) {  // plan only
+    if (npasses) *npasses = (int)(sign < 0 ? plan.fwd.size() : plan.bwd.size());
+    return 0;
+  }
   const long N = (long)nx * ny * nz * m;
   const unsigned S = (unsigned)((long)nx * ny * nz);
-  std::vector<cpx<R>> g(N);
+  std::vector<cpx<R>> g(N), gt(N);
   for (long i = 0; i < N; ++i) g[i] = cpx<R>((R)data[2 * i], (R)data[2 * i + 1]);
+  // the exec order of FFTPlan::exec_fused: pass i reads the input (i == 0)
+  // or buffer src, writes buffer dst or the output (last pass); the output is
+  // the caller's buffer here
+  std::vector<cpx<R>>* bufs[2] = {&g, &gt};
   const std::vector<BoxArgs>& ps = sign < 0 ? plan.fwd : plan.bwd;
   const cpx<R>* tw = plan.tw_host[sign < 0 ? 0 : 1].data();
   long tpts = 1;
@@ -419,14 +485,23 @@ int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign
     std::vector<cpx<R>> lds(a.lds_elems);
     const long nb = a.blocks();
     std::vector<char> seen(nb, 0);
+    const bool first = i == 0, last = i + 1 == ps.size();
+    cpx<R>* in = first ? g.data() : bufs[a.src]->data();
+    cpx<R>* out = last ? g.data() : bufs[a.dst]->data();
+    if (!first && !last && in != out && a.src == a.dst) throw Error("inconsistent pass buffers");
+    // a pass that moves data between slots must not read the buffer it writes
+    if (a.ts_mul != a.tl_mul || a.ts_str != a.tl_str) {
+      if (in == out) throw Error("slot-moving pass in place");
+    }
+    std::vector<cpx<R>> snapshot;
     for (long blk = 0; blk < nb; ++blk) {
       const unsigned lb = box_remap((unsigned)blk, (unsigned)nb, a.xcd_group);
       if (lb >= (unsigned)nb || seen[lb]) throw Error("box remap is not a permutation");
       seen[lb] = 1;
       const BoxOrigin O = box_origin(a, lb);
-      box_load<R>(lds.data(), HostGridLoad<R>{g.data(), S}, a, O, 0, 1);
+      box_load<R>(lds.data(), HostGridLoad<R>{in, S}, a, O, 0, 1);
       for (int j = 0; j < a.nsub; ++j) box_dispatch<R>(lds.data(), a, a.sub[j], tw, 0, 1);
-      box_store<R>(lds.data(), HostGridStore<R>{g.data(), S}, a, O, scale, 0, 1);
+      box_store<R>(lds.data(), HostGridStore<R>{out, S}, a, O, scale, tw, 0, 1);
     }
   }
   for (long i = 0; i < N; ++i) { data[2 * i] = (double)g[i].re; data[2 * i + 1] = (double)g[i].im; }

@@ -51,6 +51,12 @@ struct BoxArgs {
   unsigned Nx, Ny, S, M;
   int xcd_group;                // consecutive boxes that share cache lines (1: no remap)
   int src, dst;                 // intermediate buffers (0: caller buffer, 1: scratch)
+  // slot of box element i along t for the box with t index bt:
+  //   load t = bt tl_mul + i tl_str, store t = bt ts_mul + i ts_str
+  // (bt box[3] + i for an ordinary box; strided sets for the two passes of
+  // a four-step split of the t axis)
+  int tl_mul, tl_str, ts_mul, ts_str;
+  int tw4_n, tw4_off;           // four-step twiddle w_{tw4_n}^(bt k) on output k along t (0: none)
   long blocks() const { return (long)nblk[0] * nblk[1] * nblk[2] * nblk[3]; }
 };
 
@@ -121,7 +127,8 @@ __host__ __device__ __forceinline__ void box_dft(cpx<R> (&v)[RAD], const cpx<R>*
 }
 
 struct BoxOrigin {
-  int o[4];
+  int o[3];  // spatial origin
+  int bt;    // box index along t
 };
 
 __host__ __device__ inline BoxOrigin box_origin(const BoxArgs& a, unsigned b) {
@@ -131,7 +138,7 @@ __host__ __device__ inline BoxOrigin box_origin(const BoxArgs& a, unsigned b) {
     O.o[d] = (int)(r % (unsigned)a.nblk[d]) * a.box[d];
     r /= (unsigned)a.nblk[d];
   }
-  O.o[3] = (int)r * a.box[3];
+  O.bt = (int)r;
   return O;
 }
 
@@ -157,7 +164,7 @@ __host__ __device__ __forceinline__ void box_load(cpx<R>* lds, const LD& ld, con
         const int iz = q % ez;
         const int it = q / ez;
         const unsigned x = (unsigned)(O.o[0] + ix), y = (unsigned)(O.o[1] + iy), z = (unsigned)(O.o[2] + iz);
-        const unsigned t = (unsigned)(O.o[3] + it);
+        const unsigned t = (unsigned)(O.bt * a.tl_mul + it * a.tl_str);
         const unsigned sp = x + a.Nx * (y + a.Ny * z);
         GMOD_BOX_CHECK(sp < a.S && t < a.M, "load grid");
         v[u] = ld(sp, t);
@@ -250,7 +257,8 @@ __host__ __device__ __forceinline__ int box_outpos(const BoxArgs& a, int d, int 
 // ---- phase 3: LDS -> grid through the store functor
 template <typename R, class ST>
 __host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& st, const BoxArgs& a,
-                                                   const BoxOrigin& O, R scale, int tid, int nthr) {
+                                                   const BoxOrigin& O, R scale, const cpx<R>* __restrict__ tw,
+                                                   int tid, int nthr) {
   // batches of U elements: every load a store functor needs (accumulators,
   // operands, gather indices) is issued before the batch's first store
   constexpr int U = 8;
@@ -259,6 +267,7 @@ __host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& 
     unsigned spv[U], tv[U];
     int offv[U];
     typename ST::Pre pre[U];
+    cpx<R> twv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e = e0 + u * nthr;
@@ -269,7 +278,7 @@ __host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& 
         const int ky = q % ey; q /= ey;
         const int kz = q % ez;
         const int kt = q / ez;
-        const unsigned t = (unsigned)(O.o[3] + kt);
+        const unsigned t = (unsigned)(O.bt * a.ts_mul + kt * a.ts_str);
         if (!st.skip(t)) {
           offv[u] = box_outpos(a, 0, kx) * a.lds_stride[0] + box_outpos(a, 1, ky) * a.lds_stride[1] +
                     box_outpos(a, 2, kz) * a.lds_stride[2] + box_outpos(a, 3, kt) * a.lds_stride[3];
@@ -279,12 +288,20 @@ __host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& 
           tv[u] = t;
           GMOD_BOX_CHECK(spv[u] < a.S && t < a.M, "store grid");
           pre[u] = st.prep(spv[u], t);
+          if (a.tw4_n) {
+            twv[u] = tw[a.tw4_off + (int)(((long)O.bt * kt) % a.tw4_n)];
+            GMOD_BOX_CHECK(((long)O.bt * kt) % a.tw4_n < a.tw4_n, "four-step twiddle");
+          }
         }
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (offv[u] >= 0) st.commit(spv[u], tv[u], pre[u], lds[offv[u]] * scale);
+      if (offv[u] >= 0) {
+        cpx<R> v = lds[offv[u]] * scale;
+        if (a.tw4_n) v = v * twv[u];
+        st.commit(spv[u], tv[u], pre[u], v);
+      }
   }
 }
 
@@ -294,8 +311,10 @@ box_fft_kernel(LD ld, ST st, BoxArgs a, const cpx<R>* __restrict__ tw, R scale) 
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const BoxOrigin O = box_origin(a, box_remap(blockIdx.x, gridDim.x, a.xcd_group));
-  // a box whose slots the store keeps none of (the gather keeps t < Nw)
-  if (st.skip((unsigned)O.o[3]) && st.skip((unsigned)(O.o[3] + a.box[3] - 1))) return;
+  // a box whose slots the store keeps none of (the gather keeps t < Nw; the
+  // store slots of a box increase with the element index)
+  const unsigned t_first = (unsigned)(O.bt * a.ts_mul), t_last = t_first + (unsigned)((a.box[3] - 1) * a.ts_str);
+  if (st.skip(t_first) && st.skip(t_last)) return;
   const int tid = threadIdx.x;
   box_load<R>(lds, ld, a, O, tid, BOX_THREADS);
   __syncthreads();
@@ -303,7 +322,7 @@ box_fft_kernel(LD ld, ST st, BoxArgs a, const cpx<R>* __restrict__ tw, R scale) 
     box_dispatch<R>(lds, a, a.sub[i], tw, tid, BOX_THREADS);
     __syncthreads();
   }
-  box_store<R>(lds, st, a, O, scale, tid, BOX_THREADS);
+  box_store<R>(lds, st, a, O, scale, tw, tid, BOX_THREADS);
 }
 
 // Host-side plan of box passes (built without a device).

@@ -147,9 +147,11 @@ def split_target(v, nw_out, nk):
 
 def generate(count, seed, nw_out=64, eps=1e-8, precision=0, log=None, Nx=32, Ny=32, Nw=256, ranges=None):
     """Run `count` plain FLEX solves at seeded parameter points.  Returns
-    (P, Y, iters, seconds) with Y the target vectors."""
+    (P, Y, iters, seconds) with Y the target vectors, for the solves that
+    reached the tolerance (the others are dropped: a non-converged Sigma is
+    no target)."""
     P = sample_params(count, seed, ranges)
-    Y, iters = [], []
+    Y, iters, keep = [], [], []
     old = F.get_precision()
     t0 = time.perf_counter()
     try:
@@ -157,13 +159,16 @@ def generate(count, seed, nw_out=64, eps=1e-8, precision=0, log=None, Nx=32, Ny=
         for i, p in enumerate(P):
             st = lattice_state(p, Nx, Ny, Nw)
             sig, mu = L.flex(st, eps=eps, scf_loop=300, sw_rescale=True)
+            stats = F.flex_stats()
             Y.append(target_vector(sig, mu, nw_out))
-            iters.append(int(F.flex_stats()["iterations"]))
+            iters.append(int(stats["iterations"]))
+            keep.append(stats["esterr"] < eps)
             if log and (i + 1) % max(1, count // 10) == 0:
                 log(f"[surrogate] generated {i + 1}/{count} solves ({time.perf_counter() - t0:.1f} s)")
     finally:
         F.set_precision(old)
-    return P, np.array(Y), np.array(iters), time.perf_counter() - t0
+    keep = np.array(keep, dtype=bool)
+    return P[keep], np.array(Y)[keep], np.array(iters)[keep], time.perf_counter() - t0
 
 
 # --------------------------------------------------------------------------- model
@@ -226,3 +231,99 @@ class Surrogate:
         sig = F.regrid_sigma(sig_c, kmap_c, invk_c, (Nx, Ny, 1, self.nw_out), st["kmap"], st["invk"],
                              (st["Nx"], st["Ny"], st["Nz"], st["Nw"]), st["temp"])
         return sig, mu, mu
+
+
+# --------------------------------------------------------------------------- experiment
+def experiment(n_train=2000, n_eval=200, train_mesh=(32, 32, 256), eval_mesh=(64, 64, 1024), nw_out=64,
+               epochs=300, batch=64, lr=1e-3, seed=20261207, holdout=0.1, log=print, model_path=None):
+    """Generate training solves with the plain GPU method, train the network
+    on them, and evaluate the warm start at held-out parameter points on the
+    evaluation mesh: FLEX iterations and time to 1e-8 from the cold start and
+    from the network's seed, and the converged-Sigma deviation."""
+    Nx, Ny, Nw = train_mesh
+    t0 = time.perf_counter()
+    P, Y, iters, t_gen = generate(n_train, seed, nw_out=nw_out, eps=1e-8, precision=0, log=log, Nx=Nx, Ny=Ny, Nw=Nw)
+    n_ok = len(P)
+    log(f"[surrogate] {n_train} training solves in {t_gen:.1f} s: {n_ok} converged to 1e-8 "
+        f"(mean {iters.mean():.1f} iterations)")
+    rng = np.random.default_rng(seed + 1)
+    perm = rng.permutation(n_ok)
+    n_val = int(round(holdout * n_ok))
+    tr, va = perm[n_val:], perm[:n_val]
+    sur = Surrogate(nw_out=nw_out, width=512, layers=4, seed=seed % 1000)
+    t1 = time.perf_counter()
+    hist = sur.fit(P[tr], Y[tr], epochs=epochs, batch=batch, lr=lr, log=log)
+    t_fit = time.perf_counter() - t1
+    pred = sur.predict(P[va])
+    val_rel = float(np.abs(pred[:, :-1] - Y[va, :-1]).max(axis=1).mean() / np.abs(Y[va, :-1]).max(axis=1).mean())
+    log(f"[surrogate] trained in {t_fit:.1f} s; validation mean max|dSigma|/max|Sigma| {val_rel:.3e}")
+    if model_path:
+        sur.save(model_path)
+    Ex, Ey, Ew = eval_mesh
+    rows = []
+    old = F.get_precision()
+    try:
+        for i, p in enumerate(sample_params(n_eval, seed + 2)):
+            st = lattice_state(p, Ex, Ey, Ew)
+            F.set_precision(0)
+            s_cold, mu_cold = L.flex(st, eps=1e-8, sw_rescale=True)
+            c = F.flex_stats()
+            tw = time.perf_counter()
+            sd = sur.seed_for(p, st, train_mesh=(Nx, Ny))
+            t_seed = time.perf_counter() - tw
+            F.set_sigma_seed(*sd)
+            s_warm, mu_warm = L.flex(st, eps=1e-8, sw_rescale=True, sw_in=True)
+            w = F.flex_stats()
+            rows.append(dict(p=[float(x) for x in p], it_cold=int(c["iterations"]), it_warm=int(w["iterations"]),
+                             t_cold=c["solve_seconds"], t_warm=w["solve_seconds"] + t_seed,
+                             conv_cold=bool(c["esterr"] < 1e-8), conv_warm=bool(w["esterr"] < 1e-8),
+                             stoner=c["stoner"],
+                             seed_rel=float(np.abs(sd[0] - s_cold).max() / np.abs(s_cold).max()),
+                             sigma_dev=float(np.abs(s_warm - s_cold).max() / np.abs(s_cold).max()),
+                             mu_dev=abs(mu_warm - mu_cold)))
+            if (i + 1) % max(1, n_eval // 10) == 0:
+                r = rows[-1]
+                log(f"[surrogate] eval {i + 1}/{n_eval}: cold {r['it_cold']} it / warm {r['it_warm']} it, "
+                    f"seed err {r['seed_rel']:.2e}, converged dev {r['sigma_dev']:.2e}")
+    finally:
+        F.set_precision(old)
+    ok = [r for r in rows if r["conv_cold"] and r["conv_warm"]]
+    same = [r for r in ok if r["sigma_dev"] < 1e-6]
+    rep = {
+        "train": {"solves": n_train, "converged": n_ok, "mesh": list(train_mesh), "seconds": t_gen,
+                  "mean_iterations": float(iters.mean()),
+                  "fit_seconds": t_fit, "epochs": epochs, "final_loss": float(hist[-1]), "validation_rel_err": val_rel},
+        "eval": {"points": n_eval, "mesh": list(eval_mesh), "both_converged": len(ok),
+                 "same_fixed_point": len(same),
+                 "iterations_cold_mean": float(np.mean([r["it_cold"] for r in same])) if same else None,
+                 "iterations_warm_mean": float(np.mean([r["it_warm"] for r in same])) if same else None,
+                 "seconds_cold_total": float(sum(r["t_cold"] for r in same)),
+                 "seconds_warm_total": float(sum(r["t_warm"] for r in same)),
+                 "seed_rel_err_median": float(np.median([r["seed_rel"] for r in rows])) if rows else None,
+                 "converged_sigma_dev_max": float(max(r["sigma_dev"] for r in same)) if same else None},
+        "seconds": time.perf_counter() - t0,
+        "rows": rows,
+    }
+    return rep
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="train and evaluate the FLEX warm-start surrogate on the GPU")
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--eval", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--out", default=None, help="JSON report path")
+    ap.add_argument("--model", default=None, help="save the trained network (.npz)")
+    a = ap.parse_args()
+
+    def _log(msg):
+        print(msg, file=sys.stderr, flush=True)
+    rep = experiment(n_train=a.train, n_eval=a.eval, epochs=a.epochs, log=_log, model_path=a.model)
+    txt = json.dumps(rep)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(txt)
+    print(json.dumps({k: v for k, v in rep.items() if k != "rows"}), flush=True)

@@ -24,7 +24,8 @@ def _hook():
 
 
 SHAPES = [(4, 4, 1, 32), (8, 8, 1, 320), (3, 3, 1, 10), (4, 3, 1, 8), (32, 32, 16, 16), (64, 64, 1, 128),
-          (4, 4, 4, 8), (2, 2, 4, 8), (128, 128, 1, 8), (8, 4, 1, 2048), (16, 16, 1, 4096)]
+          (4, 4, 4, 8), (2, 2, 4, 8), (128, 128, 1, 8), (8, 4, 1, 2048), (16, 16, 1, 4096), (32, 8, 1, 2048),
+          (16, 4, 2, 1024)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -53,12 +54,25 @@ def test_box_fft_matches_numpy(shape, prec):
     assert covered >= 4
 
 
-@pytest.mark.parametrize("prec,passes", [(1, 2), (0, 3)])
-def test_box_plan_pass_count_of_the_4orb_3d_mesh(prec, passes):
-    """32x32x16 spatial volume: one spatial box + one t box in FP32 (128 KB),
-    x-y plane + z column + t boxes in FP64, with the 160 KB LDS of gfx950."""
-    f = _hook()
-    d = np.zeros(32 * 32 * 16 * 8, np.complex128)
+def _plan_passes(shape, prec, mask=15, sign=-1, lds=163840):
+    f = _lib_plan()
     npass = ctypes.c_int(0)
-    assert f(d, 32, 32, 16, 8, 15, -1, prec, 163840, ctypes.byref(npass)) == 0
-    assert npass.value == passes
+    assert f(None, *shape, mask, sign, prec, lds, ctypes.byref(npass)) == 0
+    return npass.value
+
+
+def _lib_plan():
+    f = _loader.lib().gmod_box_fft_host
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_int)]
+    f.restype = ctypes.c_int
+    return f
+
+
+@pytest.mark.parametrize("prec,passes", [(1, 3), (0, 4)])
+def test_box_plan_pass_count_of_the_4orb_3d_mesh(prec, passes):
+    """32x32x16 x 2048 with the 160 KB LDS of gfx950: FP32 = one spatial box
+    (128 KB) + the two passes of the split t axis (a box of 8 columns x 2048
+    slots would move 64 B rows); FP64 = x-y box + z box + the two t passes."""
+    assert _plan_passes((32, 32, 16, 2048), prec) == passes
+    assert _plan_passes((32, 32, 16, 8), prec) == passes - 1   # short t axis: one t box
