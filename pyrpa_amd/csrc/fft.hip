@@ -73,7 +73,10 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
       a.nblk[d] = n[d] / ext[d];
       elems *= ext[d];
     }
-    const int px = ext[0] + ((ext[0] > 1 && elems > ext[0]) ? 1 : 0);
+    // x rows padded by one element against LDS bank conflicts when lines run
+    // along x or the rows are long; narrow rows of a t box stay unpadded
+    const bool x_t = std::find(tdims.begin(), tdims.end(), 0) != tdims.end();
+    const int px = ext[0] + ((ext[0] > 1 && elems > ext[0] && (x_t || ext[0] >= 8)) ? 1 : 0);
     a.lds_stride[0] = 1;
     a.lds_stride[1] = px;
     a.lds_stride[2] = px * ext[1];
@@ -174,13 +177,16 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
     if (ext[0] == nx) ext[1] = largest_divisor_le(ny, budget / ((long)m * nx));
     if (ext[0] == nx && ext[1] == ny) ext[2] = largest_divisor_le(nz, budget / ((long)m * nx * ny));
     if (!make_box(ext, {3}, tb)) return;
-    // A t box whose rows are shorter than a 128 B cache line (few columns x a
-    // long t line) reads and writes scattered half lines.  Split t = t2 + M2 t1
-    // (four-step): pass A transforms t1 for each t2 (twiddle w_M^(t2 k1)),
-    // pass B transforms t2 for each k1 and stores k1 + M1 k2 in natural order;
-    // both boxes hold whole x rows.
+    // Optional (GMOD_FFT_TSPLIT=1): for a t box whose rows are shorter than a
+    // 128 B cache line, split t = t2 + M2 t1 (four-step): pass A transforms
+    // t1 for each t2 (twiddle w_M^(t2 k1)), pass B transforms t2 for each k1
+    // and stores k1 + M1 k2 in natural order; both boxes hold whole x rows.
+    // Measured on the 4-orbital 32x32x16 x 2048 grid, the two passes take
+    // longer than the one narrow t box (FP32 2 x 296 us vs 375 us, FP64
+    // 2 x 413 us vs 578 us), so it is off by default.
     const long row = (ext[0] < nx ? (long)ext[0] : (long)nx * ext[1]) * b;
-    if (row < 128 && (long)nx * b >= 128 && !getenv("GMOD_FFT_NO_TSPLIT")) {
+    const char* ts = getenv("GMOD_FFT_TSPLIT");
+    if (row < 128 && (long)nx * b >= 128 && ts && atoi(ts) != 0) {
       int M1 = 1;
       while ((long)M1 * M1 < m) M1 *= 2;
       while (M1 > 1 && m % M1 != 0) M1 /= 2;

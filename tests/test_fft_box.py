@@ -16,10 +16,10 @@ from pyrpa_amd import _loader
 
 
 def _hook():
-    f = _loader.lib().gmod_box_fft_host
-    f.argtypes = [np.ctypeslib.ndpointer(np.complex128), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_int)]
-    f.restype = ctypes.c_int
+    g = _lib_plan()
+
+    def f(d, *rest):
+        return g(d.ctypes.data_as(ctypes.c_void_p), *rest)
     return f
 
 
@@ -69,10 +69,28 @@ def _lib_plan():
     return f
 
 
-@pytest.mark.parametrize("prec,passes", [(1, 3), (0, 4)])
+@pytest.mark.parametrize("prec,passes", [(1, 2), (0, 3)])
 def test_box_plan_pass_count_of_the_4orb_3d_mesh(prec, passes):
     """32x32x16 x 2048 with the 160 KB LDS of gfx950: FP32 = one spatial box
-    (128 KB) + the two passes of the split t axis (a box of 8 columns x 2048
-    slots would move 64 B rows); FP64 = x-y box + z box + the two t passes."""
+    (128 KB) + one t box (8 columns x 2048 slots); FP64 = x-y box + z box +
+    t box."""
     assert _plan_passes((32, 32, 16, 2048), prec) == passes
-    assert _plan_passes((32, 32, 16, 8), prec) == passes - 1   # short t axis: one t box
+    assert _plan_passes((32, 32, 16, 8), prec) == passes
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_split_t_axis_plan_matches_numpy(prec, monkeypatch):
+    """The optional four-step split of the t axis (two strided-slot passes)."""
+    monkeypatch.setenv("GMOD_FFT_TSPLIT", "1")
+    f = _hook()
+    for shape in [(32, 8, 1, 2048), (16, 16, 1, 4096)]:
+        nx, ny, nz, m = shape
+        assert _plan_passes(shape, prec) == (3 if prec == 1 else 3)
+        rng = np.random.default_rng(m + prec)
+        a = rng.standard_normal((m, nz, ny, nx)) + 1j * rng.standard_normal((m, nz, ny, nx))
+        for sign in (-1, 1):
+            d = a.ravel().copy()
+            assert f(d, nx, ny, nz, m, 15, sign, prec, 163840, ctypes.byref(ctypes.c_int(0))) == 0
+            ref = np.fft.fftn(a) if sign < 0 else np.fft.ifftn(a)
+            err = np.abs(d.reshape(a.shape) - ref).max() / np.abs(ref).max()
+            assert err < (1e-13 if prec == 0 else 3e-6), (shape, prec, sign, err)
