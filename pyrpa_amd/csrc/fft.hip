@@ -152,17 +152,24 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
       }
     }
   }
+  // boxes that are not forced to the whole spatial volume may be capped at
+  // GMOD_FFT_SMALL_BOX elements so that two work-groups share a CU.  Measured
+  // on the 4-orbital 32x32x16 x 2048 grid it does not pay (FP64 unchanged,
+  // FP32 13 % slower with the 32 B rows of a 4-column t box): no cap.
+  long small = budget;
+  if (const char* sb = getenv("GMOD_FFT_SMALL_BOX")) small = std::max(1L, std::min(budget, atol(sb)));
   std::vector<BoxArgs> spatial;
   for (auto& gdims : groups) {
     int ext[4] = {1, 1, 1, 1};
     long vol = 1;
     for (int d : gdims) { ext[d] = n[d]; vol *= n[d]; }
     if (vol > budget) return;
+    const long cap = std::max(vol, small);
     if (ext[0] == 1) {  // keep rows contiguous: adjacent x columns in the box
-      ext[0] = largest_divisor_le(nx, std::min<long>(256 / b, budget / vol));
+      ext[0] = largest_divisor_le(nx, std::min<long>(256 / b, cap / vol));
       vol *= ext[0];
     }
-    ext[3] = largest_divisor_le(m, budget / vol);
+    ext[3] = largest_divisor_le(m, cap / vol);
     BoxArgs a;
     if (!make_box(ext, gdims, a)) return;
     spatial.push_back(a);
@@ -172,10 +179,11 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
   bool split = false;
   if (do_t) {
     if (m > budget) return;
+    const long cap = std::max<long>(m, small);
     int ext[4] = {1, 1, 1, m};
-    ext[0] = largest_divisor_le(nx, budget / m);
-    if (ext[0] == nx) ext[1] = largest_divisor_le(ny, budget / ((long)m * nx));
-    if (ext[0] == nx && ext[1] == ny) ext[2] = largest_divisor_le(nz, budget / ((long)m * nx * ny));
+    ext[0] = largest_divisor_le(nx, cap / m);
+    if (ext[0] == nx) ext[1] = largest_divisor_le(ny, cap / ((long)m * nx));
+    if (ext[0] == nx && ext[1] == ny) ext[2] = largest_divisor_le(nz, cap / ((long)m * nx * ny));
     if (!make_box(ext, {3}, tb)) return;
     // Optional (GMOD_FFT_TSPLIT=1): for a t box whose rows are shorter than a
     // 128 B cache line, split t = t2 + M2 t1 (four-step): pass A transforms

@@ -961,26 +961,31 @@ __global__ void k_hermitize_lower(zd* D, long tot, int No) {
   }
 }
 
-__global__ void k_sub_static(zd* sig, long Nk, int Nw, int No, int mode) {
+// static part per (k, l, m) (fself.f90:384-417): mode 1 the Hermitian part
+// of Sigma(k, i w_0), mode 2 the frequency average of the Hermitian part
+__global__ void k_static_part(zd* sub, const zd* sig, long Nk, int Nw, int No, int mode) {
   const long tot = Nk * (long)Nw;
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < Nk; k += (long)gridDim.x * blockDim.x) {
-    zd sub[8][8];
-    for (int l = 0; l < No; ++l)
-      for (int m = l; m < No; ++m) {
-        zd v(0, 0);
-        if (mode == 1) {
-          v = (sig[k + tot * (l + No * m)] + conj(sig[k + tot * (m + No * l)])) * 0.5;
-        } else {
-          for (int j = 0; j < Nw; ++j)
-            v += sig[k + Nk * j + tot * (l + No * m)] + conj(sig[k + Nk * j + tot * (m + No * l)]);
-          v = v * (0.5 / (double)Nw);
-        }
-        sub[l][m] = v;
-        if (l != m) sub[m][l] = conj(v);
-      }
-    for (int j = 0; j < Nw; ++j)
-      for (int l = 0; l < No; ++l)
-        for (int m = 0; m < No; ++m) sig[k + Nk * j + tot * (l + No * m)] -= sub[l][m];
+  const long n = Nk * No * No;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long k = e % Nk;
+    const int lm = (int)(e / Nk), l = lm % No, m = lm / No;
+    zd v(0, 0);
+    if (mode == 1) {
+      v = (sig[k + tot * (l + No * m)] + conj(sig[k + tot * (m + No * l)])) * 0.5;
+    } else {
+      for (int j = 0; j < Nw; ++j)
+        v += sig[k + Nk * j + tot * (l + No * m)] + conj(sig[k + Nk * j + tot * (m + No * l)]);
+      v = v * (0.5 / (double)Nw);
+    }
+    sub[e] = v;  // sub[k + Nk (l + No m)]
+  }
+}
+__global__ void k_sub_static(zd* sig, const zd* sub, long Nk, int Nw, int No) {
+  const long tot = Nk * (long)Nw;
+  const long n = tot * No * No;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long k = e % Nk, lm = e / tot;
+    sig[e] -= sub[k + Nk * lm];
   }
 }
 
@@ -1040,7 +1045,12 @@ void hermitize_lower(zd* D, long Nk, int Nw, int No, hipStream_t s) {
 
 void subtract_static(zd* sigma, long Nk, int Nw, int No, int mode, hipStream_t s) {
   if (No > 8) throw Error("Norb > 8 not supported");
-  hipLaunchKernelGGL(k_sub_static, dim3(nblk(Nk, 128)), dim3(128), 0, s, sigma, Nk, Nw, No, mode);
+  DBuf<zd> sub((size_t)Nk * No * No);
+  hipLaunchKernelGGL(k_static_part, dim3(nblk((long)Nk * No * No, 256)), dim3(256), 0, s, sub.p, sigma, Nk, Nw, No,
+                     mode);
+  GMOD_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sub_static, dim3(nblk((long)Nk * Nw * No * No, 256)), dim3(256), 0, s, sigma, sub.p, Nk, Nw,
+                     No);
   GMOD_CHECK_LAUNCH();
 }
 
