@@ -473,6 +473,12 @@ void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, d
   const unsigned S = (unsigned)mesh.Nkall;
   const R invN = (R)(1.0 / (double)p4->transformed_points());
   const long NC = NkNw;
+  // the vertex increment once in k space at grid precision: each element is
+  // then read (from a quarter of the bytes) by the four grid points that
+  // unfold it, instead of two FP64 reads per grid point
+  const long nC = NkNw * Nchi * Nchi;
+  if (dvbuf.n != (size_t)nC) dvbuf.alloc(nC);
+  diff_to_f32(dvbuf.p, V, Vprev, nC, s);
   for (int l1 = 0; l1 < No; ++l1)
     for (int m1 = 0; m1 < No; ++m1) {
       bool first = true;
@@ -480,8 +486,7 @@ void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, d
         for (int m = 0; m < Nchi; ++m) {
           if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
           const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
-          BosonDiffLoad<R> bl{BosonLoad<double>{V, oml, olm, zd(0, 0), md},
-                              BosonLoad<double>{Vprev, oml, olm, zd(0, 0), md}};
+          BosonLoad<R, float> bl{dvbuf.p, oml, olm, zd(0, 0), md};
           const int a = op.ol[m][1], b = op.ol[l][1];
           SigmaIncStore<R> st{acc.p, grid(Fr, a, b), grid(Fd, a, b), FVr.p + (size_t)(m + Nchi * l) * N,
                               first ? 1 : 0, S};

@@ -66,9 +66,9 @@ struct GreenLoad {
 
 // bosonic V_(m,l) (fself.f90:203-224, feliash.f90:543-564): slot 0 = nu_0,
 // slot Nw = the nu -> infinity value `stat`, slots t and M-t = +-nu_t
-template <typename R>
+template <typename R, typename S = double>
 struct BosonLoad {
-  const zd* V;
+  const cpx<S>* V;
   long off_ml, off_lm;
   zd stat;
   MeshDev m;
@@ -321,6 +321,7 @@ void axpy(zd* y, const zd* x, zd alpha, long n, hipStream_t s);      // y += alp
 void copy(zd* dst, const zd* src, long n, hipStream_t s);
 void lincomb(zd* y, const zd* a, double ca, const zd* b, double cb, long n, hipStream_t s);  // y = ca a + cb b
 void threshold_copy(zd* dst, const zd* src, long n, double eps, hipStream_t s);              // dst = thr(src)
+void diff_to_f32(cpx<float>* dst, const zd* a, const zd* b, long n, hipStream_t s);          // dst = (float)(a - b)
 template <typename R>
 void add_inplace(cpx<R>* y, const cpx<R>* x, long n, hipStream_t s);  // y += x
 

@@ -147,6 +147,15 @@ void threshold_copy(zd* dst, const zd* src, long n, double eps, hipStream_t s) {
   GMOD_CHECK_LAUNCH();
 }
 
+__global__ void k_diff_to_f32(cpx<float>* dst, const zd* a, const zd* b, long n) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    dst[e] = cpx<float>(a[e] - b[e]);
+}
+void diff_to_f32(cpx<float>* dst, const zd* a, const zd* b, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_diff_to_f32, dim3(nblocks(n)), dim3(TPB), 0, s, dst, a, b, n);
+  GMOD_CHECK_LAUNCH();
+}
+
 template <typename R>
 __global__ void k_add_inplace(cpx<R>* y, const cpx<R>* x, long n) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
