@@ -525,6 +525,190 @@ k_vertex_wave(zd* __restrict__ chi, long batch, long es, const VSparse* __restri
   }
 }
 
+// Gauss-Jordan variant, one matrix row per lane: a 16-lane group solves one
+// of the two systems of a (q, i nu) matrix ((I - A S) X = A S or
+// (I + A C) X = A C), so a wavefront carries two matrices.  Per elimination
+// step the group exchanges only two things through its LDS slot -- the
+// pivot candidates |.|_1 of the column and the normalised pivot row -- so a
+// solve costs two LDS round trips per column instead of one cross-lane
+// transfer per matrix element.  Pivots: the largest |Re|+|Im| among the rows
+// not used yet (LAPACK's izamax criterion).
+// a -= f b with four fused multiply-adds
+__device__ __forceinline__ void csub_mul(zd& a, zd f, zd b) {
+  a.re = fma(-f.re, b.re, fma(f.im, b.im, a.re));
+  a.im = fma(-f.re, b.im, fma(-f.im, b.re, a.im));
+}
+// 1 / d through one real division (|d| is far from the overflow range here); 0 for d = 0
+__device__ __forceinline__ zd crecip(zd d) {
+  const double n = d.re * d.re + d.im * d.im;
+  const double q = n == 0.0 ? 0.0 : 1.0 / n;
+  return zd(d.re * q, -d.im * q);
+}
+
+constexpr int GJ_WAVES = 4;
+constexpr int GJ_MAT = 2 * GJ_WAVES;  // matrices per block
+
+template <int N>
+__global__ void __launch_bounds__(64 * GJ_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
+k_vertex_gj(zd* __restrict__ chi, long batch, long es, const VSparse* __restrict__ spg, const double* __restrict__ Sd,
+            const double* __restrict__ Cd, int mode, int* err) {
+  static_assert(N == 16, "a 16-lane group per system");
+  constexpr int NN = N * N;
+  constexpr int PER = GJ_MAT * NN / (64 * GJ_WAVES);  // elements per thread in block-wide phases
+  __shared__ VSparse sp;
+  __shared__ double2 Abuf_[GJ_MAT * NN];  // chi0, later X_s then X_c
+  __shared__ double2 Vbuf_[GJ_MAT * NN];  // A W (mode 0), then the vertex
+  __shared__ double2 Prow_[GJ_WAVES * 4 * 2 * N];
+  __shared__ double Pval_[GJ_WAVES * 4 * N];
+  zd* Abuf = reinterpret_cast<zd*>(Abuf_);
+  zd* Vbuf = reinterpret_cast<zd*>(Vbuf_);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  {
+    const int* src = reinterpret_cast<const int*>(spg);
+    int* dst = reinterpret_cast<int*>(&sp);
+    for (int i = tid; i < (int)(sizeof(VSparse) / sizeof(int)); i += 64 * GJ_WAVES) dst[i] = src[i];
+  }
+  const int g = lane >> 4, r = lane & 15;
+  const int mloc = wave * 2 + (g >> 1);  // matrix of my group within the block
+  const int sys = g & 1;                 // 0: S system, 1: C system
+  zd* A = Abuf + mloc * NN;
+  zd* prow = reinterpret_cast<zd*>(Prow_) + (wave * 4 + g) * 2 * N;
+  double* pval = Pval_ + (wave * 4 + g) * N;
+  const double a1 = mode == 2 ? -0.5 : 1.5;  // coefficient of S X_s
+  const double a2 = mode == 0 ? 0.5 : -0.5;  // coefficient of C X_c
+  __syncthreads();
+  for (long b0 = (long)blockIdx.x * GJ_MAT; b0 < batch; b0 += (long)gridDim.x * GJ_MAT) {
+    for (int i = tid; i < GJ_MAT * NN; i += 64 * GJ_WAVES) {
+      const int m = i % GJ_MAT, e = i / GJ_MAT;
+      const long b = b0 + m;
+      Abuf[m * NN + e] = b < batch ? chi[b + (long)e * es] : zd(0, 0);
+    }
+    __syncthreads();
+    // ---- V = st (- 0.25 W A W for mode 0), block-wide
+    if (mode == 0) {
+      for (int i = tid; i < GJ_MAT * NN; i += 64 * GJ_WAVES) {
+        const int m = i / NN, e = i % NN, p = e % N, q = e / N;
+        zd acc(0, 0);
+#pragma unroll
+        for (int t = 0; t < VW_NZ; ++t) acc += Abuf[m * NN + p + N * sp.idx[2][q][t]] * sp.val[2][q][t];
+        Vbuf[i] = acc;  // (A W)[p][q]
+      }
+      __syncthreads();
+    }
+    zd vinit[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * 64 * GJ_WAVES;
+      const int m = i / NN, e = i % NN, p = e % N, q = e / N;
+      const double sv = Sd[e], cv = Cd[e];
+      const double st = mode == 0 ? 1.5 * sv - 0.5 * cv : mode == 1 ? 0.5 * (sv + cv) : 0.5 * (sv - cv);
+      zd v(st, 0);
+      if (mode == 0) {
+        zd acc(0, 0);
+#pragma unroll
+        for (int t = 0; t < VW_NZ; ++t) acc += Vbuf[m * NN + sp.idx[2][p][t] + N * q] * sp.val[2][p][t];
+        v -= acc * 0.25;
+      }
+      vinit[u] = v;
+    }
+    __syncthreads();  // every read of A W is complete
+#pragma unroll
+    for (int u = 0; u < PER; ++u) Vbuf[tid + u * 64 * GJ_WAVES] = vinit[u];
+    // ---- my row of A K (K = S or C): the right-hand side and I -+ A K
+    zd mrow[N], rrow[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      zd acc(0, 0);
+#pragma unroll
+      for (int t = 0; t < VW_NZ; ++t) acc += A[r + N * sp.idx[sys][j][t]] * sp.val[sys][j][t];
+      rrow[j] = acc;
+      mrow[j] = sys == 0 ? zd(0, 0) - acc : acc;
+      if (j == r) mrow[j] += zd(1.0, 0);
+    }
+    __syncthreads();  // all reads of A are complete before X overwrites it
+    // ---- Gauss-Jordan with partial pivoting.  Rows stay unnormalised: the
+    // pivot lane publishes its row as is, the others form the multiplier
+    // from the published pivot, and each lane divides its right-hand side by
+    // its own pivot once at the end.  The candidates of column k + 1 are
+    // published right after column k + 1 of the row is updated, so their
+    // LDS round trip overlaps the rest of the elimination.
+    bool used = false;
+    int xrow = 0;  // the X row this lane holds at the end
+    bool singular = false;
+    zd dpiv(1.0, 0);
+    pval[r] = abs1(mrow[0]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      wave_sync();
+      int pr = 0;
+      double best = pval[0];
+#pragma unroll
+      for (int i = 1; i < N; ++i) {
+        const double v = pval[i];
+        if (v > best) { best = v; pr = i; }
+      }
+      singular = singular || !(best > 0.0);
+      if (r == pr) {
+#pragma unroll
+        for (int j = k; j < N; ++j) prow[j] = mrow[j];
+#pragma unroll
+        for (int j = 0; j < N; ++j) prow[N + j] = rrow[j];
+        dpiv = mrow[k];
+        used = true;
+        xrow = k;
+      }
+      wave_sync();
+      if (r != pr) {
+        const zd f = mrow[k] * crecip(prow[k]);
+        if (k + 1 < N) {
+          csub_mul(mrow[k + 1], f, prow[k + 1]);
+          pval[r] = used ? -1.0 : abs1(mrow[k + 1]);
+        }
+#pragma unroll
+        for (int j = k + 2; j < N; ++j) csub_mul(mrow[j], f, prow[j]);
+#pragma unroll
+        for (int j = 0; j < N; ++j) csub_mul(rrow[j], f, prow[N + j]);
+      } else if (k + 1 < N) {
+        pval[r] = -1.0;
+      }
+    }
+    {
+      const zd inv = crecip(dpiv);
+#pragma unroll
+      for (int j = 0; j < N; ++j) rrow[j] = rrow[j] * inv;
+    }
+    wave_sync();
+    if (singular && r == 0 && b0 + mloc < batch) atomicOr(err, 1);
+    // ---- V += a1 S X_s, then V += a2 C X_c (X through my matrix's A slot)
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      if (sys == pass) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) A[xrow + N * q] = rrow[q];
+      }
+      __syncthreads();
+      const double coef = pass == 0 ? a1 : a2;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int i = tid + u * 64 * GJ_WAVES;
+        const int m = i / NN, e = i % NN, p = e % N, q = e / N;
+        zd acc(0, 0);
+#pragma unroll
+        for (int t = 0; t < VW_NZ; ++t)
+          acc += Abuf[m * NN + sp.idx[pass][p][t] + N * q] * sp.val[pass][p][t];
+        Vbuf[i] += acc * coef;
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < GJ_MAT * NN; i += 64 * GJ_WAVES) {
+      const int m = i % GJ_MAT, e = i / GJ_MAT;
+      const long b = b0 + m;
+      if (b < batch) chi[b + (long)e * es] = Vbuf[m * NN + e];
+    }
+    __syncthreads();
+  }
+}
+
 // sparse columns of S, C and S + C when every column has at most VW_NZ
 // nonzeros and the matrices are symmetric (rows = columns); false otherwise
 static bool vertex_sparse(const double* S_d, const double* C_d, int n, VSparse& out, hipStream_t s) {
@@ -563,10 +747,17 @@ void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, cons
     if (vertex_sparse(S, C, n, sp, s)) {
       DBuf<VSparse> spd(1);
       spd.upload(&sp, 1, s);
-      const long nb = (batch + VW_MAT - 1) / VW_MAT;
-      const unsigned grid = (unsigned)std::min<long>(nb, 256L * 16);
-      hipLaunchKernelGGL(k_vertex_wave<16>, dim3(grid), dim3(64 * VW_MAT), 0, s, chi, batch, L.es, spd.p, S, C, mode,
-                         err);
+      if (getenv("GMOD_VERTEX_WAVE")) {
+        const long nb = (batch + VW_MAT - 1) / VW_MAT;
+        const unsigned grid = (unsigned)std::min<long>(nb, 256L * 16);
+        hipLaunchKernelGGL(k_vertex_wave<16>, dim3(grid), dim3(64 * VW_MAT), 0, s, chi, batch, L.es, spd.p, S, C,
+                           mode, err);
+      } else {
+        const long nb = (batch + GJ_MAT - 1) / GJ_MAT;
+        const unsigned grid = (unsigned)std::min<long>(nb, 256L * 32);
+        hipLaunchKernelGGL(k_vertex_gj<16>, dim3(grid), dim3(64 * GJ_WAVES), 0, s, chi, batch, L.es, spd.p, S, C,
+                           mode, err);
+      }
       GMOD_CHECK_LAUNCH();
       GMOD_HIP(hipStreamSynchronize(s));  // spd is released at return
       return;
