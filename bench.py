@@ -216,7 +216,7 @@ def main():
         "mu_abs_err_vs_fp64": best["mu_err"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"fft_pass_kernel ({'FP32' if cls == 'fft_pass_fp32' else 'FP64'}), all "
+                     "kernel": f"box_fft_kernel<{'float' if cls == 'fft_pass_fp32' else 'double'}>, all "
                                f"instantiations", "avg_launch_us": round(1e3 * avg_ms, 2),
                      "launches": int(pc["launches"]), "algorithmic_bytes_per_launch": bytes_per},
         "roofline_fp64_fft_pass_gbs": round(ach64, 1),

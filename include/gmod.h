@@ -177,6 +177,14 @@ void gmod_regrid_sigma(gmod_z* sig_f, const gmod_z* sig_c, const int64_t* kmap_c
  * engine does not cover the shape, 2 on an indexing error. */
 int gmod_box_fft_host(double* data, int nx, int ny, int nz, int m, int mask, int sign, int precision,
                       long lds_bytes, int* npasses);
+/* Library extension (no counterpart in the reference; a test and benchmark
+ * hook of the transform engine every convolution above runs on): `reps`
+ * in-place transforms of an (nx,ny,nz,m) grid (x fastest, interleaved
+ * complex) in device memory `buf`, with `tmp` of the same size as the
+ * intermediate buffer, then a synchronisation.  mask, sign and precision as
+ * for gmod_box_fft_host.  Returns 0, or 1 on an error (gmod_last_error). */
+int gmod_fft_device(void* buf, void* tmp, int nx, int ny, int nz, int m, int mask, int sign, int precision,
+                    int reps);
 /* Device memory held by the library's caching pool, in bytes. */
 double gmod_pool_bytes(void);
 void gmod_release_memory(void);

@@ -58,15 +58,16 @@ constexpr int FFT_MAX_LINES = 512;
 template <typename R> struct GridLoad {
   const cpx<R>* p;
   unsigned S;
-  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return p[sp + (size_t)S * t]; }
+  // S, t < 2^24 and S t < 2^31 (checked by the plan): a 24-bit multiply
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return p[sp + __umul24(S, t)]; }
 };
 template <typename R> struct GridStore {
   cpx<R>* p;
   unsigned S;
   struct Pre {};
   __device__ Pre prep(unsigned, unsigned) const { return Pre{}; }
-  __device__ void commit(unsigned sp, unsigned t, const Pre&, cpx<R> v) const { p[sp + (size_t)S * t] = v; }
-  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { p[sp + (size_t)S * t] = v; }
+  __device__ void commit(unsigned sp, unsigned t, const Pre&, cpx<R> v) const { p[sp + __umul24(S, t)] = v; }
+  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { p[sp + __umul24(S, t)] = v; }
   __device__ bool skip(unsigned) const { return false; }
 };
 
@@ -237,6 +238,7 @@ class FFTPlan {
   long tw_len_ = 0;
   BoxPlan<R> box_;                        // box passes (preferred when box_.ok)
   cpx<R>* twb_[2] = {nullptr, nullptr};   // their twiddle pools
+  int* tabs_ = nullptr;                   // their tile tables
 };
 
 inline bool fft_is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -282,7 +284,7 @@ void FFTPlan<R>::launch_box(const BoxArgs& a, LD ld, ST st, const cpx<R>* tw, R 
   }
   ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, 2.0 * (double)size() * sizeof(cpx<R>), s);
   hipLaunchKernelGGL((box_fft_kernel<R, LD, ST>), dim3((unsigned)a.blocks()), dim3(BOX_THREADS), shm, s, ld, st, a,
-                     tw, sc);
+                     tw, (const int*)tabs_, sc);
   GMOD_CHECK_LAUNCH();
 }
 

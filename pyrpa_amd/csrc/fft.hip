@@ -44,11 +44,57 @@ static int largest_divisor_le(int n, long cap) {
   return 1;
 }
 
+// Tiled load/store of a box (see BoxArgs): power-of-two tile extents, x
+// first, whose product is BOX_THREADS; accepted only when every store
+// position is additive over (tile origin, in-tile offset), which is checked
+// here digit by digit for every dimension.
+static void set_tiles(BoxArgs& a, std::vector<int>& tabs) {
+  a.tiled = 0;
+  a.ntiles = 0;
+  if (a.elems % BOX_THREADS != 0 || a.elems / BOX_THREADS > BOX_MAX_TILES) return;
+  if (a.tl_mul != a.ts_mul || a.tl_str != a.ts_str || a.tw4_n) return;
+  int rem = BOX_THREADS, tile[4];
+  for (int d = 0; d < 4; ++d) {
+    int t = 1;
+    while (t * 2 <= rem && a.box[d] % (t * 2) == 0) t *= 2;
+    tile[d] = t;
+    rem /= t;
+    int lg = 0;
+    while ((1 << lg) < t) ++lg;
+    a.tlog[d] = lg;
+  }
+  if (rem != 1) return;
+  for (int d = 0; d < 4; ++d)
+    for (int o = 0; o < a.box[d]; o += tile[d])
+      for (int c = 0; c < tile[d]; ++c)
+        if (box_outpos(a, d, o + c) != box_outpos(a, d, o) + box_outpos(a, d, c)) return;
+  const int nt[4] = {a.box[0] / tile[0], a.box[1] / tile[1], a.box[2] / tile[2], a.box[3] / tile[3]};
+  const int n = a.elems / BOX_THREADS;
+  a.ntiles = n;
+  a.tab_off = (int)tabs.size();
+  tabs.resize(tabs.size() + 4 * (size_t)n);
+  int* t = tabs.data() + a.tab_off;
+  for (int j = 0; j < n; ++j) {
+    int r = j, o[4];
+    for (int d = 0; d < 4; ++d) {
+      o[d] = (r % nt[d]) * tile[d];
+      r /= nt[d];
+    }
+    t[j] = o[0] + (int)a.Nx * (o[1] + (int)a.Ny * o[2]);
+    t[n + j] = o[3] * a.tl_str;
+    t[2 * n + j] = o[0] + o[1] * a.lds_stride[1] + o[2] * a.lds_stride[2] + o[3] * a.lds_stride[3];
+    t[3 * n + j] = box_outpos(a, 0, o[0]) + box_outpos(a, 1, o[1]) * a.lds_stride[1] +
+                   box_outpos(a, 2, o[2]) * a.lds_stride[2] + box_outpos(a, 3, o[3]) * a.lds_stride[3];
+  }
+  a.tiled = 1;
+}
+
 template <typename R>
 void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, size_t lds_bytes) {
   ok = false;
   fwd.clear();
   bwd.clear();
+  tab_host.clear();
   const int n[4] = {nx, ny, nz, m};
   const int maxr = sizeof(R) == 4 ? 32 : 16;
   const long b = (long)sizeof(cpx<R>);
@@ -100,7 +146,7 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
         a.rad[d][i] = r;
         a.radw[d][i] = w;
         if (a.nsub >= BOX_MAX_SUB) return false;
-        BoxSub sb;
+        BoxSub sb{};
         sb.dim = d;
         sb.radix = r;
         sb.L = L;
@@ -110,10 +156,33 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
         L /= r;
       }
     }
+    // multiply-high divisors (exact while elems * divisor < 2^32)
+    auto fd = [&](long d) -> FDiv {
+      if (d < 1 || elems * d >= (1L << 32)) throw Error("box FFT: divisor out of the multiply-high range");
+      return make_fdiv((unsigned)d);
+    };
+    for (int d = 0; d < 3; ++d) a.fbox[d] = fd(ext[d]);
+    for (int d = 0; d < 4; ++d)
+      for (int i = 0; i < BOX_MAX_RAD; ++i) a.frad[d][i] = fd(i < a.nrad[d] ? a.rad[d][i] : 1);
+    for (int i = 0; i < a.nsub; ++i) {
+      BoxSub& sb = a.sub[i];
+      const int d = sb.dim;
+      int od[3], c = 0;
+      for (int e = 0; e < 4; ++e)
+        if (e != d) od[c++] = e;
+      for (int j = 0; j < 3; ++j) sb.os[j] = a.lds_stride[od[j]];
+      sb.n = ext[d];
+      sb.ld = a.lds_stride[d];
+      sb.foe[0] = fd(ext[od[0]]);
+      sb.foe[1] = fd(ext[od[1]]);
+      sb.flines = fd(elems / ext[d]);
+      sb.fs = fd(sb.L / sb.radix);
+    }
     a.Nx = (unsigned)nx;
     a.Ny = (unsigned)ny;
     a.S = (unsigned)((long)nx * ny * nz);
     a.M = (unsigned)m;
+    set_tiles(a, tab_host);
     // bytes contiguous in the grid per box row; boxes sharing a 128 B line
     // are kept on one XCD
     long seg = ext[0] * b;
@@ -217,6 +286,8 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
           tB.tl_str = 1;
           tB.ts_mul = 1;                        // stores t = k1 + M1 k2
           tB.ts_str = M1;
+          set_tiles(tA, tab_host);
+          set_tiles(tB, tab_host);
           split = true;
         }
       }
@@ -285,6 +356,8 @@ FFTPlan<R>::FFTPlan(int nx, int ny, int nz, int m, int axes_mask) {
   S_ = (unsigned)((long)nx * ny * nz);
   const long total = (long)nx * ny * nz * m;
   if (total >= (1L << 31)) throw Error("FFT grid larger than 2^31 points");
+  // the grid functors form sp + S t with 24-bit multiplies
+  if ((long)nx * ny * nz >= (1L << 24) || m >= (1 << 24)) throw Error("FFT grid: k-mesh or slot count >= 2^24");
   std::vector<int> tables;  // lengths needing a twiddle table
   auto add_table = [&](int len) -> long {
     long off = 0;
@@ -373,6 +446,9 @@ FFTPlan<R>::FFTPlan(int nx, int ny, int nz, int m, int axes_mask) {
     box_budget<R>(budget, lds);
     box_.build(nx, ny, nz, m, axes_mask, budget, lds);
     if (box_.ok) {
+      const size_t ntab = box_.tab_host.size();
+      GMOD_HIP(hipMalloc(&tabs_, std::max<size_t>(ntab, 1) * sizeof(int)));
+      if (ntab) GMOD_HIP(hipMemcpy(tabs_, box_.tab_host.data(), ntab * sizeof(int), hipMemcpyHostToDevice));
       for (int sg = 0; sg < 2; ++sg) {
         const size_t nt = box_.tw_host[sg].size();
         GMOD_HIP(hipMalloc(&twb_[sg], std::max<size_t>(nt, 1) * sizeof(cpx<R>)));
@@ -416,6 +492,7 @@ FFTPlan<R>::~FFTPlan() {
     if (p) (void)hipFree(p);
   for (auto p : twb_)
     if (p) (void)hipFree(p);
+  if (tabs_) (void)hipFree(tabs_);
 }
 
 template <typename R>
@@ -513,9 +590,9 @@ int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign
       if (lb >= (unsigned)nb || seen[lb]) throw Error("box remap is not a permutation");
       seen[lb] = 1;
       const BoxOrigin O = box_origin(a, lb);
-      box_load<R>(lds.data(), HostGridLoad<R>{in, S}, a, O, 0, 1);
+      box_load<R>(lds.data(), HostGridLoad<R>{in, S}, a, O, 0, 1, plan.tab_host.data());
       for (int j = 0; j < a.nsub; ++j) box_dispatch<R>(lds.data(), a, a.sub[j], tw, 0, 1);
-      box_store<R>(lds.data(), HostGridStore<R>{out, S}, a, O, scale, tw, 0, 1);
+      box_store<R>(lds.data(), HostGridStore<R>{out, S}, a, O, scale, tw, 0, 1, plan.tab_host.data());
     }
   }
   for (long i = 0; i < N; ++i) { data[2 * i] = (double)g[i].re; data[2 * i + 1] = (double)g[i].im; }
@@ -545,5 +622,37 @@ extern "C" int gmod_box_fft_host(double* data, int nx, int ny, int nz, int m, in
   } catch (std::exception& ex) {
     set_error(ex.what());
     return 2;
+  }
+}
+
+// Device transform of an (nx,ny,nz,m) grid (x fastest, interleaved complex)
+// held by the caller in device memory: `reps` transforms in place on buf with
+// tmp (same size) as the intermediate buffer, on the library's stream, then a
+// stream synchronisation.  sign -1 unnormalised forward, +1 backward divided
+// by the number of transformed points; precision 0 FP64, 1 FP32.  Returns 0,
+// or 1 on an error (message through gmod_last_error).
+extern "C" int gmod_fft_device(void* buf, void* tmp, int nx, int ny, int nz, int m, int mask, int sign,
+                               int precision, int reps) {
+  using namespace gmod;
+  try {
+    if (!buf || !tmp) throw Error("fft: null buffer");
+    std::lock_guard<std::mutex> g(ctx().lock);
+    hipStream_t s = ctx().stream;
+    for (int r = 0; r < std::max(reps, 1); ++r) {
+      if (precision == 1) {
+        const FFTPlan<float>& p = fft_plan<float>(nx, ny, nz, m, mask);
+        const float sc = sign > 0 ? (float)(1.0 / (double)p.transformed_points()) : 1.0f;
+        p.exec(static_cast<cpx<float>*>(buf), static_cast<cpx<float>*>(tmp), sign, sc, s);
+      } else {
+        const FFTPlan<double>& p = fft_plan<double>(nx, ny, nz, m, mask);
+        const double sc = sign > 0 ? 1.0 / (double)p.transformed_points() : 1.0;
+        p.exec(static_cast<cpx<double>*>(buf), static_cast<cpx<double>*>(tmp), sign, sc, s);
+      }
+    }
+    GMOD_HIP(hipStreamSynchronize(s));
+    return 0;
+  } catch (std::exception& ex) {
+    set_error(ex.what());
+    return 1;
   }
 }

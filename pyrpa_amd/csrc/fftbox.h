@@ -28,6 +28,38 @@ namespace gmod {
 constexpr int BOX_THREADS = 512;
 constexpr int BOX_MAX_SUB = 10;
 constexpr int BOX_MAX_RAD = 4;  // sub-passes per dimension
+constexpr int BOX_MAX_TILES = 32;  // tiles of BOX_THREADS elements per box (tiled load/store)
+
+// Division by a launch-invariant divisor as a multiply-high: with
+// m = ceil(2^32 / d), n / d = (n m) >> 32 exactly whenever n (d - 1) < 2^32
+// (the rounding error n (m d - 2^32) / 2^32 < 1 / d then never crosses an
+// integer).  The planner checks that bound for every divisor it sets.
+struct FDiv {
+  unsigned d, m;
+  unsigned one;  // all ones for d = 1 (m = 0), else 0: the quotient is then n
+};
+
+inline FDiv make_fdiv(unsigned d) {
+  FDiv f;
+  f.d = d;
+  f.m = d <= 1 ? 0u : (unsigned)((0xFFFFFFFFull) / d + 1ull);
+  f.one = d <= 1 ? 0xFFFFFFFFu : 0u;
+  return f;
+}
+
+// branch-free: one multiply-high and one and-or
+__host__ __device__ __forceinline__ unsigned fdiv(unsigned n, FDiv f) {
+  return (unsigned)(((unsigned long long)n * f.m) >> 32) | (n & f.one);
+}
+
+// product of two values below 2^24 (full-rate multiply on the device)
+__host__ __device__ __forceinline__ unsigned mul24(unsigned a, unsigned b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul24(a, b);
+#else
+  return a * b;
+#endif
+}
 
 struct BoxSub {
   int dim;      // 0:x 1:y 2:z 3:t
@@ -35,6 +67,13 @@ struct BoxSub {
   int L;        // block length; stride s = L / radix
   int tw_off;   // w_L^m table (sign-specific pool)
   int wr_off;   // w_radix^m table
+  int n;        // box extent along dim
+  int ld;       // LDS stride along dim
+  // line decomposition: the three other dimensions of the box, x first
+  int os[3];    // their LDS strides
+  FDiv foe[2];  // extents of the first two of them
+  FDiv flines;  // elems / box[dim] lines
+  FDiv fs;      // stride L / radix
 };
 
 struct BoxArgs {
@@ -49,6 +88,8 @@ struct BoxArgs {
   int rad[4][BOX_MAX_RAD];
   int radw[4][BOX_MAX_RAD];     // position weight n / (r1 ... r_i)
   unsigned Nx, Ny, S, M;
+  FDiv fbox[3];                 // box[0..2]
+  FDiv frad[4][BOX_MAX_RAD];    // rad
   int xcd_group;                // consecutive boxes that share cache lines (1: no remap)
   int src, dst;                 // intermediate buffers (0: caller buffer, 1: scratch)
   // slot of box element i along t for the box with t index bt:
@@ -57,6 +98,19 @@ struct BoxArgs {
   // a four-step split of the t axis)
   int tl_mul, tl_str, ts_mul, ts_str;
   int tw4_n, tw4_off;           // four-step twiddle w_{tw4_n}^(bt k) on output k along t (0: none)
+  // Tiled load/store (tiled = 1): the box is ntiles tiles of BOX_THREADS
+  // elements with power-of-two extents 2^tlog[d] (x first), aligned with the
+  // radix digits so that the store position of (tile origin + c) is the
+  // position of the origin plus that of c.  A thread keeps its coordinates c
+  // inside a tile; the tables give each tile's offsets from the box origin:
+  // spatial index, slot, natural LDS position (load), digit-reversed LDS
+  // position (store).
+  // The tables live in the plan's int array (not in this kernel argument, whose
+  // copy to scratch memory a run-time index would force): at tab_off, four
+  // rows of ntiles entries: spatial index, slot, load and store LDS offsets.
+  int tiled, ntiles;
+  int tlog[4];
+  int tab_off;
   long blocks() const { return (long)nblk[0] * nblk[1] * nblk[2] * nblk[3]; }
 };
 
@@ -142,10 +196,63 @@ __host__ __device__ inline BoxOrigin box_origin(const BoxArgs& a, unsigned b) {
   return O;
 }
 
+// coordinates of thread tid inside a tile (x fastest)
+__host__ __device__ __forceinline__ void box_tile_coord(const BoxArgs& a, unsigned tid, unsigned (&c)[4]) {
+  c[0] = tid & ((1u << a.tlog[0]) - 1u);
+  const unsigned r0 = tid >> a.tlog[0];
+  c[1] = r0 & ((1u << a.tlog[1]) - 1u);
+  const unsigned r1 = r0 >> a.tlog[1];
+  c[2] = r1 & ((1u << a.tlog[2]) - 1u);
+  c[3] = r1 >> a.tlog[2];
+}
+
+// spatial index of the box origin + (c0, c1, c2)
+__host__ __device__ __forceinline__ unsigned box_sp(const BoxArgs& a, const BoxOrigin& O, unsigned c0, unsigned c1,
+                                                    unsigned c2) {
+  return (unsigned)O.o[0] + c0 + mul24(a.Nx, (unsigned)O.o[1] + c1) + mul24(a.Nx * a.Ny, (unsigned)O.o[2] + c2);
+}
+
+template <typename R, class LD>
+__host__ __device__ __forceinline__ void box_load_tiled(cpx<R>* lds, const LD& ld, const BoxArgs& a,
+                                                        const BoxOrigin& O, unsigned tid,
+                                                        const int* __restrict__ tabs) {
+  const int* tsp = tabs + a.tab_off;
+  const int* tt = tsp + a.ntiles;
+  const int* tld = tt + a.ntiles;
+  constexpr int U = sizeof(R) == 4 ? 16 : 8;
+  unsigned c[4];
+  box_tile_coord(a, tid, c);
+  const unsigned sp0 = box_sp(a, O, c[0], c[1], c[2]);
+  const unsigned t0 = mul24((unsigned)O.bt, (unsigned)a.tl_mul) + mul24(c[3], (unsigned)a.tl_str);
+  const unsigned l0 = c[0] + mul24(c[1], (unsigned)a.lds_stride[1]) + mul24(c[2], (unsigned)a.lds_stride[2]) +
+                      mul24(c[3], (unsigned)a.lds_stride[3]);
+  for (int j0 = 0; j0 < a.ntiles; j0 += U) {
+    cpx<R> v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j0 + u < a.ntiles) {
+        const unsigned sp = sp0 + (unsigned)tsp[j0 + u], t = t0 + (unsigned)tt[j0 + u];
+        GMOD_BOX_CHECK(sp < a.S && t < a.M, "tiled load grid");
+        v[u] = ld(sp, t);
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j0 + u < a.ntiles) {
+        const unsigned off = l0 + (unsigned)tld[j0 + u];
+        GMOD_BOX_CHECK(off < (unsigned)a.lds_elems, "tiled load lds");
+        lds[off] = v[u];
+      }
+  }
+}
+
 // ---- phase 1: grid -> LDS (x fastest: coalesced along the grid's contiguous axis)
 template <typename R, class LD>
 __host__ __device__ __forceinline__ void box_load(cpx<R>* lds, const LD& ld, const BoxArgs& a, const BoxOrigin& O,
-                                                  int tid, int nthr) {
+                                                  int tid, int nthr, const int* __restrict__ tabs) {
+  if (a.tiled) {  // one pass on the device (nthr = BOX_THREADS); the host emulation loops over the threads
+    for (int t = tid; t < BOX_THREADS; t += nthr) box_load_tiled<R>(lds, ld, a, O, (unsigned)t, tabs);
+    return;
+  }
   // U loads in flight per thread before the first LDS store: one work-group
   // per CU has to keep ~64 KB of HBM reads outstanding
   constexpr int U = sizeof(R) == 4 ? 16 : 8;
@@ -158,17 +265,18 @@ __host__ __device__ __forceinline__ void box_load(cpx<R>* lds, const LD& ld, con
       const int e = e0 + u * nthr;
       off[u] = -1;
       if (e < a.elems) {
-        int q = e;
-        const int ix = q % ex; q /= ex;
-        const int iy = q % ey; q /= ey;
-        const int iz = q % ez;
-        const int it = q / ez;
+        const unsigned q1 = fdiv((unsigned)e, a.fbox[0]), q2 = fdiv(q1, a.fbox[1]);
+        const int it = (int)fdiv(q2, a.fbox[2]);
+        const int ix = e - (int)mul24(q1, ex);
+        const int iy = (int)(q1 - mul24(q2, ey));
+        const int iz = (int)(q2 - mul24(it, ez));
         const unsigned x = (unsigned)(O.o[0] + ix), y = (unsigned)(O.o[1] + iy), z = (unsigned)(O.o[2] + iz);
-        const unsigned t = (unsigned)(O.bt * a.tl_mul + it * a.tl_str);
-        const unsigned sp = x + a.Nx * (y + a.Ny * z);
+        const unsigned t = mul24((unsigned)O.bt, (unsigned)a.tl_mul) + mul24((unsigned)it, (unsigned)a.tl_str);
+        const unsigned sp = x + mul24(a.Nx, y) + mul24(a.Nx * a.Ny, z);
         GMOD_BOX_CHECK(sp < a.S && t < a.M, "load grid");
         v[u] = ld(sp, t);
-        off[u] = ix * a.lds_stride[0] + iy * a.lds_stride[1] + iz * a.lds_stride[2] + it * a.lds_stride[3];
+        off[u] = ix + (int)(mul24((unsigned)iy, (unsigned)a.lds_stride[1]) + mul24((unsigned)iz, (unsigned)a.lds_stride[2]) +
+                            mul24((unsigned)it, (unsigned)a.lds_stride[3]));
         GMOD_BOX_CHECK(off[u] < a.lds_elems, "load lds");
       }
     }
@@ -182,31 +290,23 @@ __host__ __device__ __forceinline__ void box_load(cpx<R>* lds, const LD& ld, con
 template <typename R, int RAD>
 __host__ __device__ __forceinline__ void box_subpass(cpx<R>* lds, const BoxArgs& a, const BoxSub& sp,
                                                      const cpx<R>* __restrict__ tw, int tid, int nthr) {
-  const int d = sp.dim;
-  const int n = a.box[d];
+  const int n = sp.n;
   const int L = sp.L, s = L / RAD;
-  const int lines = a.elems / n;
+  const int lines = (int)sp.flines.d;
   const int groups = a.elems / RAD;
-  const int ld = a.lds_stride[d];
-  // the other three dimensions of a line, x first (selects, not a local
-  // array with a run-time index, which would live in scratch memory)
-  const int q0 = d == 0 ? 1 : 0, q1 = d <= 1 ? 2 : 1, q2 = d <= 2 ? 3 : 2;
-  auto pick = [](const int* v, int q) { return q == 0 ? v[0] : q == 1 ? v[1] : q == 2 ? v[2] : v[3]; };
-  const int oe[3] = {pick(a.box, q0), pick(a.box, q1), pick(a.box, q2)};
-  const int os[3] = {pick(a.lds_stride, q0), pick(a.lds_stride, q1), pick(a.lds_stride, q2)};
+  const int ld = sp.ld;
   const cpx<R>* wr = tw + sp.wr_off;
   const cpx<R>* wl = tw + sp.tw_off;
   for (int g = tid; g < groups; g += nthr) {
-    const int line = g % lines, gl = g / lines;
-    int rem = line;
-    const int c0 = rem % oe[0];
-    rem /= oe[0];
-    const int c1 = rem % oe[1];
-    const int c2 = rem / oe[1];
-    const int base = c0 * os[0] + c1 * os[1] + c2 * os[2];
-    const int blk = gl / s, jp = gl - blk * s;
-    const int p0 = blk * L + jp;
-    GMOD_BOX_CHECK(c2 < oe[2] && p0 + s * (RAD - 1) < n, "sub-pass position");
+    const unsigned gl = fdiv((unsigned)g, sp.flines);
+    const unsigned line = (unsigned)g - mul24(gl, (unsigned)lines);
+    const unsigned r1 = fdiv(line, sp.foe[0]), c2 = fdiv(r1, sp.foe[1]);
+    const unsigned c0 = line - mul24(r1, sp.foe[0].d), c1 = r1 - mul24(c2, sp.foe[1].d);
+    const int base = (int)(mul24(c0, (unsigned)sp.os[0]) + mul24(c1, (unsigned)sp.os[1]) + mul24(c2, (unsigned)sp.os[2]));
+    const unsigned blk = fdiv(gl, sp.fs);
+    const int jp = (int)(gl - mul24(blk, (unsigned)s));
+    const int p0 = (int)mul24(blk, (unsigned)L) + jp;
+    GMOD_BOX_CHECK(line < (unsigned)lines && p0 + s * (RAD - 1) < n, "sub-pass position");
     GMOD_BOX_CHECK(base + (p0 + s * (RAD - 1)) * ld < a.lds_elems, "sub-pass lds");
     cpx<R> v[RAD];
 #pragma unroll
@@ -219,6 +319,25 @@ __host__ __device__ __forceinline__ void box_subpass(cpx<R>* lds, const BoxArgs&
     }
 #pragma unroll
     for (int k = 0; k < RAD; ++k) lds[base + (p0 + s * k) * ld] = v[k];
+  }
+}
+
+// a.sub[i] read with constant indices only: an index into the by-value
+// kernel argument that is not a constant makes the compiler copy the whole
+// argument to scratch memory
+__host__ __device__ __forceinline__ BoxSub box_sub_at(const BoxArgs& a, int i) {
+  static_assert(BOX_MAX_SUB == 10, "one case per sub-pass slot");
+  switch (i) {
+    case 0: return a.sub[0];
+    case 1: return a.sub[1];
+    case 2: return a.sub[2];
+    case 3: return a.sub[3];
+    case 4: return a.sub[4];
+    case 5: return a.sub[5];
+    case 6: return a.sub[6];
+    case 7: return a.sub[7];
+    case 8: return a.sub[8];
+    default: return a.sub[9];
   }
 }
 
@@ -245,20 +364,68 @@ __host__ __device__ __forceinline__ void box_dispatch(cpx<R>* lds, const BoxArgs
 __host__ __device__ __forceinline__ int box_outpos(const BoxArgs& a, int d, int k) {
   const int nr = a.nrad[d];
   if (nr <= 1) return k;
-  int p = 0;
-  for (int i = 0; i < nr; ++i) {
-    const int r = a.rad[d][i];
-    p += (k % r) * a.radw[d][i];
-    k /= r;
+  unsigned p = 0, q = (unsigned)k;
+#pragma unroll
+  for (int i = 0; i < BOX_MAX_RAD; ++i) {
+    if (i < nr) {
+      const unsigned qn = fdiv(q, a.frad[d][i]);
+      p += mul24(q - mul24(qn, (unsigned)a.rad[d][i]), (unsigned)a.radw[d][i]);
+      q = qn;
+    }
   }
-  return p;
+  return (int)p;
+}
+
+template <typename R, class ST>
+__host__ __device__ __forceinline__ void box_store_tiled(const cpx<R>* lds, const ST& st, const BoxArgs& a,
+                                                         const BoxOrigin& O, R scale, unsigned tid,
+                                                         const int* __restrict__ tabs) {
+  constexpr int U = 8;
+  const int* tsp = tabs + a.tab_off;
+  const int* tt = tsp + a.ntiles;
+  const int* tst = tt + 2 * a.ntiles;
+  unsigned c[4];
+  box_tile_coord(a, tid, c);
+  const unsigned sp0 = box_sp(a, O, c[0], c[1], c[2]);
+  const unsigned t0 = mul24((unsigned)O.bt, (unsigned)a.ts_mul) + mul24(c[3], (unsigned)a.ts_str);
+  const unsigned l0 = (unsigned)box_outpos(a, 0, (int)c[0]) +
+                      mul24((unsigned)box_outpos(a, 1, (int)c[1]), (unsigned)a.lds_stride[1]) +
+                      mul24((unsigned)box_outpos(a, 2, (int)c[2]), (unsigned)a.lds_stride[2]) +
+                      mul24((unsigned)box_outpos(a, 3, (int)c[3]), (unsigned)a.lds_stride[3]);
+  for (int j0 = 0; j0 < a.ntiles; j0 += U) {
+    typename ST::Pre pre[U];
+    unsigned spv[U], tv[U], offv[U];
+    bool keep[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      keep[u] = false;
+      if (j0 + u < a.ntiles) {
+        spv[u] = sp0 + (unsigned)tsp[j0 + u];
+        tv[u] = t0 + (unsigned)tt[j0 + u];
+        offv[u] = l0 + (unsigned)tst[j0 + u];
+        if (!st.skip(tv[u])) {
+          GMOD_BOX_CHECK(spv[u] < a.S && tv[u] < a.M, "tiled store grid");
+          GMOD_BOX_CHECK(offv[u] < (unsigned)a.lds_elems, "tiled store lds");
+          keep[u] = true;
+          pre[u] = st.prep(spv[u], tv[u]);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (keep[u]) st.commit(spv[u], tv[u], pre[u], lds[offv[u]] * scale);
+  }
 }
 
 // ---- phase 3: LDS -> grid through the store functor
 template <typename R, class ST>
 __host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& st, const BoxArgs& a,
                                                    const BoxOrigin& O, R scale, const cpx<R>* __restrict__ tw,
-                                                   int tid, int nthr) {
+                                                   int tid, int nthr, const int* __restrict__ tabs) {
+  if (a.tiled) {
+    for (int t = tid; t < BOX_THREADS; t += nthr) box_store_tiled<R>(lds, st, a, O, scale, (unsigned)t, tabs);
+    return;
+  }
   // batches of U elements: every load a store functor needs (accumulators,
   // operands, gather indices) is issued before the batch's first store
   constexpr int U = 8;
@@ -273,18 +440,20 @@ __host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& 
       const int e = e0 + u * nthr;
       offv[u] = -1;
       if (e < a.elems) {
-        int q = e;
-        const int kx = q % ex; q /= ex;
-        const int ky = q % ey; q /= ey;
-        const int kz = q % ez;
-        const int kt = q / ez;
-        const unsigned t = (unsigned)(O.bt * a.ts_mul + kt * a.ts_str);
+        const unsigned q1 = fdiv((unsigned)e, a.fbox[0]), q2 = fdiv(q1, a.fbox[1]);
+        const int kt = (int)fdiv(q2, a.fbox[2]);
+        const int kx = e - (int)mul24(q1, ex);
+        const int ky = (int)(q1 - mul24(q2, ey));
+        const int kz = (int)(q2 - mul24(kt, ez));
+        const unsigned t = mul24((unsigned)O.bt, (unsigned)a.ts_mul) + mul24((unsigned)kt, (unsigned)a.ts_str);
         if (!st.skip(t)) {
-          offv[u] = box_outpos(a, 0, kx) * a.lds_stride[0] + box_outpos(a, 1, ky) * a.lds_stride[1] +
-                    box_outpos(a, 2, kz) * a.lds_stride[2] + box_outpos(a, 3, kt) * a.lds_stride[3];
+          offv[u] = box_outpos(a, 0, kx) +
+                    (int)(mul24((unsigned)box_outpos(a, 1, ky), (unsigned)a.lds_stride[1]) +
+                          mul24((unsigned)box_outpos(a, 2, kz), (unsigned)a.lds_stride[2]) +
+                          mul24((unsigned)box_outpos(a, 3, kt), (unsigned)a.lds_stride[3]));
           GMOD_BOX_CHECK(offv[u] < a.lds_elems, "store lds");
           const unsigned x = (unsigned)(O.o[0] + kx), y = (unsigned)(O.o[1] + ky), z = (unsigned)(O.o[2] + kz);
-          spv[u] = x + a.Nx * (y + a.Ny * z);
+          spv[u] = x + mul24(a.Nx, y) + mul24(a.Nx * a.Ny, z);
           tv[u] = t;
           GMOD_BOX_CHECK(spv[u] < a.S && t < a.M, "store grid");
           pre[u] = st.prep(spv[u], t);
@@ -307,7 +476,7 @@ __host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& 
 
 template <typename R, class LD, class ST>
 __global__ void __launch_bounds__(BOX_THREADS)
-box_fft_kernel(LD ld, ST st, BoxArgs a, const cpx<R>* __restrict__ tw, R scale) {
+box_fft_kernel(LD ld, ST st, BoxArgs a, const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const BoxOrigin O = box_origin(a, box_remap(blockIdx.x, gridDim.x, a.xcd_group));
@@ -316,13 +485,13 @@ box_fft_kernel(LD ld, ST st, BoxArgs a, const cpx<R>* __restrict__ tw, R scale) 
   const unsigned t_first = (unsigned)(O.bt * a.ts_mul), t_last = t_first + (unsigned)((a.box[3] - 1) * a.ts_str);
   if (st.skip(t_first) && st.skip(t_last)) return;
   const int tid = threadIdx.x;
-  box_load<R>(lds, ld, a, O, tid, BOX_THREADS);
+  box_load<R>(lds, ld, a, O, tid, BOX_THREADS, tabs);
   __syncthreads();
   for (int i = 0; i < a.nsub; ++i) {
-    box_dispatch<R>(lds, a, a.sub[i], tw, tid, BOX_THREADS);
+    box_dispatch<R>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS);
     __syncthreads();
   }
-  box_store<R>(lds, st, a, O, scale, tw, tid, BOX_THREADS);
+  box_store<R>(lds, st, a, O, scale, tw, tid, BOX_THREADS, tabs);
 }
 
 // Host-side plan of box passes (built without a device).
@@ -331,6 +500,7 @@ struct BoxPlan {
   bool ok = false;
   std::vector<BoxArgs> fwd, bwd;
   std::vector<cpx<R>> tw_host[2];  // [0]: sign -1, [1]: sign +1
+  std::vector<int> tab_host;       // tile tables of the tiled passes
   size_t max_lds_bytes = 0;  // largest box of this plan
   size_t lds_limit = 0;      // the device's per-work-group LDS limit the plan was built for
   // plan the transform of the axes in `mask` of an (nx,ny,nz,m) grid with

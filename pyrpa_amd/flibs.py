@@ -6,7 +6,7 @@ Fortran shapes), so a caller of ``libs.flibs`` can switch by changing the
 import.  Differences: errors raise ``RuntimeError`` instead of terminating the
 process (the reference prints and STOPs), and the GPU library is required.
 """
-from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int32, c_int64
+from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p
 
 import numpy as np
 
@@ -83,6 +83,8 @@ def _L():
         L.gmod_profile_read.restype = c_int
         L.gmod_set_verbose.argtypes = [c_int]
         L.gmod_pool_bytes.restype = c_double
+        L.gmod_fft_device.argtypes = [c_void_p, c_void_p] + [c_int] * 8
+        L.gmod_fft_device.restype = c_int
         _bound = True
     return L
 
@@ -427,6 +429,25 @@ def release_memory():
     """Return the unused blocks of the library's device pool to HIP."""
     _L().gmod_release_memory()
     check()
+
+
+def fft_device(buf, tmp, dims, sign=-1, mask=15, reps=1):
+    """In-place transform of a device grid (gmod_fft_device): buf and tmp are
+    contiguous torch complex64 (FP32 arithmetic) or complex128 (FP64) device
+    tensors of nx*ny*nz*m elements, x fastest; dims = (nx, ny, nz, m)."""
+    import torch
+    if buf.dtype not in (torch.complex64, torch.complex128) or tmp.dtype != buf.dtype:
+        raise TypeError("fft_device: complex64 or complex128 tensors of one dtype")
+    if not (buf.is_cuda and tmp.is_cuda and buf.is_contiguous() and tmp.is_contiguous()):
+        raise ValueError("fft_device: contiguous device tensors")
+    nx, ny, nz, m = (int(v) for v in dims)
+    if buf.numel() != nx * ny * nz * m or tmp.numel() != buf.numel():
+        raise ValueError("fft_device: tensor size does not match dims")
+    prec = 1 if buf.dtype == torch.complex64 else 0
+    rc = _L().gmod_fft_device(buf.data_ptr(), tmp.data_ptr(), nx, ny, nz, m, int(mask), int(sign), prec, int(reps))
+    check()
+    if rc != 0:
+        raise RuntimeError("gmod_fft_device failed")
 
 
 def regrid_sigma(sigmak_c, kmap_c, invk_c, mesh_c, kmap_f, invk_f, mesh_f, temp):

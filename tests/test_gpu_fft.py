@@ -1,0 +1,108 @@
+"""The transform engine on the device (gmod_fft_device) against numpy's FFT,
+at the grid shapes the convolutions run on, including the 4-orbital
+32x32x16 k-mesh x 2048 imaginary-time slots of the FLEX benchmark.
+
+Conventions (the reference's FFT wrapper, fmod.f90:67-160): sign -1 is the
+unnormalised forward transform exp(-2 pi i k.x / n), sign +1 the backward
+transform divided by the number of transformed points.  The grid is x
+fastest, so a numpy array of shape (m, nz, ny, nx) in C order is the same
+memory.  Tolerances: FP64 2e-13 and FP32 2e-6 of the RMS of the exact
+result, per element, over the whole grid (a radix or twiddle slip gives O(1)
+errors)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    (32, 32, 16, 2048),   # FLEX benchmark grid
+    (16, 16, 8, 1024),    # its coarse warm-start mesh
+    (24, 24, 1, 320),     # mixed radices, 2D k-mesh
+    (8, 8, 8, 64),
+    (5, 7, 3, 42),        # odd radices
+]
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    import pyrpa_amd.flibs as F
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device")
+    return F, torch
+
+
+def _axes(mask):
+    # numpy axes of the (m, nz, ny, nx) view for mask bits x, y, z, t
+    return tuple(ax for bit, ax in ((3, 0), (2, 1), (1, 2), (0, 3)) if (mask >> bit) & 1)
+
+
+def _run(env, a, dims, sign, mask, dtype):
+    F, torch = env
+    buf = torch.from_numpy(a.astype(dtype).ravel()).cuda()
+    tmp = torch.empty_like(buf)
+    F.fft_device(buf, tmp, dims, sign=sign, mask=mask)
+    return buf.cpu().numpy().reshape(a.shape)
+
+
+def _check(got, ref, tol, what):
+    rms = np.sqrt(np.mean(np.abs(ref) ** 2))
+    err = np.max(np.abs(got - ref)) / rms
+    assert err < tol, f"{what}: max error {err:.3e} of the RMS (tolerance {tol:g})"
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_forward_backward_vs_numpy(env, dims, prec):
+    import scipy.fft as sf
+    nx, ny, nz, m = dims
+    rng = np.random.default_rng(nx * 1000 + m)
+    a = rng.standard_normal((m, nz, ny, nx)) + 1j * rng.standard_normal((m, nz, ny, nx))
+    dtype, tol = (np.complex128, 2e-13) if prec == "fp64" else (np.complex64, 2e-6)
+    a = a.astype(dtype).astype(np.complex128)  # the exact input the device sees
+    ref = sf.fftn(a, axes=(0, 1, 2, 3), workers=8)
+    got = _run(env, a, dims, -1, 15, dtype)
+    _check(got, ref, tol, f"forward {dims} {prec}")
+    back = _run(env, ref, dims, +1, 15, dtype)
+    ref_b = sf.ifftn(ref.astype(dtype).astype(np.complex128), axes=(0, 1, 2, 3), workers=8)
+    _check(back, ref_b, tol, f"backward {dims} {prec}")
+
+
+@pytest.mark.parametrize("mask", [7, 8, 3, 12])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_axis_subsets(env, mask, prec):
+    import scipy.fft as sf
+    dims = (16, 16, 8, 1024)
+    nx, ny, nz, m = dims
+    rng = np.random.default_rng(mask)
+    a = rng.standard_normal((m, nz, ny, nx)) + 1j * rng.standard_normal((m, nz, ny, nx))
+    dtype, tol = (np.complex128, 2e-13) if prec == "fp64" else (np.complex64, 2e-6)
+    a = a.astype(dtype).astype(np.complex128)
+    ref = sf.fftn(a, axes=_axes(mask), workers=8)
+    _check(_run(env, a, dims, -1, mask, dtype), ref, tol, f"mask {mask} {prec}")
+
+
+def test_repeated_transforms(env):
+    """reps > 1 applies the transform that many times: four unnormalised
+    forward transforms of a grid return N^2 times the input (F^2 x = N x(-x),
+    F^4 x = N^2 x)."""
+    F, torch = env
+    dims = (8, 8, 8, 64)
+    nx, ny, nz, m = dims
+    N = nx * ny * nz * m
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    buf = torch.from_numpy(a.astype(np.complex128)).cuda()
+    tmp = torch.empty_like(buf)
+    F.fft_device(buf, tmp, dims, sign=-1, mask=15, reps=4)
+    _check(buf.cpu().numpy() / float(N) ** 2, a, 1e-12, "F^4 = N^2")
+
+
+def test_rejects_bad_arguments(env):
+    F, torch = env
+    buf = torch.zeros(64, dtype=torch.complex64, device="cuda")
+    with pytest.raises(ValueError):
+        F.fft_device(buf, torch.zeros(32, dtype=torch.complex64, device="cuda"), (4, 4, 4, 1))
+    with pytest.raises(TypeError):
+        F.fft_device(buf, torch.zeros(64, dtype=torch.complex128, device="cuda"), (4, 4, 4, 1))
