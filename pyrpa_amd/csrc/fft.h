@@ -229,7 +229,7 @@ class FFTPlan {
   template <class LD, class ST>
   void launch(const FFTPassDesc& p, LD ld, ST st, const cpx<R>* tw, R sc, hipStream_t s) const;
   template <class LD, class ST>
-  void launch_box(const BoxArgs& a, LD ld, ST st, const cpx<R>* tw, R sc, hipStream_t s) const;
+  void launch_box(int sg, int i, LD ld, ST st, const cpx<R>* tw, R sc, hipStream_t s) const;
   int dims_[4];
   int mask_;
   unsigned S_ = 1;
@@ -239,6 +239,8 @@ class FFTPlan {
   BoxPlan<R> box_;                        // box passes (preferred when box_.ok)
   cpx<R>* twb_[2] = {nullptr, nullptr};   // their twiddle pools
   int* tabs_ = nullptr;                   // their tile tables
+  BoxArgs* argsd_[2] = {nullptr, nullptr};  // device copies of box_.fwd / box_.bwd
+  long persist_ = 256;                    // work-groups of a persistent tiled pass
 };
 
 inline bool fft_is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -274,8 +276,9 @@ void FFTPlan<R>::launch(const FFTPassDesc& p, LD ld, ST st, const cpx<R>* tw, R 
 
 template <typename R>
 template <class LD, class ST>
-void FFTPlan<R>::launch_box(const BoxArgs& a, LD ld, ST st, const cpx<R>* tw, R sc, hipStream_t s) const {
-  const size_t shm = (size_t)a.lds_elems * sizeof(cpx<R>);
+void FFTPlan<R>::launch_box(int sg, int i, LD ld, ST st, const cpx<R>* tw, R sc, hipStream_t s) const {
+  const BoxArgs& a = (sg == 0 ? box_.fwd : box_.bwd)[i];
+  const size_t shm = (size_t)(a.lds_elems + a.ntwl) * sizeof(cpx<R>);
   static bool attr_done = false;  // one per kernel instantiation: allow the device's whole LDS
   if (!attr_done) {
     GMOD_HIP(hipFuncSetAttribute((const void*)&box_fft_kernel<R, LD, ST>,
@@ -283,8 +286,11 @@ void FFTPlan<R>::launch_box(const BoxArgs& a, LD ld, ST st, const cpx<R>* tw, R 
     attr_done = true;
   }
   ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, 2.0 * (double)size() * sizeof(cpx<R>), s);
-  hipLaunchKernelGGL((box_fft_kernel<R, LD, ST>), dim3((unsigned)a.blocks()), dim3(BOX_THREADS), shm, s, ld, st, a,
-                     tw, (const int*)tabs_, sc);
+  // tiled passes: persistent work-groups, one per CU, a multiple of 8
+  long grid = a.blocks();
+  if (a.tiled && grid > persist_) grid = persist_;
+  hipLaunchKernelGGL((box_fft_kernel<R, LD, ST>), dim3((unsigned)grid), dim3(BOX_THREADS), shm, s, ld, st,
+                     (const BoxArgs*)(argsd_[sg] + i), tw, (const int*)tabs_, sc);
   GMOD_CHECK_LAUNCH();
 }
 
@@ -292,18 +298,19 @@ template <typename R>
 template <class LD, class ST>
 void FFTPlan<R>::exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R scale, hipStream_t s) const {
   if (box_.ok) {
-    const cpx<R>* twb = twb_[sign < 0 ? 0 : 1];
-    const std::vector<BoxArgs>& bp = sign < 0 ? box_.fwd : box_.bwd;
+    const int sg = sign < 0 ? 0 : 1;
+    const cpx<R>* twb = twb_[sg];
+    const std::vector<BoxArgs>& bp = sg == 0 ? box_.fwd : box_.bwd;
     const int nb = (int)bp.size();
     auto bo = [&](int which) { return which ? tmp : buf; };
     for (int i = 0; i < nb; ++i) {
       const BoxArgs& a = bp[i];
       const R sc = (i == nb - 1) ? scale : (R)1;
       const bool first = i == 0, last = i == nb - 1;
-      if (first && last) launch_box(a, ld, st, twb, sc, s);
-      else if (first) launch_box(a, ld, GridStore<R>{bo(a.dst), S_}, twb, sc, s);
-      else if (last) launch_box(a, GridLoad<R>{bo(a.src), S_}, st, twb, sc, s);
-      else launch_box(a, GridLoad<R>{bo(a.src), S_}, GridStore<R>{bo(a.dst), S_}, twb, sc, s);
+      if (first && last) launch_box(sg, i, ld, st, twb, sc, s);
+      else if (first) launch_box(sg, i, ld, GridStore<R>{bo(a.dst), S_}, twb, sc, s);
+      else if (last) launch_box(sg, i, GridLoad<R>{bo(a.src), S_}, st, twb, sc, s);
+      else launch_box(sg, i, GridLoad<R>{bo(a.src), S_}, GridStore<R>{bo(a.dst), S_}, twb, sc, s);
     }
     return;
   }

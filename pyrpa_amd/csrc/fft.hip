@@ -48,10 +48,10 @@ static int largest_divisor_le(int n, long cap) {
 // first, whose product is BOX_THREADS; accepted only when every store
 // position is additive over (tile origin, in-tile offset), which is checked
 // here digit by digit for every dimension.
-static void set_tiles(BoxArgs& a, std::vector<int>& tabs) {
+static void set_tiles(BoxArgs& a, std::vector<int>& tabs, int max_tiles) {
   a.tiled = 0;
   a.ntiles = 0;
-  if (a.elems % BOX_THREADS != 0 || a.elems / BOX_THREADS > BOX_MAX_TILES) return;
+  if (a.elems % BOX_THREADS != 0 || a.elems / BOX_THREADS > std::min(max_tiles, BOX_MAX_TILES)) return;
   if (a.tl_mul != a.ts_mul || a.tl_str != a.ts_str || a.tw4_n) return;
   int rem = BOX_THREADS, tile[4];
   for (int d = 0; d < 4; ++d) {
@@ -86,6 +86,12 @@ static void set_tiles(BoxArgs& a, std::vector<int>& tabs) {
     t[3 * n + j] = box_outpos(a, 0, o[0]) + box_outpos(a, 1, o[1]) * a.lds_stride[1] +
                    box_outpos(a, 2, o[2]) * a.lds_stride[2] + box_outpos(a, 3, o[3]) * a.lds_stride[3];
   }
+  // the prefetch of the next box goes after the last sub-pass that reads
+  // twiddles (stride L / radix > 1) from global memory: reads issued after
+  // it would wait for the prefetch (memory returns in order)
+  a.pf_at = 0;
+  for (int i = 0; i < a.nsub; ++i)
+    if (a.sub[i].L / a.sub[i].radix > 1) a.pf_at = i + 1;
   a.tiled = 1;
 }
 
@@ -178,11 +184,44 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
       sb.flines = fd(elems / ext[d]);
       sb.fs = fd(sb.L / sb.radix);
     }
+    // twiddles of the sub-passes with stride > 1 from per-dimension w_n
+    // tables in LDS after the box, when they fit
+    a.ntwl = a.ntwc = 0;
+    for (int i = 0; i < a.nsub; ++i) {
+      a.sub[i].twl = -1;
+      a.sub[i].tws = 1;
+    }
+    {
+      long need = 0;
+      for (int d : tdims)
+        if (a.nrad[d] >= 2) need += ext[d];
+      if (need > 0 && (lds_elems + need) * b <= (long)lds_bytes && a.nsub > 0) {
+        int dst = (int)lds_elems;
+        for (int d : tdims) {
+          if (a.nrad[d] < 2) continue;
+          int first = -1;
+          for (int i = 0; i < a.nsub; ++i)
+            if (a.sub[i].dim == d && first < 0) first = i;
+          a.twc_src[a.ntwc] = a.sub[first].tw_off;  // the w_{ext[d]} table (L = ext[d] on the first sub-pass)
+          a.twc_dst[a.ntwc] = dst;
+          a.twc_len[a.ntwc] = ext[d];
+          ++a.ntwc;
+          for (int i = 0; i < a.nsub; ++i)
+            if (a.sub[i].dim == d && a.sub[i].L / a.sub[i].radix > 1) {
+              a.sub[i].twl = dst;
+              a.sub[i].tws = ext[d] / a.sub[i].L;
+            }
+          dst += ext[d];
+        }
+        a.ntwl = dst - (int)lds_elems;
+        max_lds_bytes = std::max(max_lds_bytes, (size_t)((lds_elems + a.ntwl) * b));
+      }
+    }
     a.Nx = (unsigned)nx;
     a.Ny = (unsigned)ny;
     a.S = (unsigned)((long)nx * ny * nz);
     a.M = (unsigned)m;
-    set_tiles(a, tab_host);
+    set_tiles(a, tab_host, BOX_TILE_REGS<R>);
     // bytes contiguous in the grid per box row; boxes sharing a 128 B line
     // are kept on one XCD
     long seg = ext[0] * b;
@@ -286,8 +325,8 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
           tB.tl_str = 1;
           tB.ts_mul = 1;                        // stores t = k1 + M1 k2
           tB.ts_str = M1;
-          set_tiles(tA, tab_host);
-          set_tiles(tB, tab_host);
+          set_tiles(tA, tab_host, BOX_TILE_REGS<R>);
+          set_tiles(tB, tab_host, BOX_TILE_REGS<R>);
           split = true;
         }
       }
@@ -446,6 +485,15 @@ FFTPlan<R>::FFTPlan(int nx, int ny, int nz, int m, int axes_mask) {
     box_budget<R>(budget, lds);
     box_.build(nx, ny, nz, m, axes_mask, budget, lds);
     if (box_.ok) {
+      int dev = 0, ncu = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu >= 8)
+        persist_ = (long)(ncu / 8) * 8;
+      for (int sg = 0; sg < 2; ++sg) {
+        const std::vector<BoxArgs>& v = sg == 0 ? box_.fwd : box_.bwd;
+        GMOD_HIP(hipMalloc(&argsd_[sg], std::max<size_t>(v.size(), 1) * sizeof(BoxArgs)));
+        if (!v.empty()) GMOD_HIP(hipMemcpy(argsd_[sg], v.data(), v.size() * sizeof(BoxArgs), hipMemcpyHostToDevice));
+      }
       const size_t ntab = box_.tab_host.size();
       GMOD_HIP(hipMalloc(&tabs_, std::max<size_t>(ntab, 1) * sizeof(int)));
       if (ntab) GMOD_HIP(hipMemcpy(tabs_, box_.tab_host.data(), ntab * sizeof(int), hipMemcpyHostToDevice));
@@ -493,6 +541,8 @@ FFTPlan<R>::~FFTPlan() {
   for (auto p : twb_)
     if (p) (void)hipFree(p);
   if (tabs_) (void)hipFree(tabs_);
+  for (auto p : argsd_)
+    if (p) (void)hipFree(p);
 }
 
 template <typename R>
@@ -573,7 +623,7 @@ int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign
   for (size_t i = 0; i < ps.size(); ++i) {
     const BoxArgs& a = ps[i];
     const R scale = (sign > 0 && i + 1 == ps.size()) ? (R)(1.0 / (double)tpts) : (R)1;
-    std::vector<cpx<R>> lds(a.lds_elems);
+    std::vector<cpx<R>> lds(a.lds_elems + a.ntwl);
     const long nb = a.blocks();
     std::vector<char> seen(nb, 0);
     const bool first = i == 0, last = i + 1 == ps.size();
@@ -590,6 +640,7 @@ int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign
       if (lb >= (unsigned)nb || seen[lb]) throw Error("box remap is not a permutation");
       seen[lb] = 1;
       const BoxOrigin O = box_origin(a, lb);
+      box_twiddles_to_lds<R>(lds.data(), a, tw, 0, 1);
       box_load<R>(lds.data(), HostGridLoad<R>{in, S}, a, O, 0, 1, plan.tab_host.data());
       for (int j = 0; j < a.nsub; ++j) box_dispatch<R>(lds.data(), a, a.sub[j], tw, 0, 1);
       box_store<R>(lds.data(), HostGridStore<R>{out, S}, a, O, scale, tw, 0, 1, plan.tab_host.data());

@@ -29,6 +29,9 @@ constexpr int BOX_THREADS = 512;
 constexpr int BOX_MAX_SUB = 10;
 constexpr int BOX_MAX_RAD = 4;  // sub-passes per dimension
 constexpr int BOX_MAX_TILES = 32;  // tiles of BOX_THREADS elements per box (tiled load/store)
+// tiles a persistent work-group holds in registers while prefetching (one element per tile and thread)
+template <typename R>
+constexpr int BOX_TILE_REGS = sizeof(R) == 4 ? 32 : 16;
 
 // Division by a launch-invariant divisor as a multiply-high: with
 // m = ceil(2^32 / d), n / d = (n m) >> 32 exactly whenever n (d - 1) < 2^32
@@ -69,6 +72,8 @@ struct BoxSub {
   int wr_off;   // w_radix^m table
   int n;        // box extent along dim
   int ld;       // LDS stride along dim
+  int twl;      // LDS element offset of the dimension's w_n table (-1: use tw_off in global memory)
+  int tws;      // n / L: index step in that table
   // line decomposition: the three other dimensions of the box, x first
   int os[3];    // their LDS strides
   FDiv foe[2];  // extents of the first two of them
@@ -111,7 +116,11 @@ struct BoxArgs {
   int tiled, ntiles;
   int tlog[4];
   int tab_off;
-  long blocks() const { return (long)nblk[0] * nblk[1] * nblk[2] * nblk[3]; }
+  int pf_at;  // persistent tiled pass: the next box is fetched before sub-pass pf_at (nsub: after the last)
+  // twiddle tables copied to LDS after the box (lds_elems .. lds_elems + ntwl)
+  int ntwl, ntwc;
+  int twc_src[4], twc_dst[4], twc_len[4];  // copies: pool offset, LDS offset, length
+  __host__ __device__ long blocks() const { return (long)nblk[0] * nblk[1] * nblk[2] * nblk[3]; }
 };
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -313,9 +322,18 @@ __host__ __device__ __forceinline__ void box_subpass(cpx<R>* lds, const BoxArgs&
     for (int j = 0; j < RAD; ++j) v[j] = lds[base + (p0 + s * j) * ld];
     box_dft<R, RAD>(v, wr);
     if (s > 1) {
+      // w_L^(jp k) = w_n^(jp k n / L): the dimension's table, in LDS when
+      // the plan placed it there (sp.twl >= 0), else in global memory
+      const int step = jp * sp.tws;
       GMOD_BOX_CHECK(jp * (RAD - 1) < L, "sub-pass twiddle");
+      if (sp.twl >= 0) {
+        const cpx<R>* wt = lds + sp.twl;
 #pragma unroll
-      for (int k = 1; k < RAD; ++k) v[k] = v[k] * wl[jp * k];
+        for (int k = 1; k < RAD; ++k) v[k] = v[k] * wt[step * k];
+      } else {
+#pragma unroll
+        for (int k = 1; k < RAD; ++k) v[k] = v[k] * wl[jp * k];
+      }
     }
 #pragma unroll
     for (int k = 0; k < RAD; ++k) lds[base + (p0 + s * k) * ld] = v[k];
@@ -474,17 +492,111 @@ __host__ __device__ __forceinline__ void box_store(const cpx<R>* lds, const ST& 
   }
 }
 
+// the pass's twiddle tables into LDS (after the box)
+template <typename R>
+__host__ __device__ __forceinline__ void box_twiddles_to_lds(cpx<R>* lds, const BoxArgs& a,
+                                                             const cpx<R>* __restrict__ tw, int tid, int nthr) {
+  for (int c = 0; c < a.ntwc; ++c) {
+    const int src = c == 0 ? a.twc_src[0] : c == 1 ? a.twc_src[1] : c == 2 ? a.twc_src[2] : a.twc_src[3];
+    const int dst = c == 0 ? a.twc_dst[0] : c == 1 ? a.twc_dst[1] : c == 2 ? a.twc_dst[2] : a.twc_dst[3];
+    const int len = c == 0 ? a.twc_len[0] : c == 1 ? a.twc_len[1] : c == 2 ? a.twc_len[2] : a.twc_len[3];
+    for (int i = tid; i < len; i += nthr) {
+      GMOD_BOX_CHECK(dst + i < a.lds_elems + a.ntwl, "twiddle copy");
+      lds[dst + i] = tw[src + i];
+    }
+  }
+}
+
+// a box whose slots the store keeps none of (the gather keeps t < Nw; the
+// store slots of a box increase with the element index)
+template <class ST>
+__host__ __device__ __forceinline__ bool box_skipped(const ST& st, const BoxArgs& a, const BoxOrigin& O) {
+  const unsigned t_first = (unsigned)(O.bt * a.ts_mul), t_last = t_first + (unsigned)((a.box[3] - 1) * a.ts_str);
+  return st.skip(t_first) && st.skip(t_last);
+}
+
+// Tiled passes run persistent work-groups (one per CU: a box fills the LDS)
+// that loop over boxes b = blockIdx.x + k gridDim.x.  The next box is read
+// into registers (TM elements per thread) once the last sub-pass that reads
+// twiddles from global memory has run, so its HBM reads overlap the
+// remaining sub-passes and the store of the current box.  gridDim.x is a
+// multiple of 8, so a work-group keeps its XCD and box_remap's grouping holds.
+template <class ST>
+__device__ __forceinline__ unsigned box_next_live(const ST& st, const BoxArgs& a, unsigned b, unsigned nb) {
+  while (b < nb && box_skipped(st, a, box_origin(a, box_remap(b, nb, a.xcd_group)))) b += gridDim.x;
+  return b;
+}
+
+template <typename R, int TM, class LD>
+__device__ __forceinline__ void box_fetch(cpx<R> (&v)[TM], const LD& ld, const BoxArgs& a, unsigned b, unsigned nb,
+                                          unsigned csp, unsigned ct, const int* __restrict__ tsp,
+                                          const int* __restrict__ tt) {
+  const BoxOrigin O = box_origin(a, box_remap(b, nb, a.xcd_group));
+  const unsigned sp0 = box_sp(a, O, 0, 0, 0) + csp;
+  const unsigned t0 = mul24((unsigned)O.bt, (unsigned)a.tl_mul) + ct;
+#pragma unroll
+  for (int j = 0; j < TM; ++j)
+    if (j < a.ntiles) v[j] = ld(sp0 + (unsigned)tsp[j], t0 + (unsigned)tt[j]);
+}
+
+template <typename R, class LD, class ST>
+__device__ __forceinline__ void box_persistent(cpx<R>* lds, const LD& ld, const ST& st, const BoxArgs& a,
+                                               const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale,
+                                               int tid) {
+  constexpr int TM = BOX_TILE_REGS<R>;
+  const unsigned nb = (unsigned)a.blocks();
+  unsigned c[4];
+  box_tile_coord(a, (unsigned)tid, c);
+  const unsigned csp = c[0] + mul24(a.Nx, c[1]) + mul24(a.Nx * a.Ny, c[2]);
+  const unsigned ct = mul24(c[3], (unsigned)a.tl_str);
+  const unsigned l0 = c[0] + mul24(c[1], (unsigned)a.lds_stride[1]) + mul24(c[2], (unsigned)a.lds_stride[2]) +
+                      mul24(c[3], (unsigned)a.lds_stride[3]);
+  const int* tsp = tabs + a.tab_off;
+  const int* tt = tsp + a.ntiles;
+  const int* tld = tt + a.ntiles;
+  cpx<R> v[TM];
+  box_twiddles_to_lds<R>(lds, a, tw, tid, BOX_THREADS);  // ordered before use by the loop's first barrier
+  unsigned b = box_next_live(st, a, blockIdx.x, nb);
+  if (b < nb) box_fetch<R, TM>(v, ld, a, b, nb, csp, ct, tsp, tt);
+  while (b < nb) {
+#pragma unroll
+    for (int j = 0; j < TM; ++j)
+      if (j < a.ntiles) lds[l0 + (unsigned)tld[j]] = v[j];
+    __syncthreads();
+    const unsigned nxt = box_next_live(st, a, b + gridDim.x, nb);
+    for (int i = 0; i < a.nsub; ++i) {
+      if (i == a.pf_at && nxt < nb) box_fetch<R, TM>(v, ld, a, nxt, nb, csp, ct, tsp, tt);
+      box_dispatch<R>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS);
+      __syncthreads();
+    }
+    if (a.pf_at >= a.nsub && nxt < nb) box_fetch<R, TM>(v, ld, a, nxt, nb, csp, ct, tsp, tt);
+    box_store_tiled<R>(lds, st, a, box_origin(a, box_remap(b, nb, a.xcd_group)), scale, (unsigned)tid, tabs);
+    __syncthreads();  // the LDS box is rewritten by the next iteration
+    b = nxt;
+  }
+}
+
+// The pass descriptor is read from device memory (the plan's copy) rather
+// than passed by value: run-time indices into a by-value kernel argument
+// make the compiler copy the whole argument to scratch memory.
 template <typename R, class LD, class ST>
 __global__ void __launch_bounds__(BOX_THREADS)
-box_fft_kernel(LD ld, ST st, BoxArgs a, const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale) {
+box_fft_kernel(LD ld, ST st, const BoxArgs* __restrict__ ap, const cpx<R>* __restrict__ tw,
+               const int* __restrict__ tabs, R scale) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const BoxArgs& a = *ap;
+  if (a.tiled) {
+    box_persistent<R>(lds, ld, st, a, tw, tabs, scale, threadIdx.x);
+    return;
+  }
   const BoxOrigin O = box_origin(a, box_remap(blockIdx.x, gridDim.x, a.xcd_group));
   // a box whose slots the store keeps none of (the gather keeps t < Nw; the
   // store slots of a box increase with the element index)
   const unsigned t_first = (unsigned)(O.bt * a.ts_mul), t_last = t_first + (unsigned)((a.box[3] - 1) * a.ts_str);
   if (st.skip(t_first) && st.skip(t_last)) return;
   const int tid = threadIdx.x;
+  box_twiddles_to_lds<R>(lds, a, tw, tid, BOX_THREADS);
   box_load<R>(lds, ld, a, O, tid, BOX_THREADS, tabs);
   __syncthreads();
   for (int i = 0; i < a.nsub; ++i) {
