@@ -211,15 +211,17 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
     }
     const zd J = tail ? rb[l + Nchi * m] : zd(0, 0);
     zd* dst = chi + NkNw * (m + (long)Nchi * l);
-    p4->exec_fused(pl, GatherStore<R>{dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, md}, work.p, tmp.p, +1,
-                   invN, s);
+    const GatherStore<R> gst{dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, md};
+    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s);
+    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s);
     if (self_tail) {
       conj_copy(dst, dst, NkNw, s);
       ProductLoad<R> p0{};
       p0.m = md;
       p0.nterm = 1;
       p0.A[0] = grid(F0g, a13, b13); p0.B[0] = grid(F0g, a42, b42); p0.coef[0] = (R)w; p0.revt[0] = 1;
-      p4->exec_fused(p0, GatherStore<R>{dst, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1, invN, s);
+      p4->exec_fused(p0.template fixed<1>(), GatherStore<R>{dst, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1,
+                     invN, s);
       continue;
     }
     if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
@@ -444,7 +446,8 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
       IncProductLoad<R> p0{};
       p0.m = md;
       p0.add(grid(F0r, a13, b13), grid(F0d, a13, b13), grid(F0r, a42, b42), grid(F0d, a42, b42), (R)w, 1);
-      p4->exec_fused(p0, GatherStore<R>{slice.p, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1, invN, s);
+      p4->exec_fused(p0.template fixed<1>(), GatherStore<R>{slice.p, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p,
+                     +1, invN, s);
       axpy(dst, slice.p, zd(1, 0), NkNw, s);
       continue;
     }
@@ -453,13 +456,18 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
       // accumulation (fchi.f90:610): the stored value is conj(conv[G]), so the
       // increment is conjugated before it is added
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
-      p4->exec_fused(pl, GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p, +1, invN, s);
+      if (pl.nterm == 1)
+        p4->exec_fused(pl.template fixed<1>(), GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p,
+                       +1, invN, s);
+      else
+        p4->exec_fused(pl, GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p, +1, invN, s);
       conj_copy(slice.p, slice.p, NkNw, s);
       axpy(dst, slice.p, zd(1, 0), NkNw, s);
       continue;
     }
-    p4->exec_fused(pl, GatherStore<R>{dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, md}, work.p, tmp.p, +1,
-                   invN, s);
+    const GatherStore<R> gst{dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, md};
+    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s);
+    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s);
     if (pr[0] >= 0) conj_copy(chi_acc + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
 }

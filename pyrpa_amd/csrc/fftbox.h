@@ -19,6 +19,7 @@
 // explicit (thread, thread count) so that the host test hook runs the same
 // code, with bounds checks, on the CPU.
 #pragma once
+#include <type_traits>
 #include <vector>
 
 #include "gmod_common.h"
@@ -410,28 +411,27 @@ __host__ __device__ __forceinline__ void box_store_tiled(const cpx<R>* lds, cons
                       mul24((unsigned)box_outpos(a, 1, (int)c[1]), (unsigned)a.lds_stride[1]) +
                       mul24((unsigned)box_outpos(a, 2, (int)c[2]), (unsigned)a.lds_stride[2]) +
                       mul24((unsigned)box_outpos(a, 3, (int)c[3]), (unsigned)a.lds_stride[3]);
+  // the preparation (every load of the store functor) carries no branch: an
+  // element past the last tile repeats the last tile's element, which its
+  // batch commits twice with the same value; prep() is valid on slots that
+  // skip() drops, and those are not committed
+  const int last = a.ntiles - 1;
   for (int j0 = 0; j0 < a.ntiles; j0 += U) {
     typename ST::Pre pre[U];
     unsigned spv[U], tv[U], offv[U];
-    bool keep[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      keep[u] = false;
-      if (j0 + u < a.ntiles) {
-        spv[u] = sp0 + (unsigned)tsp[j0 + u];
-        tv[u] = t0 + (unsigned)tt[j0 + u];
-        offv[u] = l0 + (unsigned)tst[j0 + u];
-        if (!st.skip(tv[u])) {
-          GMOD_BOX_CHECK(spv[u] < a.S && tv[u] < a.M, "tiled store grid");
-          GMOD_BOX_CHECK(offv[u] < (unsigned)a.lds_elems, "tiled store lds");
-          keep[u] = true;
-          pre[u] = st.prep(spv[u], tv[u]);
-        }
-      }
+      const int jj = j0 + u < last ? j0 + u : last;
+      spv[u] = sp0 + (unsigned)tsp[jj];
+      tv[u] = t0 + (unsigned)tt[jj];
+      offv[u] = l0 + (unsigned)tst[jj];
+      GMOD_BOX_CHECK(spv[u] < a.S && tv[u] < a.M, "tiled store grid");
+      GMOD_BOX_CHECK(offv[u] < (unsigned)a.lds_elems, "tiled store lds");
+      pre[u] = st.prep(spv[u], tv[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (keep[u]) st.commit(spv[u], tv[u], pre[u], lds[offv[u]] * scale);
+      if (!st.skip(tv[u])) st.commit(spv[u], tv[u], pre[u], lds[offv[u]] * scale);
   }
 }
 
@@ -527,6 +527,42 @@ __device__ __forceinline__ unsigned box_next_live(const ST& st, const BoxArgs& a
   return b;
 }
 
+// Load functors that read an index before the value (an irreducible-k map,
+// the -r map) expose key(sp) / value(key, sp, t); the fetch then issues the
+// index loads of all its elements before any value load.  Others are called
+// as ld(sp, t).
+template <class LD, class = void>
+struct LoadKey {
+  struct Key {};
+  __device__ static Key key(const LD&, unsigned) { return Key{}; }
+  __device__ static auto value(const LD& ld, const Key&, unsigned sp, unsigned t) { return ld(sp, t); }
+};
+template <class LD>
+struct LoadKey<LD, std::void_t<typename LD::Key>> {
+  using Key = typename LD::Key;
+  __device__ static Key key(const LD& ld, unsigned sp) { return ld.key(sp); }
+  __device__ static auto value(const LD& ld, const Key& k, unsigned sp, unsigned t) { return ld.value(k, sp, t); }
+};
+
+// Load functors whose Raw type is the element type cpx<R> split a value into
+// raw() (loads only) and finish() (arithmetic on the loaded value and the
+// slot); the persistent pass keeps raw values in its prefetch registers and
+// finishes them as it writes them to LDS.  Others: raw = value, no finish.
+template <typename R, class LD, class = void>
+struct LoadRaw {
+  template <class K>
+  __device__ static cpx<R> raw(const LD& ld, const K& k, unsigned sp, unsigned t) {
+    return LoadKey<LD>::value(ld, k, sp, t);
+  }
+  __device__ static cpx<R> finish(const LD&, const cpx<R>& v, unsigned) { return v; }
+};
+template <typename R, class LD>
+struct LoadRaw<R, LD, std::enable_if_t<std::is_same<typename LD::Raw, cpx<R>>::value>> {
+  template <class K>
+  __device__ static cpx<R> raw(const LD& ld, const K& k, unsigned sp, unsigned t) { return ld.raw(k, sp, t); }
+  __device__ static cpx<R> finish(const LD& ld, const cpx<R>& v, unsigned t) { return ld.finish(v, t); }
+};
+
 template <typename R, int TM, class LD>
 __device__ __forceinline__ void box_fetch(cpx<R> (&v)[TM], const LD& ld, const BoxArgs& a, unsigned b, unsigned nb,
                                           unsigned csp, unsigned ct, const int* __restrict__ tsp,
@@ -534,9 +570,19 @@ __device__ __forceinline__ void box_fetch(cpx<R> (&v)[TM], const LD& ld, const B
   const BoxOrigin O = box_origin(a, box_remap(b, nb, a.xcd_group));
   const unsigned sp0 = box_sp(a, O, 0, 0, 0) + csp;
   const unsigned t0 = mul24((unsigned)O.bt, (unsigned)a.tl_mul) + ct;
+  using LK = LoadKey<LD>;
+  // no per-element branches: an element past the last tile re-reads the last
+  // tile's (valid) element and is never stored, so all loads of the fetch
+  // sit in one basic block and are issued back to back
+  const int last = a.ntiles - 1;
+  typename LK::Key kk[TM];
 #pragma unroll
-  for (int j = 0; j < TM; ++j)
-    if (j < a.ntiles) v[j] = ld(sp0 + (unsigned)tsp[j], t0 + (unsigned)tt[j]);
+  for (int j = 0; j < TM; ++j) kk[j] = LK::key(ld, sp0 + (unsigned)tsp[j < last ? j : last]);
+#pragma unroll
+  for (int j = 0; j < TM; ++j) {
+    const int jj = j < last ? j : last;
+    v[j] = LoadRaw<R, LD>::raw(ld, kk[j], sp0 + (unsigned)tsp[jj], t0 + (unsigned)tt[jj]);
+  }
 }
 
 template <typename R, class LD, class ST>
@@ -559,9 +605,13 @@ __device__ __forceinline__ void box_persistent(cpx<R>* lds, const LD& ld, const 
   unsigned b = box_next_live(st, a, blockIdx.x, nb);
   if (b < nb) box_fetch<R, TM>(v, ld, a, b, nb, csp, ct, tsp, tt);
   while (b < nb) {
+    {
+      const BoxOrigin O = box_origin(a, box_remap(b, nb, a.xcd_group));
+      const unsigned t0 = mul24((unsigned)O.bt, (unsigned)a.tl_mul) + ct;
 #pragma unroll
-    for (int j = 0; j < TM; ++j)
-      if (j < a.ntiles) lds[l0 + (unsigned)tld[j]] = v[j];
+      for (int j = 0; j < TM; ++j)
+        if (j < a.ntiles) lds[l0 + (unsigned)tld[j]] = LoadRaw<R, LD>::finish(ld, v[j], t0 + (unsigned)tt[j]);
+    }
     __syncthreads();
     const unsigned nxt = box_next_live(st, a, b + gridDim.x, nb);
     for (int i = 0; i < a.nsub; ++i) {

@@ -24,6 +24,7 @@ struct Mesh {
   DBuf<int> irr_s;    // per spatial index: irreducible k
   DBuf<int> flag_s;   // per spatial index: 0 direct / 1 reversed
   DBuf<int> kgat_s;   // per spatial index: irreducible k if this is its flag-0 point, else -1
+  DBuf<int> kf_s;     // per spatial index: 2 irr_s + (flag_s != 0), one load for both
   DBuf<double> kw;    // full-BZ multiplicity of irreducible k
   std::vector<double> kw_host;
   bool complete = true;  // every irreducible k has a direct (flag 0) full-BZ point
@@ -35,7 +36,7 @@ struct Mesh {
 // ---- fused load / store functors for FFTPlan::exec_fused
 // Address a = sp + Nkall * t on the full grid (sp: spatial index, t: slot).
 struct MeshDev {
-  const int *irr_s, *flag_s, *kgat_s, *rev;  // per spatial index
+  const int *irr_s, *flag_s, *kgat_s, *rev, *kf_s;  // per spatial index
   long Nkall, Nk;
   int Nw, M;
 };
@@ -49,19 +50,24 @@ struct GreenLoad {
   const zd* G;
   int a, b, No;
   MeshDev m;
-  __device__ cpx<R> operator()(unsigned sp, unsigned tu) const {
+  // key / value split: the box FFT issues the index loads of all its
+  // elements before their value loads (a dependent pair per element would
+  // otherwise cost two memory latencies each); value() is branch-free
+  using Key = int;  // packed 2 k + flag (Mesh::kf_s), decoded only in value()
+  __device__ Key key(unsigned sp) const { return m.kf_s[sp]; }
+  using Raw = zd;  // raw / finish as for BosonLoad
+  __device__ Raw raw(Key kf, unsigned, unsigned tu) const {
     const int t = (int)tu;
-    const long k = m.irr_s[sp];
-    const int f = m.flag_s[sp];
+    const bool lo = t < m.Nw;
+    const int j = lo ? t : m.M - 1 - t;
+    const bool d = ((kf & 1) == 0) == lo;
+    const int p = d ? a : b, r = d ? b : a;
     const long NkNw = m.Nk * (long)m.Nw;
-    if (t < m.Nw) {
-      const int p = f == 0 ? a : b, q = f == 0 ? b : a;
-      return cpx<R>(G[k + m.Nk * (long)t + NkNw * (p + (long)No * q)]);
-    }
-    const int j = m.M - 1 - t;
-    const int p = f == 0 ? b : a, q = f == 0 ? a : b;
-    return cpx<R>(conj(G[k + m.Nk * (long)j + NkNw * (p + (long)No * q)]));
+    return G[(kf >> 1) + m.Nk * (long)j + NkNw * (p + (long)No * r)];
   }
+  __device__ cpx<R> finish(const Raw& v, unsigned tu) const { return cpx<R>((int)tu < m.Nw ? v : conj(v)); }
+  __device__ cpx<R> value(Key kf, unsigned sp, unsigned t) const { return finish(raw(kf, sp, t), t); }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
 };
 
 // bosonic V_(m,l) (fself.f90:203-224, feliash.f90:543-564): slot 0 = nu_0,
@@ -72,15 +78,26 @@ struct BosonLoad {
   long off_ml, off_lm;
   zd stat;
   MeshDev m;
-  __device__ cpx<R> operator()(unsigned sp, unsigned tu) const {
+  using Key = int;  // packed 2 k + flag (Mesh::kf_s), decoded only in value()
+  __device__ Key key(unsigned sp) const { return m.kf_s[sp]; }
+  // raw (the load alone) / finish (conjugation, the nu -> inf slot): the box
+  // FFT issues the raw loads of all its elements back to back and finishes
+  // them when it writes them to LDS
+  using Raw = cpx<S>;
+  __device__ Raw raw(Key kf, unsigned, unsigned tu) const {
     const int t = (int)tu;
-    const long k = m.irr_s[sp];
-    const int f = m.flag_s[sp];
-    if (t == m.Nw) return cpx<R>(stat);
-    if (t < m.Nw) return cpx<R>(V[k + m.Nk * (long)t + (f == 0 ? off_ml : off_lm)]);
-    const int j = m.M - t;
-    return cpx<R>(conj(V[k + m.Nk * (long)j + (f == 0 ? off_lm : off_ml)]));
+    const bool hi = t > m.Nw;
+    const int j = hi ? m.M - t : (t < m.Nw ? t : m.Nw - 1);  // slot Nw reads a valid element it then drops
+    const long off = (((kf & 1) == 0) != hi) ? off_ml : off_lm;
+    return V[(kf >> 1) + m.Nk * (long)j + off];
   }
+  __device__ cpx<R> finish(const Raw& v, unsigned tu) const {
+    const int t = (int)tu;
+    const cpx<R> r(t > m.Nw ? conj(v) : v);
+    return t == m.Nw ? cpx<R>(stat) : r;
+  }
+  __device__ cpx<R> value(Key kf, unsigned sp, unsigned t) const { return finish(raw(kf, sp, t), t); }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
 };
 
 // anomalous F_(a,b) with parity factors (feliash.f90:566-577)
@@ -140,7 +157,10 @@ struct TauRefLoad {
 // sum of up to three products coef * A(r,t) * B(-r, -t)  (revt) or B(-r, t)
 // (fchi.f90:593, 824)
 constexpr int PROD_MAXT = 9;
-template <typename R>
+// NT > 0: exactly NT terms, known at compile time (the loads of an element
+// then carry no loop, and the box FFT issues them for several elements at
+// once); NT = 0: nterm terms.
+template <typename R, int NT = 0>
 struct ProductLoad {
   const cpx<R>* A[PROD_MAXT];
   const cpx<R>* B[PROD_MAXT];
@@ -152,32 +172,53 @@ struct ProductLoad {
     A[nterm] = a; B[nterm] = b; coef[nterm] = c; revt[nterm] = rt; ++nterm;
   }
   MeshDev m;
-  __device__ cpx<R> operator()(unsigned sp, unsigned t) const {
+  template <int N>
+  __host__ ProductLoad<R, N> fixed() const {
+    ProductLoad<R, N> p{};
+    for (int i = 0; i < PROD_MAXT; ++i) { p.A[i] = A[i]; p.B[i] = B[i]; p.coef[i] = coef[i]; p.revt[i] = revt[i]; }
+    p.nterm = nterm;
+    p.m = m;
+    return p;
+  }
+  using Key = unsigned;  // spatial index of -r
+  __device__ Key key(unsigned sp) const { return (unsigned)m.rev[sp]; }
+  __device__ cpx<R> value(Key rs, unsigned sp, unsigned t) const {
     const size_t ad = sp + (size_t)m.Nkall * t;
-    const unsigned rs = (unsigned)m.rev[sp];
     const unsigned rtt = t == 0 ? 0u : (unsigned)m.M - t;
     cpx<R> v(0, 0);
-    for (int i = 0; i < nterm; ++i) {
+#pragma unroll
+    for (int i = 0; i < (NT ? NT : nterm); ++i) {
       const unsigned rt = revt[i] ? rtt : t;
       v += A[i][ad] * B[i][rs + (size_t)m.Nkall * rt] * coef[i];
     }
     return v;
   }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
 };
 
 // G_ab(current) - G_ab(reference) on the full grid, formed in FP64 before
 // the conversion to the grid precision (increments of the bilinear bubble)
 template <typename R>
 struct GreenDiffLoad {
-  GreenLoad<double> cur, ref;
-  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return cpx<R>(cur(sp, t) - ref(sp, t)); }
+  GreenLoad<double> cur, ref;  // on one mesh: one key serves both
+  using Key = typename GreenLoad<double>::Key;
+  __device__ Key key(unsigned sp) const { return cur.key(sp); }
+  __device__ cpx<R> value(Key q, unsigned sp, unsigned t) const {
+    return cpx<R>(cur.value(q, sp, t) - ref.value(q, sp, t));
+  }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
 };
 
 // V(current) - V(reference): the static nu->inf slot cancels
 template <typename R>
 struct BosonDiffLoad {
   BosonLoad<double> cur, ref;
-  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return cpx<R>(cur(sp, t) - ref(sp, t)); }
+  using Key = typename BosonLoad<double>::Key;
+  __device__ Key key(unsigned sp) const { return cur.key(sp); }
+  __device__ cpx<R> value(Key q, unsigned sp, unsigned t) const {
+    return cpx<R>(cur.value(q, sp, t) - ref.value(q, sp, t));
+  }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
 };
 
 template <typename R>
@@ -191,7 +232,7 @@ struct TauRefDiffLoad {
 //   v = sum_i coef_i [ Ad13 (Br42 + Bd42) + Ar13 Bd42 ]
 // (the products of fchi.f90:593, 824 expanded to first + second order in Xd)
 constexpr int INC_MAXT = 3;
-template <typename R>
+template <typename R, int NT = 0>  // NT as for ProductLoad
 struct IncProductLoad {
   const cpx<R>* Ar[INC_MAXT];
   const cpx<R>* Ad[INC_MAXT];
@@ -201,16 +242,29 @@ struct IncProductLoad {
   int revt[INC_MAXT];
   int nterm;
   MeshDev m;
+  template <int N>
+  __host__ IncProductLoad<R, N> fixed() const {
+    IncProductLoad<R, N> p{};
+    for (int i = 0; i < INC_MAXT; ++i) {
+      p.Ar[i] = Ar[i]; p.Ad[i] = Ad[i]; p.Br[i] = Br[i]; p.Bd[i] = Bd[i]; p.coef[i] = coef[i]; p.revt[i] = revt[i];
+    }
+    p.nterm = nterm;
+    p.m = m;
+    return p;
+  }
   __host__ void add(const cpx<R>* ar, const cpx<R>* ad, const cpx<R>* br, const cpx<R>* bd, R c, int rt) {
     if (nterm >= INC_MAXT) throw Error("too many increment terms");
     Ar[nterm] = ar; Ad[nterm] = ad; Br[nterm] = br; Bd[nterm] = bd; coef[nterm] = c; revt[nterm] = rt; ++nterm;
   }
-  __device__ cpx<R> operator()(unsigned sp, unsigned t) const {
+  using Key = unsigned;  // spatial index of -r
+  __device__ Key key(unsigned sp) const { return (unsigned)m.rev[sp]; }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+  __device__ cpx<R> value(Key rs, unsigned sp, unsigned t) const {
     const size_t ad = sp + (size_t)m.Nkall * t;
-    const unsigned rs = (unsigned)m.rev[sp];
     const unsigned rtt = t == 0 ? 0u : (unsigned)m.M - t;
     cpx<R> v(0, 0);
-    for (int i = 0; i < nterm; ++i) {
+#pragma unroll
+    for (int i = 0; i < (NT ? NT : nterm); ++i) {
       const size_t bd = rs + (size_t)m.Nkall * (revt[i] ? rtt : t);
       const cpx<R> bdv = Bd[i][bd];
       v += (Ad[i][ad] * (Br[i][bd] + bdv) + Ar[i][ad] * bdv) * coef[i];
@@ -237,7 +291,10 @@ struct SigmaIncStore {
   __device__ Pre prep(unsigned sp, unsigned t) const {
     const size_t ad = sp + (size_t)S * t;
     Pre q;
-    q.acc = first ? cpx<R>(0, 0) : acc[ad];
+    // no branch around the load: on the first pass every element reads
+    // element 0 (one cached line, no HBM stream) and drops it
+    const cpx<R> a0 = acc[first ? 0 : ad];
+    q.acc = first ? cpx<R>(0, 0) : a0;
     q.fr = Fr[ad];
     q.fd = Fd[ad];
     q.fv = FVr[ad];
@@ -263,7 +320,10 @@ struct MulAccStore {
   __device__ Pre prep(unsigned sp, unsigned t) const {
     const size_t ad = sp + (size_t)S * t;
     Pre q;
-    q.acc = first ? cpx<R>(0, 0) : acc[ad];
+    // no branch around the load: on the first pass every element reads
+    // element 0 (one cached line, no HBM stream) and drops it
+    const cpx<R> a0 = acc[first ? 0 : ad];
+    q.acc = first ? cpx<R>(0, 0) : a0;
     q.f = F[ad];
     return q;
   }
@@ -286,16 +346,20 @@ struct GatherStore {
   MeshDev m;
   const zd* base = nullptr;  // optional: dst = base + w v (FP64 reference + increment)
   struct Pre { long o; zd old, bv, js; };  // o < 0: slot or point not kept
+  // branch-free: a dropped slot or point is not stored, and an operand the
+  // store does not use is read at element 0 (one cached line) and dropped
   __device__ Pre prep(unsigned sp, unsigned t) const {
     Pre q;
-    q.o = -1;
-    if ((int)t >= m.Nw) return q;
     const long k = m.kgat_s[sp];
-    if (k < 0) return q;
-    q.o = k + m.Nk * (long)t;
-    if (acc) q.old = dst[q.o];
-    if (base) q.bv = base[q.o];
-    if (sawc) q.js = J * sawc[t];
+    const int tt = (int)t < m.Nw ? (int)t : m.Nw - 1;
+    const long o = (k < 0 ? 0 : k) + m.Nk * (long)tt;
+    q.o = ((int)t < m.Nw && k >= 0) ? o : -1;
+    const zd old = dst[acc ? o : 0];
+    const zd bv = (base ? base : dst)[base ? o : 0];
+    const zd sw = (sawc ? sawc : dst)[sawc ? tt : 0];
+    q.old = acc ? old : zd(0, 0);
+    q.bv = base ? bv : zd(0, 0);
+    q.js = sawc ? J * sw : zd(0, 0);
     return q;
   }
   __device__ void commit(unsigned, unsigned, const Pre& q, cpx<R> v) const {

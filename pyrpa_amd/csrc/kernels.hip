@@ -56,9 +56,12 @@ void Mesh::build(const int64_t* kmap, const int64_t* invk, long Nkall_, long Nk_
     else complete = false;
   }
   irr.alloc(Nkall); flag.alloc(Nkall); sp.alloc(Nkall); rev.alloc(Nkall); irr_sp.alloc(Nk); kw.alloc(Nk);
-  irr_s.alloc(Nkall); flag_s.alloc(Nkall); kgat_s.alloc(Nkall);
+  irr_s.alloc(Nkall); flag_s.alloc(Nkall); kgat_s.alloc(Nkall); kf_s.alloc(Nkall);
   irr_s.upload(hirr_s.data(), Nkall, s);
   flag_s.upload(hflag_s.data(), Nkall, s);
+  std::vector<int> hkf_s(Nkall);
+  for (long i = 0; i < Nkall; ++i) hkf_s[i] = (hirr_s[i] << 1) | (hflag_s[i] != 0 ? 1 : 0);
+  kf_s.upload(hkf_s.data(), Nkall, s);
   kgat_s.upload(hkgat_s.data(), Nkall, s);
   irr.upload(hirr.data(), Nkall, s);
   flag.upload(hflag.data(), Nkall, s);
@@ -70,7 +73,7 @@ void Mesh::build(const int64_t* kmap, const int64_t* invk, long Nkall_, long Nk_
 }
 
 MeshDev mesh_dev(const Mesh& m) {
-  return {m.irr_s.p, m.flag_s.p, m.kgat_s.p, m.rev.p, m.Nkall, m.Nk, m.Nw, m.M};
+  return {m.irr_s.p, m.flag_s.p, m.kgat_s.p, m.rev.p, m.kf_s.p, m.Nkall, m.Nk, m.Nw, m.M};
 }
 
 template <typename R>
