@@ -558,8 +558,12 @@ k_vertex_gj(zd* __restrict__ chi, long batch, long es, const VSparse* __restrict
   __shared__ VSparse sp;
   __shared__ double2 Abuf_[GJ_MAT * NN];  // chi0, later X_s then X_c
   __shared__ double2 Vbuf_[GJ_MAT * NN];  // A W (mode 0), then the vertex
-  __shared__ double2 Prow_[GJ_WAVES * 4 * 2 * N];
-  __shared__ double Pval_[GJ_WAVES * 4 * N];
+  // per-group slots padded so that the four groups of a wavefront start 4
+  // dwords apart in the LDS banks (their broadcast reads do not conflict)
+  constexpr int PROW = 2 * N + 1;  // complex entries per group slot
+  constexpr int PVAL = 2 * N + 2;  // doubles per group slot (68 dwords = 4 mod 64)
+  __shared__ double2 Prow_[GJ_WAVES * 4 * PROW];
+  __shared__ double Pval_[GJ_WAVES * 4 * PVAL];
   zd* Abuf = reinterpret_cast<zd*>(Abuf_);
   zd* Vbuf = reinterpret_cast<zd*>(Vbuf_);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -572,8 +576,8 @@ k_vertex_gj(zd* __restrict__ chi, long batch, long es, const VSparse* __restrict
   const int mloc = wave * 2 + (g >> 1);  // matrix of my group within the block
   const int sys = g & 1;                 // 0: S system, 1: C system
   zd* A = Abuf + mloc * NN;
-  zd* prow = reinterpret_cast<zd*>(Prow_) + (wave * 4 + g) * 2 * N;
-  double* pval = Pval_ + (wave * 4 + g) * N;
+  zd* prow = reinterpret_cast<zd*>(Prow_) + (wave * 4 + g) * PROW;
+  double* pval = Pval_ + (wave * 4 + g) * PVAL;
   const double a1 = mode == 2 ? -0.5 : 1.5;  // coefficient of S X_s
   const double a2 = mode == 0 ? 0.5 : -0.5;  // coefficient of C X_c
   __syncthreads();
