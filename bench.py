@@ -19,7 +19,11 @@ A step is one timed pair of solves.  `value` is the speedup
 t_plain / t_mixed of the device-resident solve times (inputs in HBM, the
 copies of the host ABI excluded; the PCIe-inclusive wall times are reported
 beside them).  The dominant kernel's roofline comes from hipEvents around
-every FFT pass on the library stream during the timed mixed solve.
+every FFT pass on the library stream during the timed mixed solve; a pass's
+algorithmic bytes are what its fused load and store must move: a full grid
+read or written counts N b (N grid elements, b bytes each), an irreducible
+(k, w) slice counts its own size, so a plain pass is 2 N b and each fused
+grid operand adds N b.
 
 The CPU baseline is the numpy restatement of the reference (oracle/) timed
 on a bounded sample -- one orbital-pair FFT convolution of the reference
@@ -225,7 +229,9 @@ def main():
                      "traffic": FP32_BOX_TRAFFIC_BYTES.get(args.config) if cls == "fft_pass_fp32" else None,
                      "kernel": f"box_fft_kernel<{'float' if cls == 'fft_pass_fp32' else 'double'}>, all "
                                f"instantiations", "avg_launch_us": round(1e3 * avg_ms, 2),
-                     "launches": int(pc["launches"]), "algorithmic_bytes_per_launch": bytes_per},
+                     "launches": int(pc["launches"]), "algorithmic_bytes_per_launch": bytes_per,
+                     "bytes_model": "per pass: N b per full grid read or written (fused operands included), "
+                                    "irreducible (k,w) slices at their size; average over the launches"},
         "roofline_fp64_fft_pass_gbs": round(ach64, 1),
         "cpu_baseline": cpu,
     }

@@ -51,7 +51,13 @@ class ConvEngine {
                       const std::vector<zd>& dJ, const zd* sawc_d, hipStream_t s);
   // sig_acc += Sigma[V, G] - Sigma[Vprev, Gprev] (unthresholded); uses Fr/Fd of the
   // preceding chi0_increment and advances FVr to V.
-  void sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, double temp, hipStream_t s);
+  // dv_ready: the FP32 vertex increment is already in dv_buffer() (written by
+  // the fused vertex kernel); else it is formed here from V and Vprev
+  void sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, double temp, hipStream_t s, bool dv_ready = false);
+  cpx<float>* dv_buffer(long n) {
+    if (dvbuf.n != (size_t)n) dvbuf.alloc(n);
+    return dvbuf.p;
+  }
   // Fr += Fd (and the tail grids): the current iterate becomes the reference
   void advance(bool tail, hipStream_t s);
   // Eliashberg: cache FFT[V_Delta] of the qualifying pairs (feliash.f90:540)

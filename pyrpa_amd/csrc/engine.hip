@@ -473,7 +473,8 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
 }
 
 template <typename R>
-void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, double temp, hipStream_t s) {
+void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, double temp, hipStream_t s,
+                                    bool dv_ready) {
   const int No = op.No, Nchi = op.Nchi;
   const long NkNw = mesh.Nk * (long)mesh.Nw;
   if (acc.n != (size_t)N) acc.alloc(N);
@@ -485,8 +486,10 @@ void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, d
   // then read (from a quarter of the bytes) by the four grid points that
   // unfold it, instead of two FP64 reads per grid point
   const long nC = NkNw * Nchi * Nchi;
-  if (dvbuf.n != (size_t)nC) dvbuf.alloc(nC);
-  diff_to_f32(dvbuf.p, V, Vprev, nC, s);
+  if (!dv_ready || dvbuf.n != (size_t)nC) {
+    if (dvbuf.n != (size_t)nC) dvbuf.alloc(nC);
+    diff_to_f32(dvbuf.p, V, Vprev, nC, s);
+  }
   for (int l1 = 0; l1 < No; ++l1)
     for (int m1 = 0; m1 < No; ++m1) {
       bool first = true;

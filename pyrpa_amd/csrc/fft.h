@@ -60,6 +60,7 @@ template <typename R> struct GridLoad {
   unsigned S;
   // S, t < 2^24 and S t < 2^31 (checked by the plan): a 24-bit multiply
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return p[sp + __umul24(S, t)]; }
+  __host__ double algo_bytes(double N) const { return N * sizeof(cpx<R>); }
 };
 template <typename R> struct GridStore {
   cpx<R>* p;
@@ -68,6 +69,7 @@ template <typename R> struct GridStore {
   __device__ Pre prep(unsigned, unsigned) const { return Pre{}; }
   __device__ void commit(unsigned sp, unsigned t, const Pre&, cpx<R> v) const { p[sp + __umul24(S, t)] = v; }
   __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { p[sp + __umul24(S, t)] = v; }
+  __host__ double algo_bytes(double N) const { return N * sizeof(cpx<R>); }
   __device__ bool skip(unsigned) const { return false; }
 };
 
@@ -243,6 +245,23 @@ class FFTPlan {
   long persist_ = 256;                    // work-groups of a persistent tiled pass
 };
 
+// Algorithmic bytes of one pass: what its load and store functors must move
+// (algo_bytes; a functor without it counts as one grid read or write).
+template <class F, class = void>
+struct AlgoBytes {
+  template <typename R>
+  static double get(const F&, double N) { return N * sizeof(cpx<R>); }
+};
+template <class F>
+struct AlgoBytes<F, std::void_t<decltype(std::declval<const F&>().algo_bytes(1.0))>> {
+  template <typename R>
+  static double get(const F& f, double N) { return f.algo_bytes(N); }
+};
+template <typename R, class LD, class ST>
+double pass_bytes(const LD& ld, const ST& st, double N) {
+  return AlgoBytes<LD>::template get<R>(ld, N) + AlgoBytes<ST>::template get<R>(st, N);
+}
+
 inline bool fft_is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
 inline int fft_ilog2(long n) { int l = 0; while ((1L << l) < n) ++l; return l; }
 
@@ -264,7 +283,7 @@ void FFTPlan<R>::launch(const FFTPassDesc& p, LD ld, ST st, const cpx<R>* tw, R 
   a.loglines = fft_is_pow2(p.lines) ? fft_ilog2(p.lines) : -1;
   const long blocks = (a.total_lines + p.lines - 1) / p.lines;
   const size_t shm = (size_t)p.lines * a.pitch * sizeof(cpx<R>);
-  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, 2.0 * (double)size() * sizeof(cpx<R>), s);
+  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, pass_bytes<R>(ld, st, (double)size()), s);
   if (fft_is_pow2(p.n))
     hipLaunchKernelGGL((fft_pass_kernel<R, true, LD, ST>), dim3((unsigned)blocks), dim3(FFT_THREADS), shm, s, ld, st,
                        a, tw + p.tw_off, tw + p.tw4_off, sc);
@@ -285,7 +304,7 @@ void FFTPlan<R>::launch_box(int sg, int i, LD ld, ST st, const cpx<R>* tw, R sc,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)box_.lds_limit));
     attr_done = true;
   }
-  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, 2.0 * (double)size() * sizeof(cpx<R>), s);
+  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, pass_bytes<R>(ld, st, (double)size()), s);
   // tiled passes: persistent work-groups, one per CU, a multiple of 8
   long grid = a.blocks();
   if (a.tiled && grid > persist_) grid = persist_;

@@ -46,12 +46,21 @@ struct Upload {
   }
 };
 
-// Stoner factor: max_q max Re eig(chi0(q,0) S) (fchi.f90:352-399)
-static double stoner_factor(const zd* chi, long Nk, long Nw, int Nchi, const double* S_d, hipStream_t s) {
+// Stoner factor: max_q max Re eig(chi0(q,0) S) (fchi.f90:352-399); thr > 0:
+// of the thresholded chi0 (the first frequency slice is copied with it)
+static double stoner_factor(const zd* chi, long Nk, long Nw, int Nchi, const double* S_d, hipStream_t s,
+                            double thr = 0.0) {
   DBuf<double> out(Nk);
   DBuf<int> err(1);
   err.zero(s);
-  max_re_eig(out.p, chi, BatchLayout{1, Nk * Nw}, Nk, Nchi, S_d, 1.0, err.p, s);
+  DBuf<zd> slice;
+  if (thr > 0.0) {
+    slice.alloc((size_t)Nk * Nchi * Nchi);
+    threshold_slice(slice.p, chi, Nk, Nk * Nw, (long)Nchi * Nchi, thr, s);
+    max_re_eig(out.p, slice.p, BatchLayout{1, Nk}, Nk, Nchi, S_d, 1.0, err.p, s);
+  } else {
+    max_re_eig(out.p, chi, BatchLayout{1, Nk * Nw}, Nk, Nchi, S_d, 1.0, err.p, s);
+  }
   check_err(err, s, "ckchi");
   std::vector<double> h(Nk);
   out.download(h.data(), Nk, s);
@@ -323,10 +332,14 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
     if (verbose) printf("iter=%5ld\n", it);
     auto t0 = clk::now();
     const bool inc = mixed && e32 != nullptr;
+    // chi0 leaves the accumulation untresholded (chi_acc in the mixed loop,
+    // chi in place in the FP64 loop); the reference's 1e-9 threshold
+    // (fself.f90:375) is applied where chi0 is read: by the Stoner slice and
+    // by the vertex kernel
+    const zd* chi_src = mixed ? chi_acc.p : chi.p;
     if (!inc) {
       eng->transform_green(G.p, false, s);
-      eng->chi0(mixed ? chi_acc.p : chi.p, T, A.sw_tail, eig_d.p, uni_d.p, A.eig, A.uni, A.invk, mu, s, !mixed);
-      if (mixed) threshold_copy(chi.p, chi_acc.p, nC, 1e-9, s);
+      eng->chi0(mixed ? chi_acc.p : chi.p, T, A.sw_tail, eig_d.p, uni_d.p, A.eig, A.uni, A.invk, mu, s, false);
     } else {
       if (A.sw_tail) {
         green0(G0.p, eig_d.p, uni_d.p, mu, T, Nk, (int)Nw, No, s);
@@ -336,20 +349,28 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
       }
       e32->chi0_increment(chi_acc.p, G.p, Gprev.p, G0.p, G0prev.p, T, A.sw_tail, eig_d.p, uni_d.p, mu, mu_prev, dJ,
                           sawc_d.p, s);
-      threshold_copy(chi.p, chi_acc.p, nC, 1e-9, s);
     }
-    stoner = stoner_factor(chi.p, Nk, Nw, Nchi, S_d.p, s);
+    stoner = stoner_factor(chi_src, Nk, Nw, Nchi, S_d.p, s, 1e-9);
     t_chi += secs(t0);
     if (verbose) printf("SDW max eig(chi0 S) = %12.8f\n", stoner);
-    if (A.sw_rescale && stoner >= 1.0) {
+    const bool rescale = A.sw_rescale && stoner >= 1.0;
+    if (rescale) {
       const double rescale_t = 1.0 - 1.0e-4;  // fself.f90:371-380
       if (verbose) printf("[FLEX] Stoner factor=%10.6f >= 1: rescaling chi0 to%7.4f\n", stoner, rescale_t);
+      threshold_copy(chi.p, chi_src, nC, 1e-9, s);
       scale(chi.p, nC, rescale_t / stoner, s);
     }
     t0 = clk::now();
+    // the FP32 increment of the vertex comes out of the vertex kernel in the
+    // increment iterations (unless chi0 was rescaled)
+    const bool dv_fused = inc && !rescale;
     {
       ProfScope ps(PROF_VERTEX, (double)nC * 2 * sizeof(zd), s);
-      rpa_vertex(chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 0, err.p, s);
+      if (rescale)
+        rpa_vertex(chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 0, err.p, s);
+      else
+        rpa_vertex_fused(chi_src, chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 1e-9,
+                         dv_fused ? Vprev.p : nullptr, dv_fused ? e32->dv_buffer(nC) : nullptr, err.p, s);
     }
     check_err(err, s, "get_vsigma_flex_nosoc");
     t_vert += secs(t0);
@@ -376,7 +397,7 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
         }
       }
     } else {
-      e32->sigma_increment(sig_acc.p, chi.p, Vprev.p, T, s);
+      e32->sigma_increment(sig_acc.p, chi.p, Vprev.p, T, s, dv_fused);
       e32->advance(A.sw_tail, s);
       threshold_copy(sig.p, sig_acc.p, nS, 1e-9, s);
       std::swap(Vprev, chi);

@@ -39,7 +39,12 @@ struct MeshDev {
   const int *irr_s, *flag_s, *kgat_s, *rev, *kf_s;  // per spatial index
   long Nkall, Nk;
   int Nw, M;
+  __host__ double irr_elems() const { return (double)Nk * Nw; }  // elements of one irreducible (k, w) slice
 };
+
+// Algorithmic bytes of a fused functor per pass (the roofline accounting):
+// algo_bytes(N) with N = elements of the full grid.  A grid read or write
+// counts N b; a read of irreducible slices counts their Nk Nw elements.
 MeshDev mesh_dev(const Mesh& m);
 
 // G_ab on the full grid (fchi.f90:563-583, fself.f90:226-236):
@@ -68,6 +73,7 @@ struct GreenLoad {
   __device__ cpx<R> finish(const Raw& v, unsigned tu) const { return cpx<R>((int)tu < m.Nw ? v : conj(v)); }
   __device__ cpx<R> value(Key kf, unsigned sp, unsigned t) const { return finish(raw(kf, sp, t), t); }
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+  __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(zd); }
 };
 
 // bosonic V_(m,l) (fself.f90:203-224, feliash.f90:543-564): slot 0 = nu_0,
@@ -98,6 +104,7 @@ struct BosonLoad {
   }
   __device__ cpx<R> value(Key kf, unsigned sp, unsigned t) const { return finish(raw(kf, sp, t), t); }
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+  __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(cpx<S>); }  // V_ml, V_lm slices
 };
 
 // anomalous F_(a,b) with parity factors (feliash.f90:566-577)
@@ -194,6 +201,7 @@ struct ProductLoad {
     return v;
   }
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+  __host__ double algo_bytes(double N) const { return 2.0 * nterm * N * sizeof(cpx<R>); }  // A and B per term
 };
 
 // G_ab(current) - G_ab(reference) on the full grid, formed in FP64 before
@@ -207,6 +215,7 @@ struct GreenDiffLoad {
     return cpx<R>(cur.value(q, sp, t) - ref.value(q, sp, t));
   }
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+  __host__ double algo_bytes(double N) const { return cur.algo_bytes(N) + ref.algo_bytes(N); }
 };
 
 // V(current) - V(reference): the static nu->inf slot cancels
@@ -219,6 +228,7 @@ struct BosonDiffLoad {
     return cpx<R>(cur.value(q, sp, t) - ref.value(q, sp, t));
   }
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+  __host__ double algo_bytes(double N) const { return cur.algo_bytes(N) + ref.algo_bytes(N); }
 };
 
 template <typename R>
@@ -271,6 +281,7 @@ struct IncProductLoad {
     }
     return v;
   }
+  __host__ double algo_bytes(double N) const { return 4.0 * nterm * N * sizeof(cpx<R>); }  // Ar, Ad, Br, Bd per term
 };
 
 // Sigma increment per vertex pair: acc (+)= dV~ (Fr + Fd) + FVr Fd, where dV~
@@ -306,6 +317,9 @@ struct SigmaIncStore {
     FVr[ad] = q.fv + v;
   }
   __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
+  __host__ double algo_bytes(double N) const {  // acc (read unless first), Fr, Fd, FVr read; acc, FVr written
+    return ((first ? 3.0 : 4.0) + 2.0) * N * sizeof(cpx<R>);
+  }
   __device__ bool skip(unsigned) const { return false; }
 };
 
@@ -331,6 +345,9 @@ struct MulAccStore {
     acc[sp + (size_t)S * t] = q.acc + v * q.f;
   }
   __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
+  __host__ double algo_bytes(double N) const {  // acc (read unless first) and F read, acc written
+    return ((first ? 1.0 : 2.0) + 1.0) * N * sizeof(cpx<R>);
+  }
   __device__ bool skip(unsigned) const { return false; }
 };
 
@@ -370,6 +387,9 @@ struct GatherStore {
     dst[q.o] = acc ? q.old + x : x;
   }
   __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
+  __host__ double algo_bytes(double) const {  // the kept (k, w) slice written, read too when accumulating / rebasing
+    return (1.0 + (acc ? 1.0 : 0.0) + (base ? 1.0 : 0.0)) * m.irr_elems() * sizeof(zd);
+  }
   __device__ bool skip(unsigned t) const { return (int)t >= m.Nw; }
 };
 
@@ -380,6 +400,9 @@ void mul_acc(cpx<R>* acc, const cpx<R>* A, const cpx<R>* B, bool first, long n, 
 // ---- elementwise
 void conj_copy(zd* dst, const zd* src, long n, hipStream_t s);
 void threshold(zd* a, long n, double eps, hipStream_t s);
+// dst(k, e) = src(k, w = 0, e) with threshold eps, e < nmat (the first
+// frequency slice of a (Nk, Nw, nmat) array, as a (Nk, nmat) array)
+void threshold_slice(zd* dst, const zd* src, long Nk, long NkNw, long nmat, double eps, hipStream_t s);
 void scale(zd* a, long n, double f, hipStream_t s);
 void axpy(zd* y, const zd* x, zd alpha, long n, hipStream_t s);      // y += alpha x
 void copy(zd* dst, const zd* src, long n, hipStream_t s);

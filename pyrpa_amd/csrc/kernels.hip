@@ -150,6 +150,22 @@ void threshold_copy(zd* dst, const zd* src, long n, double eps, hipStream_t s) {
   GMOD_CHECK_LAUNCH();
 }
 
+__global__ void k_threshold_slice(zd* dst, const zd* src, long Nk, long NkNw, long nmat, double eps) {
+  const long n = Nk * nmat;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+    const long k = g % Nk, e = g / Nk;
+    zd v = src[k + NkNw * e];
+    if (fabs(v.re) < eps) v.re = 0.0;
+    if (fabs(v.im) < eps) v.im = 0.0;
+    dst[g] = v;
+  }
+}
+void threshold_slice(zd* dst, const zd* src, long Nk, long NkNw, long nmat, double eps, hipStream_t s) {
+  const long n = Nk * nmat;
+  hipLaunchKernelGGL(k_threshold_slice, dim3(nblocks(n)), dim3(TPB), 0, s, dst, src, Nk, NkNw, nmat, eps);
+  GMOD_CHECK_LAUNCH();
+}
+
 __global__ void k_diff_to_f32(cpx<float>* dst, const zd* a, const zd* b, long n) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
     dst[e] = cpx<float>(a[e] - b[e]);

@@ -17,6 +17,13 @@ struct BatchLayout {
 //  X_s = (I - chi0 S)^-1 chi0 S,  X_c = (I + chi0 C)^-1 chi0 C  (LU with partial pivoting)
 void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, const double* C, int mode,
                 int* err, hipStream_t s);
+// mode 0 out of place on a matrix-major batch (L.bs = 1): reads chi_in with
+// the 1e-9 threshold of the FLEX loop applied as it is read (thr > 0),
+// writes V_sigma to chi_out (may equal chi_in) and, when vprev is given,
+// dv = V_sigma - vprev in FP32 (the increment the mixed-precision loop
+// transforms)
+void rpa_vertex_fused(const zd* chi_in, zd* chi_out, long batch, BatchLayout L, int n, const double* S,
+                      const double* C, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s);
 // X = (I + alpha A W)^-1 A  (fchi.f90:319-350, fself.f90:783-816); out may alias nothing
 void rpa_solve(zd* out, BatchLayout Lo, const zd* A, BatchLayout La, long batch, int n, const double* W,
                double alpha, int* err, hipStream_t s);

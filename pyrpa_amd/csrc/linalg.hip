@@ -10,6 +10,7 @@
 #include <cmath>
 
 #include "linalg.h"
+#include "kernels.h"
 
 namespace gmod {
 
@@ -548,10 +549,14 @@ __device__ __forceinline__ zd crecip(zd d) {
 constexpr int GJ_WAVES = 4;
 constexpr int GJ_MAT = 2 * GJ_WAVES;  // matrices per block
 
+// chi_in may equal chi_out (in place: a work-group reads all its matrices
+// before it writes any).  thr > 0 zeroes real and imaginary parts of the
+// input below thr as they are read; dv (with vprev) receives V - vprev in FP32.
 template <int N>
 __global__ void __launch_bounds__(64 * GJ_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
-k_vertex_gj(zd* __restrict__ chi, long batch, long es, const VSparse* __restrict__ spg, const double* __restrict__ Sd,
-            const double* __restrict__ Cd, int mode, int* err) {
+k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* __restrict__ spg,
+            const double* __restrict__ Sd, const double* __restrict__ Cd, int mode, double thr,
+            const zd* __restrict__ vprev, cpx<float>* __restrict__ dv, int* err) {
   static_assert(N == 16, "a 16-lane group per system");
   constexpr int NN = N * N;
   constexpr int PER = GJ_MAT * NN / (64 * GJ_WAVES);  // elements per thread in block-wide phases
@@ -585,7 +590,12 @@ k_vertex_gj(zd* __restrict__ chi, long batch, long es, const VSparse* __restrict
     for (int i = tid; i < GJ_MAT * NN; i += 64 * GJ_WAVES) {
       const int m = i % GJ_MAT, e = i / GJ_MAT;
       const long b = b0 + m;
-      Abuf[m * NN + e] = b < batch ? chi[b + (long)e * es] : zd(0, 0);
+      zd v = b < batch ? chi_in[b + (long)e * es] : zd(0, 0);
+      if (thr > 0.0) {
+        if (fabs(v.re) < thr) v.re = 0.0;
+        if (fabs(v.im) < thr) v.im = 0.0;
+      }
+      Abuf[m * NN + e] = v;
     }
     __syncthreads();
     // ---- V = st (- 0.25 W A W for mode 0), block-wide
@@ -707,7 +717,12 @@ k_vertex_gj(zd* __restrict__ chi, long batch, long es, const VSparse* __restrict
     for (int i = tid; i < GJ_MAT * NN; i += 64 * GJ_WAVES) {
       const int m = i % GJ_MAT, e = i / GJ_MAT;
       const long b = b0 + m;
-      if (b < batch) chi[b + (long)e * es] = Vbuf[m * NN + e];
+      if (b < batch) {
+        const zd v = Vbuf[m * NN + e];
+        const long o = b + (long)e * es;
+        chi_out[o] = v;
+        if (dv) dv[o] = cpx<float>(v - vprev[o]);
+      }
     }
     __syncthreads();
   }
@@ -743,25 +758,51 @@ static void check_lds(size_t bytes) {
   if (bytes > 160 * 1024) throw Error("Nchi too large for the LDS-resident solver");
 }
 
+// the Gauss-Jordan kernel for Nchi = 16 with sparse symmetric S, C; false
+// when it does not apply (the caller uses the general kernels)
+static bool vertex_gj(const zd* in, zd* out, long batch, BatchLayout L, int n, const double* S, const double* C,
+                      int mode, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s) {
+  if (!(L.bs == 1 && n == 16) || getenv("GMOD_VERTEX_LDS") || getenv("GMOD_VERTEX_REG") ||
+      getenv("GMOD_VERTEX_WAVE"))
+    return false;
+  VSparse sp;
+  if (!vertex_sparse(S, C, n, sp, s)) return false;
+  DBuf<VSparse> spd(1);
+  spd.upload(&sp, 1, s);
+  const long nb = (batch + GJ_MAT - 1) / GJ_MAT;
+  const unsigned grid = (unsigned)std::min<long>(nb, 256L * 32);
+  hipLaunchKernelGGL(k_vertex_gj<16>, dim3(grid), dim3(64 * GJ_WAVES), 0, s, in, out, batch, L.es, spd.p, S, C,
+                     mode, thr, vprev, dv, err);
+  GMOD_CHECK_LAUNCH();
+  GMOD_HIP(hipStreamSynchronize(s));  // spd is released at return
+  return true;
+}
+
+void rpa_vertex_fused(const zd* chi_in, zd* chi_out, long batch, BatchLayout L, int n, const double* S,
+                      const double* C, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s) {
+  if (batch <= 0) return;
+  if (vertex_gj(chi_in, chi_out, batch, L, n, S, C, 0, thr, vprev, dv, err, s)) return;
+  const long nel = (long)n * n * L.es;  // the batch's extent (L.bs == 1 layouts; general kernels otherwise)
+  if (L.bs != 1) throw Error("rpa_vertex_fused: matrix-major layouts only");
+  if (thr > 0.0) threshold_copy(chi_out, chi_in, nel, thr, s);
+  else if (chi_out != chi_in) GMOD_HIP(hipMemcpyAsync(chi_out, chi_in, nel * sizeof(zd), hipMemcpyDeviceToDevice, s));
+  rpa_vertex(chi_out, batch, L, n, S, C, 0, err, s);
+  if (dv) diff_to_f32(dv, chi_out, vprev, nel, s);
+}
+
 void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, const double* C, int mode, int* err,
                 hipStream_t s) {
   if (batch <= 0) return;
+  if (vertex_gj(chi, chi, batch, L, n, S, C, mode, 0.0, nullptr, nullptr, err, s)) return;
   if (L.bs == 1 && n == 16 && !getenv("GMOD_VERTEX_LDS") && !getenv("GMOD_VERTEX_REG")) {
     VSparse sp;
     if (vertex_sparse(S, C, n, sp, s)) {
       DBuf<VSparse> spd(1);
       spd.upload(&sp, 1, s);
-      if (getenv("GMOD_VERTEX_WAVE")) {
-        const long nb = (batch + VW_MAT - 1) / VW_MAT;
-        const unsigned grid = (unsigned)std::min<long>(nb, 256L * 16);
-        hipLaunchKernelGGL(k_vertex_wave<16>, dim3(grid), dim3(64 * VW_MAT), 0, s, chi, batch, L.es, spd.p, S, C,
-                           mode, err);
-      } else {
-        const long nb = (batch + GJ_MAT - 1) / GJ_MAT;
-        const unsigned grid = (unsigned)std::min<long>(nb, 256L * 32);
-        hipLaunchKernelGGL(k_vertex_gj<16>, dim3(grid), dim3(64 * GJ_WAVES), 0, s, chi, batch, L.es, spd.p, S, C,
-                           mode, err);
-      }
+      const long nb = (batch + VW_MAT - 1) / VW_MAT;
+      const unsigned grid = (unsigned)std::min<long>(nb, 256L * 16);
+      hipLaunchKernelGGL(k_vertex_wave<16>, dim3(grid), dim3(64 * VW_MAT), 0, s, chi, batch, L.es, spd.p, S, C,
+                         mode, err);
       GMOD_CHECK_LAUNCH();
       GMOD_HIP(hipStreamSynchronize(s));  // spd is released at return
       return;
