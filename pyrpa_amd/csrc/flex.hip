@@ -239,13 +239,18 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
   // ---- chemical potential state (renew_mu, fself.f90:544-685)
   bool bracket_valid = false;
   double muS_cache = 0.0, muL_cache = 0.0;
+  // n(mu) (fself.f90:687-714) through the characteristic polynomials of
+  // H + Sigma, formed once per renew_mu (Sigma is fixed while mu moves)
+  DBuf<zd> cpoly;
   auto get_rn = [&](double rmu) {
     double parts[2];
-    filling_parts(parts, sig.p, ham_d.p, eig_d.p, rmu, T, Nk, (int)Nw, No, err.p, s);
+    filling_parts_cp(parts, cpoly.p, eig_d.p, rmu, T, Nk, (int)Nw, No, s);
     return (parts[0] + 2.0 * T * parts[1]) / (double)Nk;
   };
   const double rfill = A.rfill;
   auto renew_mu = [&]() {
+    if (cpoly.n != (size_t)(NkNw * No)) cpoly.alloc((size_t)NkNw * No);
+    filling_charpoly(cpoly.p, sig.p, ham_d.p, Nk, (int)Nw, No, s);
     double eps = esterr > 1.0e-2 ? 1.0e-8 : esterr * 0.1;
     double dmu = std::fabs(mu - mu_old) * 2.0;
     if (dmu < eps * 4.0) dmu = eps * 4.0;

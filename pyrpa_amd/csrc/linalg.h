@@ -37,6 +37,12 @@ void dyson(zd* G, const zd* sigma, const zd* hamk, double mu, double temp, long 
 // n(mu) pieces (fself.f90:687-714): returns {sum_k,n f(e), sum_{k,j} Re[Tr(z-H-Sigma)^-1 - sum_n 1/(z-e_n)]}
 void filling_parts(double out[2], const zd* sigma, const zd* hamk, const double* eig, double rmu, double temp,
                    long Nk, int Nw, int No, int* err, hipStream_t s);
+// The same n(mu) pieces through the characteristic polynomials of H + Sigma:
+// filling_charpoly writes coef(NkNw, No) once per Sigma, filling_parts_cp
+// evaluates Tr (z - H - Sigma)^-1 = p'(z)/p(z) at any mu
+void filling_charpoly(zd* coef, const zd* sigma, const zd* hamk, long Nk, int Nw, int No, hipStream_t s);
+void filling_parts_cp(double out[2], const zd* coef, const double* eig, double rmu, double temp, long Nk, int Nw,
+                      int No, hipStream_t s);
 // G0 (fself.f90:1-41)
 void green0(zd* G, const double* eig, const zd* uni, double mu, double temp, long Nk, int Nw, int No,
             hipStream_t s);

@@ -1125,6 +1125,92 @@ __global__ void k_filling(const zd* sig, const zd* H, const double* eig, double 
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = sh[0][0]; part[2 * blockIdx.x + 1] = sh[1][0]; }
 }
 
+// Characteristic polynomial of M = H(k) + Sigma(k, i w_j) per (k, j):
+// det(z - M) = z^NO + a_1 z^(NO-1) + ... + a_NO by Faddeev-LeVerrier
+// (B_0 = I, A_i = M B_(i-1), a_i = -Tr(A_i) / i, B_i = A_i + a_i I).
+// coef(e, i-1) = a_i.  Tr (z - M)^-1 = p'(z) / p(z) then serves every trial
+// mu of the chemical-potential search at the cost of a Horner evaluation.
+template <int NO>
+__global__ void k_charpoly(zd* coef, const zd* sig, const zd* H, long Nk, int Nw) {
+  const long tot = Nk * (long)Nw;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    const long k = e % Nk;
+    zd M[NO][NO], B[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        M[p][q] = H[p + NO * (q + NO * k)] + sig[e + tot * (p + NO * q)];
+        B[p][q] = M[p][q];  // A_1 = M
+      }
+#pragma unroll
+    for (int i = 1; i <= NO; ++i) {
+      if (i > 1) {  // A_i = M B_(i-1), into B
+        zd A[NO][NO];
+#pragma unroll
+        for (int p = 0; p < NO; ++p)
+#pragma unroll
+          for (int q = 0; q < NO; ++q) {
+            zd acc(0, 0);
+#pragma unroll
+            for (int r = 0; r < NO; ++r) acc += M[p][r] * B[r][q];
+            A[p][q] = acc;
+          }
+#pragma unroll
+        for (int p = 0; p < NO; ++p)
+#pragma unroll
+          for (int q = 0; q < NO; ++q) B[p][q] = A[p][q];
+      }
+      zd tr(0, 0);
+#pragma unroll
+      for (int p = 0; p < NO; ++p) tr += B[p][p];
+      const zd a = tr * (-1.0 / (double)i);
+      coef[e + tot * (i - 1)] = a;
+#pragma unroll
+      for (int p = 0; p < NO; ++p) B[p][p] += a;  // B_i = A_i + a_i I
+    }
+  }
+}
+
+// n(mu) pieces from the characteristic polynomials (see k_charpoly)
+template <int NO>
+__global__ void k_filling_cp(const zd* coef, const double* eig, double rmu, double temp, long Nk, int Nw,
+                             double* part) {
+  __shared__ double sh[2][256];
+  const long tot = Nk * (long)Nw;
+  double fsum = 0.0, dsum = 0.0;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    const long k = e % Nk;
+    const long j = e / Nk;
+    const zd z(rmu, (double)(2 * j + 1) * PI_D * temp);
+    zd p(1.0, 0), dp(0, 0);  // Horner for p and p'
+#pragma unroll
+    for (int i = 0; i < NO; ++i) {
+      dp = dp * z + p;
+      p = p * z + coef[e + tot * i];
+    }
+    double d = (dp / p).re;
+#pragma unroll
+    for (int n = 0; n < NO; ++n) d -= (zd(1.0, 0) / (z - zd(eig[n + NO * k], 0))).re;
+    dsum += d;
+    if (j == 0) {
+#pragma unroll
+      for (int n = 0; n < NO; ++n) fsum += 0.5 * (1.0 - tanh(0.5 * (eig[n + NO * k] - rmu) / temp));
+    }
+  }
+  sh[0][threadIdx.x] = fsum;
+  sh[1][threadIdx.x] = dsum;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = sh[0][0]; part[2 * blockIdx.x + 1] = sh[1][0]; }
+}
+
 template <int NO>
 __global__ void k_green0(zd* G, const double* eig, const zd* U, double mu, double temp, long Nk, int Nw) {
   const long tot = Nk * (long)Nw;
@@ -1253,6 +1339,25 @@ void filling_parts(double out[2], const zd* sigma, const zd* hamk, const double*
   LAUNCH_NO(No, k_filling, dim3(NB), dim3(256), sigma, hamk, eig, rmu, temp, Nk, Nw, part.p, err);
   GMOD_CHECK_LAUNCH();
   (void)tot;
+  std::vector<double> h(2 * NB);
+  GMOD_HIP(hipMemcpyAsync(h.data(), part.p, 2 * NB * sizeof(double), hipMemcpyDeviceToHost, s));
+  GMOD_HIP(hipStreamSynchronize(s));
+  out[0] = out[1] = 0.0;
+  for (int b = 0; b < NB; ++b) { out[0] += h[2 * b]; out[1] += h[2 * b + 1]; }
+}
+
+void filling_charpoly(zd* coef, const zd* sigma, const zd* hamk, long Nk, int Nw, int No, hipStream_t s) {
+  const long tot = Nk * (long)Nw;
+  LAUNCH_NO(No, k_charpoly, dim3(nblk(tot, 128)), dim3(128), coef, sigma, hamk, Nk, Nw);
+  GMOD_CHECK_LAUNCH();
+}
+
+void filling_parts_cp(double out[2], const zd* coef, const double* eig, double rmu, double temp, long Nk, int Nw,
+                      int No, hipStream_t s) {
+  constexpr int NB = 1024;
+  DBuf<double> part(2 * NB);
+  LAUNCH_NO(No, k_filling_cp, dim3(NB), dim3(256), coef, eig, rmu, temp, Nk, Nw, part.p);
+  GMOD_CHECK_LAUNCH();
   std::vector<double> h(2 * NB);
   GMOD_HIP(hipMemcpyAsync(h.data(), part.p, 2 * NB * sizeof(double), hipMemcpyDeviceToHost, s));
   GMOD_HIP(hipStreamSynchronize(s));
