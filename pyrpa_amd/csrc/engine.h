@@ -72,6 +72,7 @@ class ConvEngine {
   const FFTPlan<R>* p3;
   DBuf<cpx<R>> Fg, F0g, TA, TB, work, tmp, acc, Vft, Ff;
   DBuf<cpx<R>> Fr, Fd, F0r, F0d, TAr, TBr, TAd, TBd, FVr;  // increment mode
+  DBuf<cpx<R>> Mg;  // mirrored Fg (G(-r,-t)): the second factor of the bubble, read in place
   DBuf<cpx<float>> dvbuf;                                  // vertex increment, k space
   std::vector<std::array<int, 2>> gap_pairs;
   bool vft_cached = false;

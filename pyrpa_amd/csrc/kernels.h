@@ -179,23 +179,25 @@ struct ProductLoad {
     A[nterm] = a; B[nterm] = b; coef[nterm] = c; revt[nterm] = rt; ++nterm;
   }
   MeshDev m;
+  int bmir = 0;  // 1: every B is a mirrored grid (mirror_grid, all revt = 1) read at (r, t)
   template <int N>
   __host__ ProductLoad<R, N> fixed() const {
     ProductLoad<R, N> p{};
     for (int i = 0; i < PROD_MAXT; ++i) { p.A[i] = A[i]; p.B[i] = B[i]; p.coef[i] = coef[i]; p.revt[i] = revt[i]; }
     p.nterm = nterm;
     p.m = m;
+    p.bmir = bmir;
     return p;
   }
-  using Key = unsigned;  // spatial index of -r
-  __device__ Key key(unsigned sp) const { return (unsigned)m.rev[sp]; }
+  using Key = unsigned;  // spatial index of -r (no load with mirrored B)
+  __device__ Key key(unsigned sp) const { return bmir ? sp : (unsigned)m.rev[sp]; }
   __device__ cpx<R> value(Key rs, unsigned sp, unsigned t) const {
     const size_t ad = sp + (size_t)m.Nkall * t;
     const unsigned rtt = t == 0 ? 0u : (unsigned)m.M - t;
     cpx<R> v(0, 0);
 #pragma unroll
     for (int i = 0; i < (NT ? NT : nterm); ++i) {
-      const unsigned rt = revt[i] ? rtt : t;
+      const unsigned rt = (revt[i] && !bmir) ? rtt : t;
       v += A[i][ad] * B[i][rs + (size_t)m.Nkall * rt] * coef[i];
     }
     return v;
@@ -252,6 +254,7 @@ struct IncProductLoad {
   int revt[INC_MAXT];
   int nterm;
   MeshDev m;
+  int bmir = 0;  // 1: every Br, Bd is a mirrored grid (all revt = 1) read at (r, t)
   template <int N>
   __host__ IncProductLoad<R, N> fixed() const {
     IncProductLoad<R, N> p{};
@@ -260,14 +263,15 @@ struct IncProductLoad {
     }
     p.nterm = nterm;
     p.m = m;
+    p.bmir = bmir;
     return p;
   }
   __host__ void add(const cpx<R>* ar, const cpx<R>* ad, const cpx<R>* br, const cpx<R>* bd, R c, int rt) {
     if (nterm >= INC_MAXT) throw Error("too many increment terms");
     Ar[nterm] = ar; Ad[nterm] = ad; Br[nterm] = br; Bd[nterm] = bd; coef[nterm] = c; revt[nterm] = rt; ++nterm;
   }
-  using Key = unsigned;  // spatial index of -r
-  __device__ Key key(unsigned sp) const { return (unsigned)m.rev[sp]; }
+  using Key = unsigned;  // spatial index of -r (no load with mirrored B)
+  __device__ Key key(unsigned sp) const { return bmir ? sp : (unsigned)m.rev[sp]; }
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
   __device__ cpx<R> value(Key rs, unsigned sp, unsigned t) const {
     const size_t ad = sp + (size_t)m.Nkall * t;
@@ -275,7 +279,7 @@ struct IncProductLoad {
     cpx<R> v(0, 0);
 #pragma unroll
     for (int i = 0; i < (NT ? NT : nterm); ++i) {
-      const size_t bd = rs + (size_t)m.Nkall * (revt[i] ? rtt : t);
+      const size_t bd = rs + (size_t)m.Nkall * ((revt[i] && !bmir) ? rtt : t);
       const cpx<R> bdv = Bd[i][bd];
       v += (Ad[i][ad] * (Br[i][bd] + bdv) + Ar[i][ad] * bdv) * coef[i];
     }
@@ -400,6 +404,10 @@ void mul_acc(cpx<R>* acc, const cpx<R>* A, const cpx<R>* B, bool first, long n, 
 // ---- elementwise
 void conj_copy(zd* dst, const zd* src, long n, hipStream_t s);
 void threshold(zd* a, long n, double eps, hipStream_t s);
+// dst(r, t) = src(-r, -t) on an (Nx, Ny, Nz, M) grid (the operand a
+// (-r, -tau) product reads, laid out to be read in place)
+template <typename R>
+void mirror_grid(cpx<R>* dst, const cpx<R>* src, int Nx, int Ny, int Nz, int M, hipStream_t s);
 // dst(k, e) = src(k, w = 0, e) with threshold eps, e < nmat (the first
 // frequency slice of a (Nk, Nw, nmat) array, as a (Nk, nmat) array)
 void threshold_slice(zd* dst, const zd* src, long Nk, long NkNw, long nmat, double eps, hipStream_t s);

@@ -150,6 +150,28 @@ void threshold_copy(zd* dst, const zd* src, long n, double eps, hipStream_t s) {
   GMOD_CHECK_LAUNCH();
 }
 
+// dst(r, t) = src(-r, -t) on the full grid (t -> (M - t) mod M): an output
+// row of x reads one input row in reverse order, so both sides coalesce
+template <typename R>
+__global__ void k_mirror_grid(cpx<R>* __restrict__ dst, const cpx<R>* __restrict__ src, int Nx, int Ny, int Nz,
+                              int M) {
+  const long S = (long)Nx * Ny * Nz, n = S * M;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+    const long sp = g % S, t = g / S;
+    const int x = (int)(sp % Nx), y = (int)((sp / Nx) % Ny), z = (int)(sp / ((long)Nx * Ny));
+    const long rs = (x ? Nx - x : 0) + (long)Nx * ((y ? Ny - y : 0) + (long)Ny * (z ? Nz - z : 0));
+    dst[g] = src[rs + S * (t ? M - t : 0)];
+  }
+}
+template <typename R>
+void mirror_grid(cpx<R>* dst, const cpx<R>* src, int Nx, int Ny, int Nz, int M, hipStream_t s) {
+  const long n = (long)Nx * Ny * Nz * M;
+  hipLaunchKernelGGL((k_mirror_grid<R>), dim3(nblocks(n)), dim3(TPB), 0, s, dst, src, Nx, Ny, Nz, M);
+  GMOD_CHECK_LAUNCH();
+}
+template void mirror_grid<float>(cpx<float>*, const cpx<float>*, int, int, int, int, hipStream_t);
+template void mirror_grid<double>(cpx<double>*, const cpx<double>*, int, int, int, int, hipStream_t);
+
 __global__ void k_threshold_slice(zd* dst, const zd* src, long Nk, long NkNw, long nmat, double eps) {
   const long n = Nk * nmat;
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
