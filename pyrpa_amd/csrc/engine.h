@@ -73,6 +73,11 @@ class ConvEngine {
   DBuf<cpx<R>> Fg, F0g, TA, TB, work, tmp, acc, Vft, Ff;
   DBuf<cpx<R>> Fr, Fd, F0r, F0d, TAr, TBr, TAd, TBd, FVr;  // increment mode
   DBuf<cpx<R>> Mg;  // mirrored Fg (G(-r,-t)): the second factor of the bubble, read in place
+  // Sigma through the accumulating last pass: the sources of one output
+  // after their first passes, and the device table of their operand grids
+  DBuf<cpx<R>> srcs;
+  DBuf<cpx<R>*> accum_tab;
+  std::vector<cpx<R>*> accum_tab_h;
   DBuf<cpx<float>> dvbuf;                                  // vertex increment, k space
   std::vector<std::array<int, 2>> gap_pairs;
   bool vft_cached = false;

@@ -11,6 +11,8 @@
 //    combined in (r,tau) space before a single inverse FFT per pair.
 #include <cmath>
 
+#include <array>
+
 #include "engine.h"
 
 namespace gmod {
@@ -246,18 +248,53 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double*
   const MeshDev md = mesh_dev(mesh);
   const R invN = (R)(1.0 / (double)p4->transformed_points());
   const long NC = NkNw;
+  // the accumulating last pass sums the pairs of one output in registers
+  const bool fused = p4->accum_ok(-1) && !getenv("GMOD_SIGMA_UNFUSED");
+  std::vector<std::array<int, 2>> pairs;
+  if (fused) {
+    accum_tab_h.clear();
+    int nmax = 0;
+    for (int l1 = 0; l1 < No; ++l1)
+      for (int m1 = 0; m1 < No; ++m1) {
+        int n = 0;
+        for (int l = 0; l < Nchi; ++l)
+          for (int m = 0; m < Nchi; ++m)
+            if (op.ol[m][0] == m1 && op.ol[l][0] == l1) {
+              accum_tab_h.push_back(grid(Fg, op.ol[m][1], op.ol[l][1]));
+              ++n;
+            }
+        nmax = std::max(nmax, n);
+      }
+    if (srcs.n != (size_t)nmax * N) srcs.alloc((size_t)nmax * N);
+    accum_tab.alloc(accum_tab_h.size());
+    accum_tab.upload(accum_tab_h.data(), accum_tab_h.size(), s);
+    GMOD_HIP(hipStreamSynchronize(s));
+  }
+  long tab_off = 0;
   for (int l1 = 0; l1 < No; ++l1)
     for (int m1 = 0; m1 < No; ++m1) {
       bool first = true;
+      int nsrc = 0;
       for (int l = 0; l < Nchi; ++l)
         for (int m = 0; m < Nchi; ++m) {
           if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
           zd stat(1.5 * S_h[m + Nchi * l] - 0.5 * C_h[m + Nchi * l], 0);
           BosonLoad<R> bl{V, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md};
-          p4->exec_fused(bl, MulAccStore<R>{acc.p, grid(Fg, op.ol[m][1], op.ol[l][1]), first ? 1 : 0, (unsigned)mesh.Nkall}, work.p,
-                         tmp.p, -1, (R)1, s);
+          if (fused) {
+            p4->exec_head(bl, srcs.p + (long)nsrc * N, tmp.p, -1, s);
+          } else {
+            p4->exec_fused(bl, MulAccStore<R>{acc.p, grid(Fg, op.ol[m][1], op.ol[l][1]), first ? 1 : 0,
+                                              (unsigned)mesh.Nkall},
+                           work.p, tmp.p, -1, (R)1, s);
+          }
           first = false;
+          ++nsrc;
         }
+      if (fused && nsrc > 0) {
+        p4->exec_accum_last(srcs.p, N, nsrc, MulAccAccum<R>{accum_tab.p + tab_off, acc.p, (unsigned)mesh.Nkall},
+                            -1, (R)1, s);
+        tab_off += nsrc;
+      }
       zd* dst = sig + NkNw * (m1 + (long)No * l1);
       if (first) {
         GMOD_HIP(hipMemsetAsync(dst, 0, NkNw * sizeof(zd), s));
@@ -501,20 +538,54 @@ void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, d
     if (dvbuf.n != (size_t)nC) dvbuf.alloc(nC);
     diff_to_f32(dvbuf.p, V, Vprev, nC, s);
   }
+  const bool fused = p4->accum_ok(-1) && !getenv("GMOD_SIGMA_UNFUSED");
+  if (fused) {
+    accum_tab_h.clear();
+    int nmax = 0;
+    for (int l1 = 0; l1 < No; ++l1)
+      for (int m1 = 0; m1 < No; ++m1) {
+        int n = 0;
+        for (int l = 0; l < Nchi; ++l)
+          for (int m = 0; m < Nchi; ++m)
+            if (op.ol[m][0] == m1 && op.ol[l][0] == l1) {
+              const int a = op.ol[m][1], b = op.ol[l][1];
+              accum_tab_h.push_back(grid(Fr, a, b));
+              accum_tab_h.push_back(grid(Fd, a, b));
+              accum_tab_h.push_back(FVr.p + (size_t)(m + Nchi * l) * N);
+              ++n;
+            }
+        nmax = std::max(nmax, n);
+      }
+    if (srcs.n != (size_t)nmax * N) srcs.alloc((size_t)nmax * N);
+    accum_tab.alloc(accum_tab_h.size());
+    accum_tab.upload(accum_tab_h.data(), accum_tab_h.size(), s);
+    GMOD_HIP(hipStreamSynchronize(s));
+  }
+  long tab_off = 0;
   for (int l1 = 0; l1 < No; ++l1)
     for (int m1 = 0; m1 < No; ++m1) {
       bool first = true;
+      int nsrc = 0;
       for (int l = 0; l < Nchi; ++l)
         for (int m = 0; m < Nchi; ++m) {
           if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
           const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
           BosonLoad<R, float> bl{dvbuf.p, oml, olm, zd(0, 0), md};
-          const int a = op.ol[m][1], b = op.ol[l][1];
-          SigmaIncStore<R> st{acc.p, grid(Fr, a, b), grid(Fd, a, b), FVr.p + (size_t)(m + Nchi * l) * N,
-                              first ? 1 : 0, S};
-          p4->exec_fused(bl, st, work.p, tmp.p, -1, (R)1, s);
+          if (fused) {
+            p4->exec_head(bl, srcs.p + (long)nsrc * N, tmp.p, -1, s);
+          } else {
+            const int a = op.ol[m][1], b = op.ol[l][1];
+            SigmaIncStore<R> st{acc.p, grid(Fr, a, b), grid(Fd, a, b), FVr.p + (size_t)(m + Nchi * l) * N,
+                                first ? 1 : 0, S};
+            p4->exec_fused(bl, st, work.p, tmp.p, -1, (R)1, s);
+          }
           first = false;
+          ++nsrc;
         }
+      if (fused && nsrc > 0) {
+        p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccum<R>{accum_tab.p + 3 * tab_off, acc.p, S}, -1, (R)1, s);
+        tab_off += nsrc;
+      }
       if (first) continue;
       zd* dst = sig_acc + NkNw * (m1 + (long)No * l1);
       p4->exec_fused(GridLoad<R>{acc.p, S}, GatherStore<R>{dst, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 1, md},

@@ -221,6 +221,17 @@ class FFTPlan {
   // the intermediate passes.
   template <class LD, class ST>
   void exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R scale, hipStream_t s) const;
+  // Accumulating form (box_accum_kernel): exec_head runs every pass but the
+  // last for one source (ld -> out; tmp as intermediate); exec_accum_last runs
+  // the last pass of nsrc such sources (src + c stride) into the accumulator
+  // functor ac.  accum_ok: the plan supports it for this sign (box passes,
+  // 2 or 3 of them, a tiled last pass of exactly BOX_TILE_REGS tiles with
+  // radices <= 16).
+  bool accum_ok(int sign) const;
+  template <class LD>
+  void exec_head(LD ld, cpx<R>* out, cpx<R>* tmp, int sign, hipStream_t s) const;
+  template <class AC>
+  void exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale, hipStream_t s) const;
   long size() const { return (long)dims_[0] * dims_[1] * dims_[2] * dims_[3]; }
   unsigned spatial() const { return S_; }
   long transformed_points() const;
@@ -347,6 +358,57 @@ void FFTPlan<R>::exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R 
     else if (last) launch(p, GridLoad<R>{buf_of(p.src), S_}, st, tw, sc, s);
     else launch(p, GridLoad<R>{buf_of(p.src), S_}, GridStore<R>{buf_of(p.dst), S_}, tw, sc, s);
   }
+}
+
+template <typename R>
+bool FFTPlan<R>::accum_ok(int sign) const {
+  if (!box_.ok) return false;
+  const std::vector<BoxArgs>& bp = sign < 0 ? box_.fwd : box_.bwd;
+  const int nb = (int)bp.size();
+  if (nb < 2 || nb > 3) return false;
+  for (const BoxArgs& a : bp)
+    if (a.tl_mul != a.ts_mul || a.tl_str != a.ts_str || a.tw4_n) return false;  // no slot-moving (split) pass
+  const BoxArgs& l = bp[nb - 1];
+  return l.tiled && l.ntiles == BOX_TILE_REGS<R> && box_max_radix(l) <= 16;
+}
+
+template <typename R>
+template <class LD>
+void FFTPlan<R>::exec_head(LD ld, cpx<R>* out, cpx<R>* tmp, int sign, hipStream_t s) const {
+  const int sg = sign < 0 ? 0 : 1;
+  const int nb = (int)(sg == 0 ? box_.fwd : box_.bwd).size();
+  const cpx<R>* twb = twb_[sg];
+  if (nb == 2) {
+    launch_box(sg, 0, ld, GridStore<R>{out, S_}, twb, (R)1, s);
+  } else {  // 3 passes: ld -> tmp -> out
+    launch_box(sg, 0, ld, GridStore<R>{tmp, S_}, twb, (R)1, s);
+    launch_box(sg, 1, GridLoad<R>{tmp, S_}, GridStore<R>{out, S_}, twb, (R)1, s);
+  }
+}
+
+template <typename R>
+template <class AC>
+void FFTPlan<R>::exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale,
+                                 hipStream_t s) const {
+  const int sg = sign < 0 ? 0 : 1;
+  const std::vector<BoxArgs>& bp = sg == 0 ? box_.fwd : box_.bwd;
+  const int i = (int)bp.size() - 1;
+  const BoxArgs& a = bp[i];
+  const size_t shm = (size_t)(a.lds_elems + a.ntwl) * sizeof(cpx<R>);
+  static bool attr_done = false;
+  if (!attr_done) {
+    GMOD_HIP(hipFuncSetAttribute((const void*)&box_accum_kernel<R, 16, AC>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)box_.lds_limit));
+    attr_done = true;
+  }
+  const double N = (double)size();
+  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT,
+               nsrc * (N * sizeof(cpx<R>) + ac.src_bytes(N)) + ac.sum_bytes(N), s);
+  long grid = a.blocks();
+  if (grid > persist_) grid = persist_;
+  hipLaunchKernelGGL((box_accum_kernel<R, 16, AC>), dim3((unsigned)grid), dim3(BOX_THREADS), shm, s, src, stride,
+                     nsrc, ac, (const BoxArgs*)(argsd_[sg] + i), twb_[sg], (const int*)tabs_, scale);
+  GMOD_CHECK_LAUNCH();
 }
 
 // Cached plans keyed by shape and precision.

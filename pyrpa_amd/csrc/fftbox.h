@@ -360,7 +360,9 @@ __host__ __device__ __forceinline__ BoxSub box_sub_at(const BoxArgs& a, int i) {
   }
 }
 
-template <typename R>
+// MAXR: the largest radix compiled in (a kernel that never runs radix 32
+// then does not reserve its registers)
+template <typename R, int MAXR = 32>
 __host__ __device__ __forceinline__ void box_dispatch(cpx<R>* lds, const BoxArgs& a, const BoxSub& sp,
                                                       const cpx<R>* __restrict__ tw, int tid, int nthr) {
   switch (sp.radix) {
@@ -370,12 +372,21 @@ __host__ __device__ __forceinline__ void box_dispatch(cpx<R>* lds, const BoxArgs
     case 5: box_subpass<R, 5>(lds, a, sp, tw, tid, nthr); break;
     case 7: box_subpass<R, 7>(lds, a, sp, tw, tid, nthr); break;
     case 8: box_subpass<R, 8>(lds, a, sp, tw, tid, nthr); break;
-    case 16: box_subpass<R, 16>(lds, a, sp, tw, tid, nthr); break;
+    case 16:
+      if constexpr (MAXR >= 16) box_subpass<R, 16>(lds, a, sp, tw, tid, nthr);
+      break;
     case 32:
-      if constexpr (sizeof(R) == 4) box_subpass<R, 32>(lds, a, sp, tw, tid, nthr);
+      if constexpr (sizeof(R) == 4 && MAXR >= 32) box_subpass<R, 32>(lds, a, sp, tw, tid, nthr);
       break;
     default: break;
   }
+}
+
+// largest radix of a pass's sub-passes
+inline int box_max_radix(const BoxArgs& a) {
+  int r = 1;
+  for (int i = 0; i < a.nsub; ++i) r = a.sub[i].radix > r ? a.sub[i].radix : r;
+  return r;
 }
 
 // position of output index k of a transformed dimension (undoes the in-place
@@ -623,6 +634,118 @@ __device__ __forceinline__ void box_persistent(cpx<R>* lds, const LD& ld, const 
     box_store_tiled<R>(lds, st, a, box_origin(a, box_remap(b, nb, a.xcd_group)), scale, (unsigned)tid, tabs);
     __syncthreads();  // the LDS box is rewritten by the next iteration
     b = nxt;
+  }
+}
+
+template <typename R>
+struct GridLoad;  // fft.h
+
+// Accumulating last pass: nsrc transforms that share this last pass (their
+// earlier passes left them in src + c src_stride, c < nsrc) are summed, each
+// with its own operands, into one grid.  A persistent work-group transforms
+// source c = 0 .. nsrc-1 of its box in turn (the next source, or the next
+// box, prefetched into registers as in box_persistent), adds each source's
+// contribution to a sum kept in registers, and writes the sum once: the sum
+// crosses HBM once instead of twice per source.  The accumulator functor AC
+// provides prep(c, sp, t) (its operand loads for source c), contrib(c, pre,
+// v) (the contribution of transformed value v), side(c, sp, t, pre, v) (its
+// writes besides the sum) and finish(sp, t, sum).  MAXR caps the radices compiled in.
+template <typename R, int MAXR, class AC>
+__global__ void __launch_bounds__(BOX_THREADS)
+box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC ac, const BoxArgs* __restrict__ ap,
+                 const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const BoxArgs& a = *ap;
+  constexpr int TM = BOX_TILE_REGS<R>;
+  constexpr int U = 8;
+  const int tid = threadIdx.x;
+  const unsigned nb = (unsigned)a.blocks();
+  unsigned c4[4];
+  box_tile_coord(a, (unsigned)tid, c4);
+  const unsigned csp = c4[0] + mul24(a.Nx, c4[1]) + mul24(a.Nx * a.Ny, c4[2]);
+  const unsigned ct = mul24(c4[3], (unsigned)a.tl_str);
+  const unsigned l0 = c4[0] + mul24(c4[1], (unsigned)a.lds_stride[1]) + mul24(c4[2], (unsigned)a.lds_stride[2]) +
+                      mul24(c4[3], (unsigned)a.lds_stride[3]);
+  const unsigned ls0 = (unsigned)box_outpos(a, 0, (int)c4[0]) +
+                       mul24((unsigned)box_outpos(a, 1, (int)c4[1]), (unsigned)a.lds_stride[1]) +
+                       mul24((unsigned)box_outpos(a, 2, (int)c4[2]), (unsigned)a.lds_stride[2]) +
+                       mul24((unsigned)box_outpos(a, 3, (int)c4[3]), (unsigned)a.lds_stride[3]);
+  const int* tsp = tabs + a.tab_off;
+  const int* tt = tsp + a.ntiles;
+  const int* tld = tt + a.ntiles;
+  const int* tst = tld + a.ntiles;
+  // the plan guarantees a.ntiles == TM (FFTPlan::accum_ok): every register
+  // slot is a tile
+  // FP32: the next source's box is prefetched into registers; FP64 (twice
+  // the registers per element) loads each box straight into LDS instead
+  constexpr bool PF = sizeof(R) == 4;
+  constexpr int TV = PF ? TM : 1;
+  cpx<R> v[TV], sum[TM];
+  box_twiddles_to_lds<R>(lds, a, tw, tid, BOX_THREADS);  // ordered before use by the loop's first barrier
+  unsigned b = blockIdx.x;
+  int c = 0;
+  if constexpr (PF) {
+    if (b < nb) box_fetch<R, TM>(v, GridLoad<R>{src, a.S}, a, b, nb, csp, ct, tsp, tt);
+  }
+  while (b < nb) {
+    if constexpr (PF) {
+#pragma unroll
+      for (int j = 0; j < TM; ++j)
+        if (j < a.ntiles) lds[l0 + (unsigned)tld[j]] = v[j];
+    } else {
+      const BoxOrigin O = box_origin(a, box_remap(b, nb, a.xcd_group));
+      const unsigned sp0 = box_sp(a, O, 0, 0, 0) + csp;
+      const unsigned t0 = mul24((unsigned)O.bt, (unsigned)a.tl_mul) + ct;
+      const cpx<R>* sc = src + (long)c * src_stride;
+#pragma unroll
+      for (int j0 = 0; j0 < TM; j0 += 8) {
+        cpx<R> w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = sc[sp0 + (unsigned)tsp[j0 + u] + __umul24(a.S, t0 + (unsigned)tt[j0 + u])];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) lds[l0 + (unsigned)tld[j0 + u]] = w[u];
+      }
+    }
+    __syncthreads();
+    const bool more = c + 1 < nsrc;
+    const unsigned b2 = more ? b : b + gridDim.x;
+    const int c2 = more ? c + 1 : 0;
+    const GridLoad<R> ld2{src + (long)c2 * src_stride, a.S};
+    for (int i = 0; i < a.nsub; ++i) {
+      if constexpr (PF) {
+        if (i == a.pf_at && b2 < nb) box_fetch<R, TM>(v, ld2, a, b2, nb, csp, ct, tsp, tt);
+      }
+      box_dispatch<R, MAXR>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS);
+      __syncthreads();
+    }
+    if constexpr (PF) {
+      if (a.pf_at >= a.nsub && b2 < nb) box_fetch<R, TM>(v, ld2, a, b2, nb, csp, ct, tsp, tt);
+    }
+    // contributions of source c
+    const BoxOrigin O = box_origin(a, box_remap(b, nb, a.xcd_group));
+    const unsigned sp0 = box_sp(a, O, 0, 0, 0) + csp;
+    const unsigned t0 = mul24((unsigned)O.bt, (unsigned)a.ts_mul) + mul24(c4[3], (unsigned)a.ts_str);
+#pragma unroll
+    for (int j0 = 0; j0 < TM; j0 += U) {
+      typename AC::Pre pre[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) pre[u] = ac.prep(c, sp0 + (unsigned)tsp[j0 + u], t0 + (unsigned)tt[j0 + u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u;
+        const cpx<R> val = lds[ls0 + (unsigned)tst[j]] * scale;
+        sum[j] = c == 0 ? ac.contrib(c, pre[u], val) : sum[j] + ac.contrib(c, pre[u], val);
+        ac.side(c, sp0 + (unsigned)tsp[j], t0 + (unsigned)tt[j], pre[u], val);
+      }
+    }
+    if (!more) {
+#pragma unroll
+      for (int j = 0; j < TM; ++j) ac.finish(sp0 + (unsigned)tsp[j], t0 + (unsigned)tt[j], sum[j]);
+    }
+    __syncthreads();  // the LDS box is rewritten by the next iteration
+    b = b2;
+    c = c2;
   }
 }
 

@@ -355,6 +355,44 @@ struct MulAccStore {
   __device__ bool skip(unsigned) const { return false; }
 };
 
+// Accumulator functors of FFTPlan::exec_accum_last.  The per-source operand
+// pointers sit in a device table (tab), so that a run-time source index does
+// not index a kernel argument.
+//   SigmaIncAccum: sum_c [dV~_c (Fr_c + Fd_c) + FVr_c Fd_c], FVr_c += dV~_c
+//   (SigmaIncStore's increment summed over the sources of one output)
+//   MulAccAccum:   sum_c v_c F_c  (MulAccStore's sum)
+template <typename R>
+struct SigmaIncAccum {
+  cpx<R>* const* tab;  // [c][0..2] = Fr_c, Fd_c, FVr_c
+  cpx<R>* out;
+  unsigned S;
+  struct Pre { cpx<R> fr, fd, fv; };
+  __device__ Pre prep(int c, unsigned sp, unsigned t) const {
+    const size_t ad = sp + __umul24(S, t);
+    return Pre{tab[3 * c][ad], tab[3 * c + 1][ad], tab[3 * c + 2][ad]};
+  }
+  __device__ cpx<R> contrib(int, const Pre& q, cpx<R> v) const { return v * (q.fr + q.fd) + q.fv * q.fd; }
+  __device__ void side(int c, unsigned sp, unsigned t, const Pre& q, cpx<R> v) const {  // FVr_c += v
+    tab[3 * c + 2][sp + __umul24(S, t)] = q.fv + v;
+  }
+  __device__ void finish(unsigned sp, unsigned t, cpx<R> sum) const { out[sp + __umul24(S, t)] = sum; }
+  __host__ double src_bytes(double N) const { return 4.0 * N * sizeof(cpx<R>); }  // Fr, Fd, FVr read, FVr written
+  __host__ double sum_bytes(double N) const { return N * sizeof(cpx<R>); }
+};
+template <typename R>
+struct MulAccAccum {
+  cpx<R>* const* tab;  // [c] = F_c
+  cpx<R>* out;
+  unsigned S;
+  struct Pre { cpx<R> f; };
+  __device__ Pre prep(int c, unsigned sp, unsigned t) const { return Pre{tab[c][sp + __umul24(S, t)]}; }
+  __device__ cpx<R> contrib(int, const Pre& q, cpx<R> v) const { return v * q.f; }
+  __device__ void side(int, unsigned, unsigned, const Pre&, cpx<R>) const {}
+  __device__ void finish(unsigned sp, unsigned t, cpx<R> sum) const { out[sp + __umul24(S, t)] = sum; }
+  __host__ double src_bytes(double N) const { return N * sizeof(cpx<R>); }
+  __host__ double sum_bytes(double N) const { return N * sizeof(cpx<R>); }
+};
+
 // dst(k, t) (+)= w * v + J * sawc(t) for the flag-0 point of irreducible k and
 // slots t < Nw (fchi.f90:601-609, fself.f90:258-267); other slots are dropped
 template <typename R>

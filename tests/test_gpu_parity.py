@@ -173,6 +173,40 @@ def test_mkself_mixed_precision_matches_fp64(F, case):
     assert abs(m32 - m64) < 1e-6
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, monkeypatch):
+    """The Sigma step's accumulating last pass (all vertex pairs of one output
+    transformed in one persistent pass, their sum kept in registers) against
+    the per-pair passes that accumulate through memory
+    (GMOD_SIGMA_UNFUSED=1), on a 16x16x4 x 64-slot grid, where the last pass
+    of the forward transform fills every tile register (the only grids that
+    take the accumulating path).  Four iterations of FP64 FLEX and of the
+    mixed loop (FP64 first iteration, then FP32 increments): the same
+    arithmetic in the same order up to contraction, so 1e-12 (FP64) and 1e-6
+    (FP32 increments) of max|Sigma|."""
+    st = MD.two_orbital(Nx=16, Ny=16, Nz=4, Nw=32, temp=0.05)
+    S, C = O.get_scmat(st['olist'], st['site'], 0.8, 0.1)
+    args = (S, C, st['kmap'], st['invk'], st['olist'], st['hamk'], st['eig'], st['uni'], 0.0, 0.9, st['temp'],
+            st['Nw'], st['Nx'], st['Ny'], st['Nz'], False, False)
+    opts = dict(eps=1e-14, pp=0.5, m_diis=1, scf_loop=4)
+    try:
+        F.set_precision(prec)
+        monkeypatch.delenv("GMOD_SIGMA_UNFUSED", raising=False)
+        sf, mf = F.mkself(*args, **opts)
+        nf = F.flex_stats()
+        monkeypatch.setenv("GMOD_SIGMA_UNFUSED", "1")
+        su, mu_ = F.mkself(*args, **opts)
+        nu = F.flex_stats()
+    finally:
+        F.set_precision(0)
+    assert nf['iterations'] == nu['iterations'] == 4
+    if prec == 1:
+        assert nf['increment_iterations'] >= 2
+    tol = 1e-12 if prec == 0 else 1e-6
+    assert rel(sf, su) < tol
+    assert abs(mf - mu_) < tol
+
+
 def test_mkself_warm_start_from_memory_and_sigma_bin(F, tmp_path, monkeypatch):
     st = MD.one_orbital(Nx=4, Ny=4, Nw=16, temp=0.1)
     S, C = O.get_scmat(st['olist'], st['site'], 1.2, 0.0)
