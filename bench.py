@@ -46,12 +46,13 @@ if ROOT not in sys.path:
 
 METRIC = "time-to-converged FLEX Σ (tol 1e-8): FP32+warm-start vs plain FP64, same GPU"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per FP32 box-pass launch on the 4-orbital 32x32x16 x 1024-Matsubara
-# workload, averaged over all its FP32 box dispatches: rocprofv3 FETCH_SIZE x 2
+# HBM bytes per FP32 FFT-pass launch (box_fft_kernel and box_accum_kernel) on
+# the 4-orbital 32x32x16 x 1024-Matsubara workload, averaged over all its FP32
+# pass dispatches: rocprofv3 FETCH_SIZE x 2
 # + WRITE_SIZE (KiB x 1024), both calibrated on a device copy of known size
 # (FETCH_SIZE reads half the bytes on this device).  Measured offline: counter
 # runs cannot be part of the timed run.
-FP32_BOX_TRAFFIC_BYTES = {3: 1133769162}
+FP32_BOX_TRAFFIC_BYTES = {3: 1491628168}
 
 
 def log(*a):
@@ -227,8 +228,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": FP32_BOX_TRAFFIC_BYTES.get(args.config) if cls == "fft_pass_fp32" else None,
-                     "kernel": f"box_fft_kernel<{'float' if cls == 'fft_pass_fp32' else 'double'}>, all "
-                               f"instantiations", "avg_launch_us": round(1e3 * avg_ms, 2),
+                     "kernel": f"box_fft_kernel<{'float' if cls == 'fft_pass_fp32' else 'double'}> and "
+                               f"box_accum_kernel (the FFT passes), all instantiations", "avg_launch_us": round(1e3 * avg_ms, 2),
                      "launches": int(pc["launches"]), "algorithmic_bytes_per_launch": bytes_per,
                      "bytes_model": "per pass: N b per full grid read or written (fused operands included), "
                                     "irreducible (k,w) slices at their size; average over the launches"},
