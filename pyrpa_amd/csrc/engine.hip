@@ -188,7 +188,9 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
   const R invN = (R)(1.0 / (double)p4->transformed_points());
   if (!mesh.complete) GMOD_HIP(hipMemsetAsync(chi, 0, NkNw * Nchi * Nchi * sizeof(zd), s));
   // without the tail every product is G13(r,t) G42(-r,-t): the mirrored G
-  // grids are formed once (coalesced) and then read in place
+  // grids are formed once (coalesced) and then read in place.  The inverse
+  // transforms run spatial-first, so a (-r,-t) operand that is not mirrored
+  // (the tail terms) is read as whole reversed planes.
   if (!tail) {
     if (Mg.n != Fg.n) Mg.alloc(Fg.n);
     for (int b = 0; b < No; ++b)
@@ -222,8 +224,8 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
     const zd J = tail ? rb[l + Nchi * m] : zd(0, 0);
     zd* dst = chi + NkNw * (m + (long)Nchi * l);
     const GatherStore<R> gst{dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, md};
-    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s);
-    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s);
+    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s, true);
+    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s, true);
     if (self_tail) {
       conj_copy(dst, dst, NkNw, s);
       ProductLoad<R> p0{};
@@ -231,7 +233,7 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
       p0.nterm = 1;
       p0.A[0] = grid(F0g, a13, b13); p0.B[0] = grid(F0g, a42, b42); p0.coef[0] = (R)w; p0.revt[0] = 1;
       p4->exec_fused(p0.template fixed<1>(), GatherStore<R>{dst, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1,
-                     invN, s);
+                     invN, s, true);
       continue;
     }
     if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
@@ -477,7 +479,8 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
     IncProductLoad<R> pl{};
     pl.m = md;
     // (mirrored copies as in chi0 measured no faster here: the 2 No^2 copies
-    // per iteration cost what the in-place reads save)
+    // per iteration cost what the in-place reads save, and the spatial-first
+    // inverse reads the (-r,-t) operands as whole reversed planes)
     pl.add(grid(Fr, a13, b13), grid(Fd, a13, b13), grid(Fr, a42, b42), grid(Fd, a42, b42), (R)(tail ? -w : -1.0), 1);
     if (tail) {
       if (!self_tail)
@@ -489,13 +492,13 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
     if (self_tail) {
       // increment of conj(conv[G] + ref) - conv[G0], formed apart and then added
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
-      p4->exec_fused(pl, GatherStore<R>{slice.p, 1.0, J, sawc_d, 0, md}, work.p, tmp.p, +1, invN, s);
+      p4->exec_fused(pl, GatherStore<R>{slice.p, 1.0, J, sawc_d, 0, md}, work.p, tmp.p, +1, invN, s, true);
       conj_copy(slice.p, slice.p, NkNw, s);
       IncProductLoad<R> p0{};
       p0.m = md;
       p0.add(grid(F0r, a13, b13), grid(F0d, a13, b13), grid(F0r, a42, b42), grid(F0d, a42, b42), (R)w, 1);
       p4->exec_fused(p0.template fixed<1>(), GatherStore<R>{slice.p, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p,
-                     +1, invN, s);
+                     +1, invN, s, true);
       axpy(dst, slice.p, zd(1, 0), NkNw, s);
       continue;
     }
@@ -506,16 +509,16 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
       if (pl.nterm == 1)
         p4->exec_fused(pl.template fixed<1>(), GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p,
-                       +1, invN, s);
+                       +1, invN, s, true);
       else
-        p4->exec_fused(pl, GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p, +1, invN, s);
+        p4->exec_fused(pl, GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p, +1, invN, s, true);
       conj_copy(slice.p, slice.p, NkNw, s);
       axpy(dst, slice.p, zd(1, 0), NkNw, s);
       continue;
     }
     const GatherStore<R> gst{dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, md};
-    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s);
-    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s);
+    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s, true);
+    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s, true);
     if (pr[0] >= 0) conj_copy(chi_acc + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
 }

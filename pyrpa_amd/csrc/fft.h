@@ -219,8 +219,13 @@ class FFTPlan {
   }
   // First pass reads through `ld`, last pass writes through `st`; buf/tmp hold
   // the intermediate passes.
+  // spatial_first (box plans): run the passes in the forward order
+  // (spatial, then t) whatever the sign.  A product load that reads an
+  // operand at (-r, -t) then reads whole mirrored spatial planes, not row
+  // segments that straddle cache lines.
   template <class LD, class ST>
-  void exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R scale, hipStream_t s) const;
+  void exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R scale, hipStream_t s,
+                  bool spatial_first = false) const;
   // Accumulating form (box_accum_kernel): exec_head runs every pass but the
   // last for one source (ld -> out; tmp as intermediate); exec_accum_last runs
   // the last pass of nsrc such sources (src + c stride) into the accumulator
@@ -326,10 +331,11 @@ void FFTPlan<R>::launch_box(int sg, int i, LD ld, ST st, const cpx<R>* tw, R sc,
 
 template <typename R>
 template <class LD, class ST>
-void FFTPlan<R>::exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R scale, hipStream_t s) const {
+void FFTPlan<R>::exec_fused(LD ld, ST st, cpx<R>* buf, cpx<R>* tmp, int sign, R scale, hipStream_t s,
+                            bool spatial_first) const {
   if (box_.ok) {
-    const int sg = sign < 0 ? 0 : 1;
-    const cpx<R>* twb = twb_[sg];
+    const cpx<R>* twb = twb_[sign < 0 ? 0 : 1];  // the passes of either list run with the sign's twiddles
+    const int sg = (sign < 0 || spatial_first) ? 0 : 1;
     const std::vector<BoxArgs>& bp = sg == 0 ? box_.fwd : box_.bwd;
     const int nb = (int)bp.size();
     auto bo = [&](int which) { return which ? tmp : buf; };
