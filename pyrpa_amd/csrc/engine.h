@@ -79,6 +79,13 @@ class ConvEngine {
   DBuf<cpx<R>*> accum_tab;
   std::vector<cpx<R>*> accum_tab_h;
   DBuf<cpx<float>> dvbuf;                                  // vertex increment, k space
+  // boxes of p4's forward last pass (it spans the t axis) that hold a kept
+  // spatial point: a spatial-first gather skips the others
+  DBuf<unsigned char> tlive;
+  double tlive_frac = 1.0;
+  bool tlive_built = false;
+  // the gather of a spatial-first inverse transform
+  GatherStore<R> gather_sf(zd* dst, double w, zd J, const zd* sawc, int acc, hipStream_t s);
   std::vector<std::array<int, 2>> gap_pairs;
   bool vft_cached = false;
   const zd* Vd_dev = nullptr;

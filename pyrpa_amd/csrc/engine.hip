@@ -155,6 +155,25 @@ void ConvEngine<R>::tail_constants(double temp, const double* eig_h, const zd* u
 }
 
 template <typename R>
+GatherStore<R> ConvEngine<R>::gather_sf(zd* dst, double w, zd J, const zd* sawc, int acc, hipStream_t s) {
+  if (!tlive_built) {
+    const std::vector<unsigned char> h = p4->fwd_last_live(mesh.kgat_host, &tlive_frac);
+    if (!h.empty()) {
+      tlive.alloc(h.size());
+      tlive.upload(h.data(), h.size(), s);
+      GMOD_HIP(hipStreamSynchronize(s));  // h is released at return
+    }
+    tlive_built = true;
+  }
+  GatherStore<R> g{dst, w, J, sawc, acc, mesh_dev(mesh)};
+  if (tlive.n) {
+    g.live = tlive.p;
+    g.live_frac = tlive_frac;
+  }
+  return g;
+}
+
+template <typename R>
 void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, const zd* uni_d,
                          const double* eig_h, const zd* uni_h, const int64_t* invk, double mu, hipStream_t s,
                          bool apply_threshold) {
@@ -223,7 +242,7 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
     }
     const zd J = tail ? rb[l + Nchi * m] : zd(0, 0);
     zd* dst = chi + NkNw * (m + (long)Nchi * l);
-    const GatherStore<R> gst{dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, md};
+    const GatherStore<R> gst = gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, s);
     if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s, true);
     else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s, true);
     if (self_tail) {
@@ -232,8 +251,8 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
       p0.m = md;
       p0.nterm = 1;
       p0.A[0] = grid(F0g, a13, b13); p0.B[0] = grid(F0g, a42, b42); p0.coef[0] = (R)w; p0.revt[0] = 1;
-      p4->exec_fused(p0.template fixed<1>(), GatherStore<R>{dst, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p, +1,
-                     invN, s, true);
+      p4->exec_fused(p0.template fixed<1>(), gather_sf(dst, 1.0, zd(0, 0), nullptr, 1, s), work.p, tmp.p, +1, invN, s,
+                     true);
       continue;
     }
     if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
@@ -492,13 +511,13 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
     if (self_tail) {
       // increment of conj(conv[G] + ref) - conv[G0], formed apart and then added
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
-      p4->exec_fused(pl, GatherStore<R>{slice.p, 1.0, J, sawc_d, 0, md}, work.p, tmp.p, +1, invN, s, true);
+      p4->exec_fused(pl, gather_sf(slice.p, 1.0, J, sawc_d, 0, s), work.p, tmp.p, +1, invN, s, true);
       conj_copy(slice.p, slice.p, NkNw, s);
       IncProductLoad<R> p0{};
       p0.m = md;
       p0.add(grid(F0r, a13, b13), grid(F0d, a13, b13), grid(F0r, a42, b42), grid(F0d, a42, b42), (R)w, 1);
-      p4->exec_fused(p0.template fixed<1>(), GatherStore<R>{slice.p, 1.0, zd(0, 0), nullptr, 1, md}, work.p, tmp.p,
-                     +1, invN, s, true);
+      p4->exec_fused(p0.template fixed<1>(), gather_sf(slice.p, 1.0, zd(0, 0), nullptr, 1, s), work.p, tmp.p, +1, invN,
+                     s, true);
       axpy(dst, slice.p, zd(1, 0), NkNw, s);
       continue;
     }
@@ -508,15 +527,15 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
       // increment is conjugated before it is added
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
       if (pl.nterm == 1)
-        p4->exec_fused(pl.template fixed<1>(), GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p,
-                       +1, invN, s, true);
+        p4->exec_fused(pl.template fixed<1>(), gather_sf(slice.p, w, zd(0, 0), nullptr, 0, s), work.p, tmp.p, +1, invN,
+                       s, true);
       else
-        p4->exec_fused(pl, GatherStore<R>{slice.p, w, zd(0, 0), nullptr, 0, md}, work.p, tmp.p, +1, invN, s, true);
+        p4->exec_fused(pl, gather_sf(slice.p, w, zd(0, 0), nullptr, 0, s), work.p, tmp.p, +1, invN, s, true);
       conj_copy(slice.p, slice.p, NkNw, s);
       axpy(dst, slice.p, zd(1, 0), NkNw, s);
       continue;
     }
-    const GatherStore<R> gst{dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, md};
+    const GatherStore<R> gst = gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, s);
     if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s, true);
     else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s, true);
     if (pr[0] >= 0) conj_copy(chi_acc + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);

@@ -237,6 +237,12 @@ class FFTPlan {
   void exec_head(LD ld, cpx<R>* out, cpx<R>* tmp, int sign, hipStream_t s) const;
   template <class AC>
   void exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale, hipStream_t s) const;
+  // Boxes of the forward list's last pass (the last pass of a sign -1 or a
+  // spatial-first transform) that a gather keeping the spatial points with
+  // kept_sp[sp] >= 0 needs: per box index (after box_remap) 1 if the box
+  // holds such a point.  Empty unless that pass's boxes span the t axis and
+  // hold few spatial points each; *frac receives the share of live boxes.
+  std::vector<unsigned char> fwd_last_live(const std::vector<int>& kept_sp, double* frac) const;
   long size() const { return (long)dims_[0] * dims_[1] * dims_[2] * dims_[3]; }
   unsigned spatial() const { return S_; }
   long transformed_points() const;
@@ -273,9 +279,19 @@ struct AlgoBytes<F, std::void_t<decltype(std::declval<const F&>().algo_bytes(1.0
   template <typename R>
   static double get(const F& f, double N) { return f.algo_bytes(N); }
 };
+// Share of the input a pass reads when its store drops whole boxes
+// (read_fraction(t_box); t_box: the pass's boxes span the t axis).
+template <class F, class = void>
+struct ReadFraction {
+  static double get(const F&, bool) { return 1.0; }
+};
+template <class F>
+struct ReadFraction<F, std::void_t<decltype(std::declval<const F&>().read_fraction(true))>> {
+  static double get(const F& f, bool t_box) { return f.read_fraction(t_box); }
+};
 template <typename R, class LD, class ST>
-double pass_bytes(const LD& ld, const ST& st, double N) {
-  return AlgoBytes<LD>::template get<R>(ld, N) + AlgoBytes<ST>::template get<R>(st, N);
+double pass_bytes(const LD& ld, const ST& st, double N, bool t_box = false) {
+  return AlgoBytes<LD>::template get<R>(ld, N * ReadFraction<ST>::get(st, t_box)) + AlgoBytes<ST>::template get<R>(st, N);
 }
 
 inline bool fft_is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -320,7 +336,8 @@ void FFTPlan<R>::launch_box(int sg, int i, LD ld, ST st, const cpx<R>* tw, R sc,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)box_.lds_limit));
     attr_done = true;
   }
-  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, pass_bytes<R>(ld, st, (double)size()), s);
+  ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, pass_bytes<R>(ld, st, (double)size(), a.box[3] == dims_[3]),
+               s);
   // tiled passes: persistent work-groups, one per CU, a multiple of 8
   long grid = a.blocks();
   if (a.tiled && grid > persist_) grid = persist_;

@@ -566,6 +566,34 @@ const FFTPlan<R>& fft_plan(int nx, int ny, int nz, int m, int axes_mask) {
   return *p;
 }
 
+template <typename R>
+std::vector<unsigned char> FFTPlan<R>::fwd_last_live(const std::vector<int>& kept_sp, double* frac) const {
+  std::vector<unsigned char> live;
+  if (frac) *frac = 1.0;
+  if (!box_.ok || box_.fwd.empty() || kept_sp.size() != (size_t)S_) return live;
+  const BoxArgs& a = box_.fwd.back();
+  const long pts = (long)a.box[0] * a.box[1] * a.box[2];
+  if (a.box[3] != dims_[3] || pts > 64) return live;
+  const long nb = a.blocks();
+  live.assign((size_t)nb, 0);
+  long n = 0;
+  for (long lb = 0; lb < nb; ++lb) {
+    const BoxOrigin O = box_origin(a, (unsigned)lb);
+    bool any = false;
+    for (int z = 0; z < a.box[2] && !any; ++z)
+      for (int y = 0; y < a.box[1] && !any; ++y)
+        for (int x = 0; x < a.box[0] && !any; ++x) {
+          const unsigned sp = box_sp(a, O, (unsigned)x, (unsigned)y, (unsigned)z);
+          if (sp >= (unsigned)S_) throw Error("fwd_last_live: spatial index out of range");
+          any = kept_sp[sp] >= 0;
+        }
+    live[(size_t)lb] = any ? 1 : 0;
+    n += any ? 1 : 0;
+  }
+  if (frac) *frac = (double)n / (double)nb;
+  return live;
+}
+
 template struct BoxPlan<double>;
 template struct BoxPlan<float>;
 template class FFTPlan<double>;

@@ -518,12 +518,26 @@ __host__ __device__ __forceinline__ void box_twiddles_to_lds(cpx<R>* lds, const 
   }
 }
 
-// a box whose slots the store keeps none of (the gather keeps t < Nw; the
-// store slots of a box increase with the element index)
+// A store may also drop whole boxes by their index (skip_box(lb), lb the
+// box index after box_remap): the gather of a pass whose boxes span the
+// whole t axis keeps only the boxes that hold a kept spatial point.
+template <class ST, class = void>
+struct BoxSkip {
+  __host__ __device__ static bool skip(const ST&, unsigned) { return false; }
+};
 template <class ST>
-__host__ __device__ __forceinline__ bool box_skipped(const ST& st, const BoxArgs& a, const BoxOrigin& O) {
+struct BoxSkip<ST, std::void_t<decltype(std::declval<const ST&>().skip_box(0u))>> {
+  __device__ static bool skip(const ST& st, unsigned lb) { return st.skip_box(lb); }
+};
+
+// a box whose slots the store keeps none of (the gather keeps t < Nw; the
+// store slots of a box increase with the element index), or that the store
+// drops as a whole
+template <class ST>
+__host__ __device__ __forceinline__ bool box_skipped(const ST& st, const BoxArgs& a, unsigned lb,
+                                                     const BoxOrigin& O) {
   const unsigned t_first = (unsigned)(O.bt * a.ts_mul), t_last = t_first + (unsigned)((a.box[3] - 1) * a.ts_str);
-  return st.skip(t_first) && st.skip(t_last);
+  return (st.skip(t_first) && st.skip(t_last)) || BoxSkip<ST>::skip(st, lb);
 }
 
 // Tiled passes run persistent work-groups (one per CU: a box fills the LDS)
@@ -534,7 +548,11 @@ __host__ __device__ __forceinline__ bool box_skipped(const ST& st, const BoxArgs
 // multiple of 8, so a work-group keeps its XCD and box_remap's grouping holds.
 template <class ST>
 __device__ __forceinline__ unsigned box_next_live(const ST& st, const BoxArgs& a, unsigned b, unsigned nb) {
-  while (b < nb && box_skipped(st, a, box_origin(a, box_remap(b, nb, a.xcd_group)))) b += gridDim.x;
+  while (b < nb) {
+    const unsigned lb = box_remap(b, nb, a.xcd_group);
+    if (!box_skipped(st, a, lb, box_origin(a, lb))) break;
+    b += gridDim.x;
+  }
   return b;
 }
 
@@ -763,11 +781,9 @@ box_fft_kernel(LD ld, ST st, const BoxArgs* __restrict__ ap, const cpx<R>* __res
     box_persistent<R>(lds, ld, st, a, tw, tabs, scale, threadIdx.x);
     return;
   }
-  const BoxOrigin O = box_origin(a, box_remap(blockIdx.x, gridDim.x, a.xcd_group));
-  // a box whose slots the store keeps none of (the gather keeps t < Nw; the
-  // store slots of a box increase with the element index)
-  const unsigned t_first = (unsigned)(O.bt * a.ts_mul), t_last = t_first + (unsigned)((a.box[3] - 1) * a.ts_str);
-  if (st.skip(t_first) && st.skip(t_last)) return;
+  const unsigned lb = box_remap(blockIdx.x, gridDim.x, a.xcd_group);
+  const BoxOrigin O = box_origin(a, lb);
+  if (box_skipped(st, a, lb, O)) return;
   const int tid = threadIdx.x;
   box_twiddles_to_lds<R>(lds, a, tw, tid, BOX_THREADS);
   box_load<R>(lds, ld, a, O, tid, BOX_THREADS, tabs);

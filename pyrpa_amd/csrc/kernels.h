@@ -24,6 +24,7 @@ struct Mesh {
   DBuf<int> irr_s;    // per spatial index: irreducible k
   DBuf<int> flag_s;   // per spatial index: 0 direct / 1 reversed
   DBuf<int> kgat_s;   // per spatial index: irreducible k if this is its flag-0 point, else -1
+  std::vector<int> kgat_host;  // the same on the host
   DBuf<int> kf_s;     // per spatial index: 2 irr_s + (flag_s != 0), one load for both
   DBuf<double> kw;    // full-BZ multiplicity of irreducible k
   std::vector<double> kw_host;
@@ -404,6 +405,11 @@ struct GatherStore {
   int acc;
   MeshDev m;
   const zd* base = nullptr;  // optional: dst = base + w v (FP64 reference + increment)
+  // optional, for a last pass whose boxes span the t axis: per box (index
+  // after box_remap), nonzero if it holds a kept spatial point; live_frac is
+  // the fraction of boxes that do
+  const unsigned char* live = nullptr;
+  double live_frac = 1.0;
   struct Pre { long o; zd old, bv, js; };  // o < 0: slot or point not kept
   // branch-free: a dropped slot or point is not stored, and an operand the
   // store does not use is read at element 0 (one cached line) and dropped
@@ -433,6 +439,10 @@ struct GatherStore {
     return (1.0 + (acc ? 1.0 : 0.0) + (base ? 1.0 : 0.0)) * m.irr_elems() * sizeof(zd);
   }
   __device__ bool skip(unsigned t) const { return (int)t >= m.Nw; }
+  __device__ bool skip_box(unsigned lb) const { return live && !live[lb]; }
+  // share of the pass's input the kept boxes read: boxes spanning the t
+  // axis are dropped through `live`, boxes of a few slots by skip()
+  __host__ double read_fraction(bool t_box) const { return t_box ? (live ? live_frac : 1.0) : (double)m.Nw / m.M; }
 };
 
 // acc (+)= A * B

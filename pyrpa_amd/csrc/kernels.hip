@@ -63,6 +63,7 @@ void Mesh::build(const int64_t* kmap, const int64_t* invk, long Nkall_, long Nk_
   for (long i = 0; i < Nkall; ++i) hkf_s[i] = (hirr_s[i] << 1) | (hflag_s[i] != 0 ? 1 : 0);
   kf_s.upload(hkf_s.data(), Nkall, s);
   kgat_s.upload(hkgat_s.data(), Nkall, s);
+  kgat_host = hkgat_s;
   irr.upload(hirr.data(), Nkall, s);
   flag.upload(hflag.data(), Nkall, s);
   sp.upload(hsp.data(), Nkall, s);

@@ -585,12 +585,29 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
   double* pval = Pval_ + (wave * 4 + g) * PVAL;
   const double a1 = mode == 2 ? -0.5 : 1.5;  // coefficient of S X_s
   const double a2 = mode == 0 ? 0.5 : -0.5;  // coefficient of C X_c
-  __syncthreads();
-  for (long b0 = (long)blockIdx.x * GJ_MAT; b0 < batch; b0 += (long)gridDim.x * GJ_MAT) {
-    for (int i = tid; i < GJ_MAT * NN; i += 64 * GJ_WAVES) {
+  const long bstep = (long)gridDim.x * GJ_MAT;
+  // the work-group's next matrices are read into registers while the
+  // current ones are solved (element i = tid + u 256 of the batch: matrix
+  // i % GJ_MAT, entry i / GJ_MAT); a matrix past the batch reads entry 0 of
+  // matrix 0 and is dropped
+  zd nxt[PER];
+  auto fetch = [&](long b0) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * 64 * GJ_WAVES;
       const int m = i % GJ_MAT, e = i / GJ_MAT;
       const long b = b0 + m;
-      zd v = b < batch ? chi_in[b + (long)e * es] : zd(0, 0);
+      nxt[u] = chi_in[b < batch ? b + (long)e * es : 0];
+    }
+  };
+  __syncthreads();
+  if ((long)blockIdx.x * GJ_MAT < batch) fetch((long)blockIdx.x * GJ_MAT);
+  for (long b0 = (long)blockIdx.x * GJ_MAT; b0 < batch; b0 += bstep) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * 64 * GJ_WAVES;
+      const int m = i % GJ_MAT, e = i / GJ_MAT;
+      zd v = b0 + m < batch ? nxt[u] : zd(0, 0);
       if (thr > 0.0) {
         if (fabs(v.re) < thr) v.re = 0.0;
         if (fabs(v.im) < thr) v.im = 0.0;
@@ -598,6 +615,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
       Abuf[m * NN + e] = v;
     }
     __syncthreads();
+    if (b0 + bstep < batch) fetch(b0 + bstep);
     // ---- V = st (- 0.25 W A W for mode 0), block-wide
     if (mode == 0) {
       for (int i = tid; i < GJ_MAT * NN; i += 64 * GJ_WAVES) {
