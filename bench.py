@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 # + WRITE_SIZE (KiB x 1024), both calibrated on a device copy of known size
 # (FETCH_SIZE reads half the bytes on this device).  Measured offline: counter
 # runs cannot be part of the timed run.
-FP32_BOX_TRAFFIC_BYTES = {3: 1491628168}
+FP32_BOX_TRAFFIC_BYTES = {3: 1249746321}
 
 
 def log(*a):
