@@ -171,7 +171,9 @@ void gmod_regrid_sigma(gmod_z* sig_f, const gmod_z* sig_c, const int64_t* kmap_c
  * runs the box-pass FFT plan of an (nx,ny,nz,m) grid (x fastest, interleaved
  * complex, in place) on the host with the kernel's own load / sub-pass /
  * store code and bounds checks.  mask selects the axes (bit 0: x .. bit 3:
- * t), sign -1 forward / +1 normalised backward, precision 0 FP64 / 1 FP32,
+ * t), sign -1 forward / +1 normalised backward (2: the same backward
+ * transform with the passes in the forward order, spatial axes first, as
+ * the chi0 inverse transforms run it), precision 0 FP64 / 1 FP32,
  * lds_bytes the per-work-group LDS limit to plan for; npasses receives the
  * number of box passes (data == NULL: plan only).  Returns 0, 1 if the box
  * engine does not cover the shape, 2 on an indexing error. */

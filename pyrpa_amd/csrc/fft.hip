@@ -627,6 +627,10 @@ struct HostGridStore {
 template <typename R>
 int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign, long budget, size_t lds_bytes,
                  int* npasses) {
+  // sign 2: the backward transform with the forward pass order
+  // (FFTPlan::exec_fused's spatial_first)
+  const bool spatial_first = sign == 2;
+  if (spatial_first) sign = 1;
   BoxPlan<R> plan;
   plan.build(nx, ny, nz, m, mask, budget, lds_bytes);
   if (!plan.ok) return 1;
@@ -642,7 +646,7 @@ int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign
   // or buffer src, writes buffer dst or the output (last pass); the output is
   // the caller's buffer here
   std::vector<cpx<R>>* bufs[2] = {&g, &gt};
-  const std::vector<BoxArgs>& ps = sign < 0 ? plan.fwd : plan.bwd;
+  const std::vector<BoxArgs>& ps = (sign < 0 || spatial_first) ? plan.fwd : plan.bwd;
   const cpx<R>* tw = plan.tw_host[sign < 0 ? 0 : 1].data();
   long tpts = 1;
   const int n[4] = {nx, ny, nz, m};

@@ -94,3 +94,31 @@ def test_split_t_axis_plan_matches_numpy(prec, monkeypatch):
             ref = np.fft.fftn(a) if sign < 0 else np.fft.ifftn(a)
             err = np.abs(d.reshape(a.shape) - ref).max() / np.abs(ref).max()
             assert err < (1e-13 if prec == 0 else 3e-6), (shape, prec, sign, err)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_spatial_first_backward_matches_numpy(prec):
+    """The chi0 inverse transforms run the forward pass list (spatial axes
+    first) with the backward twiddles (sign 2 of the hook).  A transform is
+    separable, so the order must not change the result: against numpy's
+    ifftn, on the benchmark's box shapes and on the split plans of a small LDS
+    budget.  A pass list run with the wrong twiddle table or without the
+    final 1/N gives an O(1) error here."""
+    f = _hook()
+    tol = 1e-13 if prec == 0 else 3e-6
+    covered = 0
+    for shape in [(32, 32, 16, 16), (8, 8, 1, 320), (16, 4, 2, 1024), (5, 3, 1, 42)]:
+        nx, ny, nz, m = shape
+        rng = np.random.default_rng(m + prec)
+        a = rng.standard_normal((m, nz, ny, nx)) + 1j * rng.standard_normal((m, nz, ny, nx))
+        for lds in (163840, 16384):
+            d = a.ravel().copy()
+            rc = f(d, nx, ny, nz, m, 15, 2, prec, lds, ctypes.byref(ctypes.c_int(0)))
+            assert rc in (0, 1), (shape, lds, rc)
+            if rc == 1:
+                continue
+            covered += 1
+            ref = np.fft.ifftn(a)
+            err = np.abs(d.reshape(a.shape) - ref).max() / np.abs(ref).max()
+            assert err < tol, (shape, prec, lds, err)
+    assert covered >= 4
