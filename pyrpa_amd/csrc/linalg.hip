@@ -528,7 +528,9 @@ k_vertex_wave(zd* __restrict__ chi, long batch, long es, const VSparse* __restri
 
 // Gauss-Jordan variant, one matrix row per lane: a 16-lane group solves one
 // of the two systems of a (q, i nu) matrix ((I - A S) X = A S or
-// (I + A C) X = A C), so a wavefront carries two matrices.  Per elimination
+// (I + A C) X = A C), so a wavefront carries two matrices.  The solution
+// comes from the inverse: X = (I - A S)^-1 - I and X = I - (I + A C)^-1,
+// the inverse formed in place (no separate right-hand side).  Per elimination
 // step the group exchanges only two things through its LDS slot -- the
 // pivot candidates |.|_1 of the column and the normalised pivot row -- so a
 // solve costs two LDS round trips per column instead of one cross-lane
@@ -569,6 +571,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
   constexpr int PVAL = 2 * N + 2;  // doubles per group slot (68 dwords = 4 mod 64)
   __shared__ double2 Prow_[GJ_WAVES * 4 * PROW];
   __shared__ double Pval_[GJ_WAVES * 4 * PVAL];
+  __shared__ int Piv_[GJ_WAVES * 4 * N];  // per group: the pivot row of each column
   zd* Abuf = reinterpret_cast<zd*>(Abuf_);
   zd* Vbuf = reinterpret_cast<zd*>(Vbuf_);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -583,6 +586,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
   zd* A = Abuf + mloc * NN;
   zd* prow = reinterpret_cast<zd*>(Prow_) + (wave * 4 + g) * PROW;
   double* pval = Pval_ + (wave * 4 + g) * PVAL;
+  int* piv = Piv_ + (wave * 4 + g) * N;
   const double a1 = mode == 2 ? -0.5 : 1.5;  // coefficient of S X_s
   const double a2 = mode == 0 ? 0.5 : -0.5;  // coefficient of C X_c
   const long bstep = (long)gridDim.x * GJ_MAT;
@@ -646,26 +650,29 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
     __syncthreads();  // every read of A W is complete
 #pragma unroll
     for (int u = 0; u < PER; ++u) Vbuf[tid + u * 64 * GJ_WAVES] = vinit[u];
-    // ---- my row of A K (K = S or C): the right-hand side and I -+ A K
-    zd mrow[N], rrow[N];
+    // ---- my row of M = I -+ A K (K = S or C)
+    zd mrow[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       zd acc(0, 0);
 #pragma unroll
       for (int t = 0; t < VW_NZ; ++t) acc += A[r + N * sp.idx[sys][j][t]] * sp.val[sys][j][t];
-      rrow[j] = acc;
       mrow[j] = sys == 0 ? zd(0, 0) - acc : acc;
       if (j == r) mrow[j] += zd(1.0, 0);
     }
     __syncthreads();  // all reads of A are complete before X overwrites it
-    // ---- Gauss-Jordan with partial pivoting.  Rows stay unnormalised: the
-    // pivot lane publishes its row as is, the others form the multiplier
-    // from the published pivot, and each lane divides its right-hand side by
-    // its own pivot once at the end.  The candidates of column k + 1 are
-    // published right after column k + 1 of the row is updated, so their
-    // LDS round trip overlaps the rest of the elimination.
+    // ---- Gauss-Jordan inversion in place with partial pivoting and no row
+    // exchanges.  Column k of the row storage turns into a column of the
+    // inverse once column k is eliminated.  Rows stay unnormalised: the
+    // pivot lane publishes its row as is and keeps 1 in the pivot column,
+    // the others form the multiplier from the published pivot, and each
+    // lane divides its row by its own pivot once at the end.  The
+    // candidates of column k + 1 are published right after column k + 1 of
+    // the row is updated, so their LDS round trip overlaps the rest of the
+    // elimination.  At the end the lane that was pivot of column k holds row
+    // k of M^-1, entry j in column piv[j] (the pivot row of column j).
     bool used = false;
-    int xrow = 0;  // the X row this lane holds at the end
+    int xrow = 0;  // the row of M^-1 this lane holds at the end
     bool singular = false;
     zd dpiv(1.0, 0);
     pval[r] = abs1(mrow[0]);
@@ -682,10 +689,10 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
       singular = singular || !(best > 0.0);
       if (r == pr) {
 #pragma unroll
-        for (int j = k; j < N; ++j) prow[j] = mrow[j];
-#pragma unroll
-        for (int j = 0; j < N; ++j) prow[N + j] = rrow[j];
+        for (int j = 0; j < N; ++j) prow[j] = mrow[j];
+        piv[k] = r;
         dpiv = mrow[k];
+        mrow[k] = zd(1.0, 0);
         used = true;
         xrow = k;
       }
@@ -697,9 +704,9 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
           pval[r] = used ? -1.0 : abs1(mrow[k + 1]);
         }
 #pragma unroll
-        for (int j = k + 2; j < N; ++j) csub_mul(mrow[j], f, prow[j]);
-#pragma unroll
-        for (int j = 0; j < N; ++j) csub_mul(rrow[j], f, prow[N + j]);
+        for (int j = 0; j < N; ++j)
+          if (j != k && j != k + 1) csub_mul(mrow[j], f, prow[j]);
+        mrow[k] = zd(0, 0) - f;
       } else if (k + 1 < N) {
         pval[r] = -1.0;
       }
@@ -707,7 +714,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
     {
       const zd inv = crecip(dpiv);
 #pragma unroll
-      for (int j = 0; j < N; ++j) rrow[j] = rrow[j] * inv;
+      for (int j = 0; j < N; ++j) mrow[j] = mrow[j] * inv;
     }
     wave_sync();
     if (singular && r == 0 && b0 + mloc < batch) atomicOr(err, 1);
@@ -715,8 +722,14 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
       if (sys == pass) {
+        // X = (M^-1 - I) for the S system, I - M^-1 for the C system
 #pragma unroll
-        for (int q = 0; q < N; ++q) A[xrow + N * q] = rrow[q];
+        for (int j = 0; j < N; ++j) {
+          const int q = piv[j];
+          zd x = mrow[j];
+          if (q == xrow) x -= zd(1.0, 0);
+          A[xrow + N * q] = sys == 0 ? x : zd(0, 0) - x;
+        }
       }
       __syncthreads();
       const double coef = pass == 0 ? a1 : a2;
