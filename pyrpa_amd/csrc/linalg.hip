@@ -7,6 +7,7 @@
 // * dyson / filling_parts / green0 / anomalous_f: one lane per (k, i w) with
 //   the No x No matrix held in registers (No <= 8).
 // * max_re_eig: one lane per matrix; Hessenberg reduction + shifted complex QR.
+#include <cfloat>
 #include <cmath>
 
 #include "linalg.h"
@@ -548,6 +549,18 @@ __device__ __forceinline__ zd crecip(zd d) {
   return zd(d.re * q, -d.im * q);
 }
 
+// Pivot candidate of row r as one double whose maximum names the pivot:
+// |.|_1 (non-negative; -1 for a used row; inf and NaN clamped to the
+// largest double) with its low 4 mantissa bits replaced by 15 - r.  The
+// bit patterns of non-negative doubles order like the values, so the
+// maximum key carries the largest candidate and, among candidates equal
+// in their upper 60 bits, the lowest row; a max tree of 16 keys replaces a
+// compare-and-select scan.
+__device__ __forceinline__ double gj_key(double v, int r) {
+  const long long b = __double_as_longlong(fmin(v, DBL_MAX));
+  return __longlong_as_double((b & ~0xFLL) | (long long)(15 - r));
+}
+
 constexpr int GJ_WAVES = 4;
 constexpr int GJ_MAT = 2 * GJ_WAVES;  // matrices per block
 
@@ -675,18 +688,21 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
     int xrow = 0;  // the row of M^-1 this lane holds at the end
     bool singular = false;
     zd dpiv(1.0, 0);
-    pval[r] = abs1(mrow[0]);
+    pval[r] = gj_key(abs1(mrow[0]), r);
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       wave_sync();
-      int pr = 0;
-      double best = pval[0];
+      double kv[N];
 #pragma unroll
-      for (int i = 1; i < N; ++i) {
-        const double v = pval[i];
-        if (v > best) { best = v; pr = i; }
-      }
-      singular = singular || !(best > 0.0);
+      for (int i = 0; i < N; ++i) kv[i] = pval[i];
+#pragma unroll
+      for (int w = N / 2; w >= 1; w /= 2)
+#pragma unroll
+        for (int i = 0; i < w; ++i) kv[i] = fmax(kv[i], kv[i + w]);
+      const long long kb = __double_as_longlong(kv[0]);
+      const int pr = 15 - (int)(kb & 0xF);
+      const double best = __longlong_as_double(kb & ~0xFLL);
+      singular = singular || !(best > 0.0 && best < 1e300);  // zero, or a clamped inf / NaN
       if (r == pr) {
 #pragma unroll
         for (int j = 0; j < N; ++j) prow[j] = mrow[j];
@@ -701,14 +717,14 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
         const zd f = mrow[k] * crecip(prow[k]);
         if (k + 1 < N) {
           csub_mul(mrow[k + 1], f, prow[k + 1]);
-          pval[r] = used ? -1.0 : abs1(mrow[k + 1]);
+          pval[r] = gj_key(used ? -1.0 : abs1(mrow[k + 1]), r);
         }
 #pragma unroll
         for (int j = 0; j < N; ++j)
           if (j != k && j != k + 1) csub_mul(mrow[j], f, prow[j]);
         mrow[k] = zd(0, 0) - f;
       } else if (k + 1 < N) {
-        pval[r] = -1.0;
+        pval[r] = gj_key(-1.0, r);
       }
     }
     {
