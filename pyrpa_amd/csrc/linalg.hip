@@ -532,11 +532,11 @@ k_vertex_wave(zd* __restrict__ chi, long batch, long es, const VSparse* __restri
 // (I + A C) X = A C), so a wavefront carries two matrices.  The solution
 // comes from the inverse: X = (I - A S)^-1 - I and X = I - (I + A C)^-1,
 // the inverse formed in place (no separate right-hand side).  Per elimination
-// step the group exchanges only two things through its LDS slot -- the
-// pivot candidates |.|_1 of the column and the normalised pivot row -- so a
-// solve costs two LDS round trips per column instead of one cross-lane
-// transfer per matrix element.  Pivots: the largest |Re|+|Im| among the rows
-// not used yet (LAPACK's izamax criterion).
+// step the group finds the pivot by a maximum over its 16 lanes (DPP
+// rotations within the row) and exchanges one thing through its LDS slot,
+// the unnormalised pivot row: one LDS round trip per column instead of one
+// cross-lane transfer per matrix element.  Pivots: the largest |Re|+|Im|
+// among the rows not used yet (LAPACK's izamax criterion).
 // a -= f b with four fused multiply-adds
 __device__ __forceinline__ void csub_mul(zd& a, zd f, zd b) {
   a.re = fma(-f.re, b.re, fma(f.im, b.im, a.re));
@@ -554,11 +554,30 @@ __device__ __forceinline__ zd crecip(zd d) {
 // largest double) with its low 4 mantissa bits replaced by 15 - r.  The
 // bit patterns of non-negative doubles order like the values, so the
 // maximum key carries the largest candidate and, among candidates equal
-// in their upper 60 bits, the lowest row; a max tree of 16 keys replaces a
-// compare-and-select scan.
-__device__ __forceinline__ double gj_key(double v, int r) {
+// in their upper 60 bits, the lowest row; a maximum over the group's lanes
+// replaces a compare-and-select scan.
+__device__ __forceinline__ long long gj_key(double v, int r) {
   const long long b = __double_as_longlong(fmin(v, DBL_MAX));
-  return __longlong_as_double((b & ~0xFLL) | (long long)(15 - r));
+  return (b & ~0xFLL) | (long long)(15 - r);
+}
+// One rotation step of the keys within the DPP row of 16 lanes (the group of
+// one system): the larger of this lane's key and that of the lane ROT
+// places further round.  Keys of used rows are negative, every other key
+// non-negative, so signed 64-bit order is the order of the candidates.
+template <int ROT>
+__device__ __forceinline__ long long gj_row_max_step(long long k) {
+  constexpr int ctrl = 0x120 + ROT;  // DPP row_ror:ROT
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(k & 0xffffffffLL), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(k >> 32), ctrl, 0xF, 0xF, false);
+  const long long o = (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+  return o > k ? o : k;
+}
+// the maximum key of the lane's 16-lane row, in every lane of the row
+__device__ __forceinline__ long long gj_row_max(long long k) {
+  k = gj_row_max_step<8>(k);
+  k = gj_row_max_step<4>(k);
+  k = gj_row_max_step<2>(k);
+  return gj_row_max_step<1>(k);
 }
 
 constexpr int GJ_WAVES = 4;
@@ -581,9 +600,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
   // per-group slots padded so that the four groups of a wavefront start 4
   // dwords apart in the LDS banks (their broadcast reads do not conflict)
   constexpr int PROW = 2 * N + 1;  // complex entries per group slot
-  constexpr int PVAL = 2 * N + 2;  // doubles per group slot (68 dwords = 4 mod 64)
   __shared__ double2 Prow_[GJ_WAVES * 4 * PROW];
-  __shared__ double Pval_[GJ_WAVES * 4 * PVAL];
   __shared__ int Piv_[GJ_WAVES * 4 * N];  // per group: the pivot row of each column
   zd* Abuf = reinterpret_cast<zd*>(Abuf_);
   zd* Vbuf = reinterpret_cast<zd*>(Vbuf_);
@@ -598,7 +615,6 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
   const int sys = g & 1;                 // 0: S system, 1: C system
   zd* A = Abuf + mloc * NN;
   zd* prow = reinterpret_cast<zd*>(Prow_) + (wave * 4 + g) * PROW;
-  double* pval = Pval_ + (wave * 4 + g) * PVAL;
   int* piv = Piv_ + (wave * 4 + g) * N;
   const double a1 = mode == 2 ? -0.5 : 1.5;  // coefficient of S X_s
   const double a2 = mode == 0 ? 0.5 : -0.5;  // coefficient of C X_c
@@ -679,27 +695,19 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
     // inverse once column k is eliminated.  Rows stay unnormalised: the
     // pivot lane publishes its row as is and keeps 1 in the pivot column,
     // the others form the multiplier from the published pivot, and each
-    // lane divides its row by its own pivot once at the end.  The
-    // candidates of column k + 1 are published right after column k + 1 of
-    // the row is updated, so their LDS round trip overlaps the rest of the
-    // elimination.  At the end the lane that was pivot of column k holds row
+    // lane divides its row by its own pivot once at the end.  A lane's
+    // candidate for column k + 1 is formed right after column k + 1 of its
+    // row is updated.  At the end the lane that was pivot of column k holds row
     // k of M^-1, entry j in column piv[j] (the pivot row of column j).
     bool used = false;
     int xrow = 0;  // the row of M^-1 this lane holds at the end
     bool singular = false;
     zd dpiv(1.0, 0);
-    pval[r] = gj_key(abs1(mrow[0]), r);
+    long long key = gj_key(abs1(mrow[0]), r);  // my candidate for the next column
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       wave_sync();
-      double kv[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) kv[i] = pval[i];
-#pragma unroll
-      for (int w = N / 2; w >= 1; w /= 2)
-#pragma unroll
-        for (int i = 0; i < w; ++i) kv[i] = fmax(kv[i], kv[i + w]);
-      const long long kb = __double_as_longlong(kv[0]);
+      const long long kb = gj_row_max(key);
       const int pr = 15 - (int)(kb & 0xF);
       const double best = __longlong_as_double(kb & ~0xFLL);
       singular = singular || !(best > 0.0 && best < 1e300);  // zero, or a clamped inf / NaN
@@ -717,14 +725,14 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
         const zd f = mrow[k] * crecip(prow[k]);
         if (k + 1 < N) {
           csub_mul(mrow[k + 1], f, prow[k + 1]);
-          pval[r] = gj_key(used ? -1.0 : abs1(mrow[k + 1]), r);
+          key = gj_key(used ? -1.0 : abs1(mrow[k + 1]), r);
         }
 #pragma unroll
         for (int j = 0; j < N; ++j)
           if (j != k && j != k + 1) csub_mul(mrow[j], f, prow[j]);
         mrow[k] = zd(0, 0) - f;
       } else if (k + 1 < N) {
-        pval[r] = gj_key(-1.0, r);
+        key = gj_key(-1.0, r);
       }
     }
     {
