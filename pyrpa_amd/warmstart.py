@@ -32,9 +32,12 @@ def coarse_mesh(st, factor=2, nw_coarse=None):
     return cut(st['Nx']), cut(st['Ny']), cut(st['Nz']), nw
 
 
-def coarse_seed(st, factor=2, nw_coarse=None, eps=1e-5, precision=1, scf_loop=60):
+def coarse_seed(st, factor=2, nw_coarse=None, eps=1e-3, precision=1, scf_loop=60):
     """Solve FLEX on the coarse mesh and re-grid Sigma onto st's mesh.
-    Returns ((sigma, mu, mu_old), info)."""
+    Returns ((sigma, mu, mu_old), info).  The seed's distance from the fine
+    solution is the coarse mesh's discretisation error (~1e-2 at factor 2),
+    so the coarse solve stops at eps = 1e-3, an order below it: tighter
+    tolerances cost coarse iterations without moving the seed."""
     Nxc, Nyc, Nzc, Nwc = coarse_mesh(st, factor, nw_coarse)
     t0 = time.perf_counter()
     sc = L.build(st['rvec'], st['ham_r'], Nxc, Nyc, Nzc, Nwc, st['temp'], st['U'], st['J'], st['fill'], mu=st['mu'])

@@ -2,7 +2,9 @@
 what the mathematics fixes: Fourier interpolation reproduces a
 band-limited Sigma(k) exactly, the tail fit reproduces an exact
 c0 + c1/(i w) tail, and a Sigma on the same mesh comes back unchanged
-(up to the dropped outermost frequency, which the tail replaces)."""
+(up to the dropped outermost frequency, which the tail replaces).  End to
+end, the k-coarse seed brings the mixed-precision solve to the FP64 fixed
+point."""
 import numpy as np
 import pytest
 
@@ -98,3 +100,36 @@ def test_same_mesh_identity(F):
     for a in range(2):
         ref[a, a] = ref[a, a].real + 1j * np.minimum(ref[a, a].imag, 0.0)
     assert np.abs(out - ref).max() < 1e-12
+
+
+def test_coarse_seed_converges_to_the_same_fixed_point(F):
+    """The k-coarse seed of the benchmark's mixed arm on a reduced 4-orbital
+    3D workload (16x16x8, 256 Matsubara; coarse solve 8x8x4 to 1e-3): the
+    seed lies within a few per cent of the converged FP64 Sigma, and the
+    mixed-precision solve started from it reaches the FP64 fixed point
+    (tolerance 1e-8 on both) in no more iterations than the cold FP64 solve."""
+    from pyrpa_amd import lattice as L
+    from pyrpa_amd import warmstart as W
+
+    st = L.config(3, Nx=16, Ny=16, Nz=8, Nw=256)
+
+    def solve(precision, seed=None):
+        F.set_precision(precision)
+        if seed is not None:
+            F.set_sigma_seed(*seed)
+        sig, mu = L.flex(st, eps=1e-8, pp=0.5, m_diis=1, scf_loop=200, sw_tail=False, sw_rescale=True,
+                         sub_sigma=1, sw_in=seed is not None)
+        return sig, mu, F.flex_stats()
+
+    try:
+        sig64, mu64, s64 = solve(0)
+        seed, info = W.coarse_seed(st)
+        assert info["coarse_iterations"] > 0
+        seed_err = np.abs(seed[0] - sig64).max() / np.abs(sig64).max()
+        assert seed_err < 5e-2, seed_err
+        sigm, mum, sm = solve(1, seed)
+    finally:
+        F.set_precision(0)
+    assert sm["iterations"] <= s64["iterations"], (sm["iterations"], s64["iterations"])
+    assert np.abs(sigm - sig64).max() / np.abs(sig64).max() < 1e-6
+    assert abs(mum - mu64) < 1e-8
