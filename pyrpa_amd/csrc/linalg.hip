@@ -928,7 +928,9 @@ void rpa_solve(zd* out, BatchLayout Lo, const zd* A, BatchLayout La, long batch,
 // Complex general eigenvalues: Gaussian reduction to Hessenberg form with
 // partial pivoting, then single-shift QR (Wilkinson shift, exceptional shift
 // every 10 sweeps) with Givens rotations restricted to the active block.
-// Column-major H (ld = n).  Returns 0 on success.
+// Column-major H (ld = n); element (i, j) at H[(i + n j) st] and eigenvalue i
+// at w[i st], so a batch of matrices can be interleaved (st = batch size)
+// and a wavefront's lanes read adjacent elements.  Returns 0 on success.
 __host__ __device__ inline zd zsqrt_(zd a) {
   double r = hypot(a.re, a.im);
   if (r == 0.0) return zd(0, 0);
@@ -938,8 +940,9 @@ __host__ __device__ inline zd zsqrt_(zd a) {
   return zd(fabs(y), a.im >= 0 ? x : -x);
 }
 
-__host__ __device__ int zhess_eig(zd* H, int n, zd* w) {
-#define HH(i, j) H[(i) + n * (j)]
+__host__ __device__ int zhess_eig(zd* H, int n, zd* w, long st = 1) {
+#define HH(i, j) H[((i) + n * (j)) * st]
+#define WW(i) w[(i) * st]
   // Hessenberg reduction (elementary stabilised transformations)
   for (int m = 1; m < n - 1; ++m) {
     int piv = m;
@@ -971,14 +974,14 @@ __host__ __device__ int zhess_eig(zd* H, int n, zd* w) {
   int hi = n - 1;
   int iter = 0, total = 0;
   while (hi >= 0) {
-    if (hi == 0) { w[0] = HH(0, 0); break; }
+    if (hi == 0) { WW(0) = HH(0, 0); break; }
     int l = hi;
     for (; l > 0; --l) {
       double sref = abs1(HH(l - 1, l - 1)) + abs1(HH(l, l));
       if (sref == 0.0) sref = anorm;
       if (abs1(HH(l, l - 1)) <= eps * sref) break;
     }
-    if (l == hi) { w[hi] = HH(hi, hi); --hi; iter = 0; continue; }
+    if (l == hi) { WW(hi) = HH(hi, hi); --hi; iter = 0; continue; }
     ++iter;
     if (++total > 60 * n) return 1;
     zd sigma;
@@ -1029,14 +1032,17 @@ __host__ __device__ int zhess_eig(zd* H, int n, zd* w) {
     for (int k = l; k <= hi; ++k) HH(k, k) += sigma;
   }
   return 0;
+#undef WW
 #undef HH
 }
 
 __global__ void k_max_re_eig(double* out, const zd* A, BatchLayout La, long batch, int n, const double* W,
                              double alpha, zd* scratch, int* err) {
+  // the matrices are interleaved in scratch (matrix b's element e at
+  // scratch[e batch + b]): the lanes of a wavefront touch adjacent elements
   for (long b = blockIdx.x * (long)blockDim.x + threadIdx.x; b < batch; b += (long)gridDim.x * blockDim.x) {
-    zd* H = scratch + b * (long)(n * n + n);
-    zd* w = H + n * n;
+    zd* H = scratch + b;
+    zd* w = scratch + (long)n * n * batch + b;
     const zd* base = A + b * La.bs;
     for (int q = 0; q < n; ++q)
       for (int p = 0; p < n; ++p) {
@@ -1046,11 +1052,11 @@ __global__ void k_max_re_eig(double* out, const zd* A, BatchLayout La, long batc
         } else {
           v = base[(p + n * q) * La.es];
         }
-        H[p + n * q] = v * alpha;
+        H[(p + n * q) * batch] = v * alpha;
       }
-    if (zhess_eig(H, n, w)) atomicOr(err, 2);
+    if (zhess_eig(H, n, w, batch)) atomicOr(err, 2);
     double m = -HUGE_VAL;
-    for (int i = 0; i < n; ++i) m = fmax(m, w[i].re);
+    for (int i = 0; i < n; ++i) m = fmax(m, w[i * batch].re);
     out[b] = m;
   }
 }
