@@ -1117,8 +1117,13 @@ __device__ inline bool gj_inverse(zd (&a)[NO][NO]) {
   return true;
 }
 
+// The per-(k, w) register kernels below declare the block size they are
+// launched with (128 threads; 256 for the filling reductions): the bound
+// lets the compiler keep their No x No complex matrices in registers
+// (without it, it plans for 1024-thread blocks, caps a thread at 128
+// registers and spills the matrices to scratch memory).
 template <int NO>
-__global__ void k_dyson(zd* G, const zd* sig, const zd* H, double mu, double temp, long Nk, int Nw, int* err) {
+__global__ void __launch_bounds__(128) k_dyson(zd* G, const zd* sig, const zd* H, double mu, double temp, long Nk, int Nw, int* err) {
   const long tot = Nk * (long)Nw;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
     long k = e % Nk;
@@ -1142,7 +1147,7 @@ __global__ void k_dyson(zd* G, const zd* sig, const zd* H, double mu, double tem
 }
 
 template <int NO>
-__global__ void k_filling(const zd* sig, const zd* H, const double* eig, double rmu, double temp, long Nk, int Nw,
+__global__ void __launch_bounds__(256) k_filling(const zd* sig, const zd* H, const double* eig, double rmu, double temp, long Nk, int Nw,
                           double* part, int* err) {
   __shared__ double sh[2][256];
   const long tot = Nk * (long)Nw;
@@ -1192,7 +1197,7 @@ __global__ void k_filling(const zd* sig, const zd* H, const double* eig, double 
 // coef(e, i-1) = a_i.  Tr (z - M)^-1 = p'(z) / p(z) then serves every trial
 // mu of the chemical-potential search at the cost of a Horner evaluation.
 template <int NO>
-__global__ void k_charpoly(zd* coef, const zd* sig, const zd* H, long Nk, int Nw) {
+__global__ void __launch_bounds__(128) k_charpoly(zd* coef, const zd* sig, const zd* H, long Nk, int Nw) {
   const long tot = Nk * (long)Nw;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
     const long k = e % Nk;
@@ -1235,7 +1240,7 @@ __global__ void k_charpoly(zd* coef, const zd* sig, const zd* H, long Nk, int Nw
 
 // n(mu) pieces from the characteristic polynomials (see k_charpoly)
 template <int NO>
-__global__ void k_filling_cp(const zd* coef, const double* eig, double rmu, double temp, long Nk, int Nw,
+__global__ void __launch_bounds__(256) k_filling_cp(const zd* coef, const double* eig, double rmu, double temp, long Nk, int Nw,
                              double* part) {
   __shared__ double sh[2][256];
   const long tot = Nk * (long)Nw;
@@ -1295,7 +1300,7 @@ __global__ void k_green0(zd* G, const double* eig, const zd* U, double mu, doubl
 }
 
 template <int NO>
-__global__ void k_anom_f(zd* F, const zd* G, const zd* D, long Nk, int Nw) {
+__global__ void __launch_bounds__(128) k_anom_f(zd* F, const zd* G, const zd* D, long Nk, int Nw) {
   const long tot = Nk * (long)Nw;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
     zd g[NO][NO], gd[NO][NO];
@@ -1457,7 +1462,7 @@ void subtract_static(zd* sigma, long Nk, int Nw, int No, int mode, hipStream_t s
 }
 
 template <int NO>
-__global__ void k_inv(zd* A, long tot, int* err) {
+__global__ void __launch_bounds__(128) k_inv(zd* A, long tot, int* err) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
     zd a[NO][NO];
 #pragma unroll
