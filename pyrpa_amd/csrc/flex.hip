@@ -170,8 +170,9 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
   DBuf<zd> G(nS), sig(nS), sig0(nS), sd(nS), chi(nC);
   DBuf<int> err(1);
   err.zero(s);
-  std::vector<DBuf<zd>> xout(A.m_diis), res(A.m_diis);
-  for (long i = 0; i < A.m_diis; ++i) { xout[i].alloc(nS); res[i].alloc(nS); }
+  std::vector<DBuf<zd>> xout(A.m_diis), res(A.m_diis);  // DIIS history (unused for m_diis = 1)
+  if (A.m_diis > 1)
+    for (long i = 0; i < A.m_diis; ++i) { xout[i].alloc(nS); res[i].alloc(nS); }
   std::unique_ptr<ConvEngine<R>> eng(new ConvEngine<R>(mesh, op));
   std::unique_ptr<ConvEngine<float>> e32;
   // mixed-precision state: FP64 accumulators and the previous iterate
@@ -416,34 +417,44 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
     }
     if (A.sub_sigma > 0) subtract_static(sig.p, Nk, (int)Nw, No, (int)A.sub_sigma, s);
     // ---- compare_sigma
-    i_hist = i_hist % m + 1;
-    n_hist = std::min(n_hist + 1, m);
-    copy(xout[i_hist - 1].p, sig.p, nS, s);
-    lincomb(res[i_hist - 1].p, sig.p, 1.0, sig0.p, -1.0, nS, s);
-    const int n = n_hist;
-    std::vector<int> idx(n);
-    for (int ih = 1; ih <= n; ++ih) idx[ih - 1] = ((i_hist - n + ih - 1) % m + m) % m;
-    std::vector<double> B((n + 1) * (n + 1), 0.0), c(n + 1, 0.0);
-    for (int a = 0; a < n; ++a) {
-      for (int b = a; b < n; ++b) {
-        double v = dot_w(res[idx[a]].p, res[idx[b]].p, nullptr, (long)nS, 1, s).re;
-        B[a + (n + 1) * b] = B[b + (n + 1) * a] = v;
+    if (m == 1) {
+      // one history entry: the DIIS system [[r.r + 1e-12, 1], [1, 0]] c =
+      // [0, 1] gives c0 = 1 exactly (either pivot order divides a number by
+      // itself or by 1), so the extrapolated Sigma is Sigma itself and the
+      // step is the linear mixing; the same values without the history
+      // copy, the residual, its norm and the scaling
+      esterr = max_rel_err(sig0.p, sig.p, nS, 1.0e-10, s, nullptr);
+      lincomb(sig.p, sig.p, A.pp, sig0.p, 1.0 - A.pp, nS, s);
+    } else {
+      i_hist = i_hist % m + 1;
+      n_hist = std::min(n_hist + 1, m);
+      copy(xout[i_hist - 1].p, sig.p, nS, s);
+      lincomb(res[i_hist - 1].p, sig.p, 1.0, sig0.p, -1.0, nS, s);
+      const int n = n_hist;
+      std::vector<int> idx(n);
+      for (int ih = 1; ih <= n; ++ih) idx[ih - 1] = ((i_hist - n + ih - 1) % m + m) % m;
+      std::vector<double> B((n + 1) * (n + 1), 0.0), c(n + 1, 0.0);
+      for (int a = 0; a < n; ++a) {
+        for (int b = a; b < n; ++b) {
+          double v = dot_w(res[idx[a]].p, res[idx[b]].p, nullptr, (long)nS, 1, s).re;
+          B[a + (n + 1) * b] = B[b + (n + 1) * a] = v;
+        }
+        B[a + (n + 1) * n] = B[n + (n + 1) * a] = 1.0;
       }
-      B[a + (n + 1) * n] = B[n + (n + 1) * a] = 1.0;
+      for (int a = 0; a < n; ++a) B[a + (n + 1) * a] += 1.0e-12;
+      c[n] = 1.0;
+      if (!host_dgesv(B, n + 1, c)) {
+        if (verbose) printf("DIIS: dgesv failed, fallback to most-recent entry\n");
+        std::fill(c.begin(), c.end(), 0.0);
+        c[n - 1] = 1.0;
+      }
+      copy(sd.p, xout[idx[0]].p, nS, s);
+      scale(sd.p, nS, c[0], s);
+      for (int a = 1; a < n; ++a) axpy(sd.p, xout[idx[a]].p, zd(c[a], 0), nS, s);
+      esterr = max_rel_err(sig0.p, sd.p, nS, 1.0e-10, s, nullptr);
+      if (n >= 2) copy(sig.p, sd.p, nS, s);
+      else lincomb(sig.p, sd.p, A.pp, sig0.p, 1.0 - A.pp, nS, s);
     }
-    for (int a = 0; a < n; ++a) B[a + (n + 1) * a] += 1.0e-12;
-    c[n] = 1.0;
-    if (!host_dgesv(B, n + 1, c)) {
-      if (verbose) printf("DIIS: dgesv failed, fallback to most-recent entry\n");
-      std::fill(c.begin(), c.end(), 0.0);
-      c[n - 1] = 1.0;
-    }
-    copy(sd.p, xout[idx[0]].p, nS, s);
-    scale(sd.p, nS, c[0], s);
-    for (int a = 1; a < n; ++a) axpy(sd.p, xout[idx[a]].p, zd(c[a], 0), nS, s);
-    esterr = max_rel_err(sig0.p, sd.p, nS, 1.0e-10, s, nullptr);
-    if (n >= 2) copy(sig.p, sd.p, nS, s);
-    else lincomb(sig.p, sd.p, A.pp, sig0.p, 1.0 - A.pp, nS, s);
     t_sig += secs(t0);
     if (verbose) printf("esterr=%9.2E\n", esterr);
     if (esterr < A.eps) break;
