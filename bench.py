@@ -46,6 +46,9 @@ if ROOT not in sys.path:
 
 METRIC = "time-to-converged FLEX Σ (tol 1e-8): FP32+warm-start vs plain FP64, same GPU"
 HBM_PEAK_GBS = 8000.0
+# both arms compute chi0 with the FP64 Matsubara tail correction
+# (libs/src/fchi.f90:852-892; sw_tail as main.py:165 sets it)
+SW_TAIL = True
 # HBM bytes per FP32 FFT-pass launch (box_fft_kernel and box_accum_kernel) on
 # the 4-orbital 32x32x16 x 1024-Matsubara workload, averaged over all its FP32
 # pass dispatches: rocprofv3 FETCH_SIZE x 2
@@ -72,7 +75,7 @@ def solve(F, L, st, precision, eps, scf_loop=300, seed=None, profile=False):
     if profile:
         F.profile(True)
     t0 = time.perf_counter()
-    sig, mu = L.flex(st, eps=eps, pp=0.5, m_diis=1, scf_loop=scf_loop, sw_tail=False, sw_rescale=True,
+    sig, mu = L.flex(st, eps=eps, pp=0.5, m_diis=1, scf_loop=scf_loop, sw_tail=SW_TAIL, sw_rescale=True,
                      sub_sigma=1, sw_in=sw_in)
     wall = time.perf_counter() - t0
     prof = None
@@ -150,7 +153,7 @@ def main():
     for w in range(args.warmup):
         solve(F, L, st, 0, 0.0, scf_loop=2)
         solve(F, L, st, 1, 0.0, scf_loop=3)
-        W.warm_start(st, args.warm)
+        W.warm_start(st, args.warm, sw_tail=SW_TAIL)
         log(f"warmup {w + 1}/{args.warmup} done")
 
     steps = []
@@ -159,7 +162,7 @@ def main():
         log(f"step {k + 1}: plain FP64 cold: {int(s64['iterations'])} iterations, {s64['solve_seconds']:.2f} s "
             f"(wall {s64['wall_seconds']:.2f} s), esterr {s64['esterr']:.2e}")
         tw = time.perf_counter()
-        seed, winfo = W.warm_start(st, args.warm)
+        seed, winfo = W.warm_start(st, args.warm, sw_tail=SW_TAIL)
         t_warm = time.perf_counter() - tw
         if seed is not None:
             winfo["seed_rel_err_vs_converged"] = rel(seed[0], sig64)
@@ -211,7 +214,8 @@ def main():
         "data": "synthetic (seeded lattice)",
         "config": {"workload": f"FLEX {st['Norb']}-orbital {st['Nx']}x{st['Ny']}x{st['Nz']} k-mesh x "
                                f"{st['Nw']} Matsubara, T={st['temp']}, U={st['U']}, J={st['J']}, "
-                               f"rfill={st['fill']}, tol {args.eps:g}, linear mixing 0.5",
+                               f"rfill={st['fill']}, tol {args.eps:g}, linear mixing 0.5, "
+                               f"{'tail-corrected chi0 (FP64 tail)' if SW_TAIL else 'sharp-cutoff chi0'}",
                    "warm_start": best["winfo"]["kind"], "parallelism": "single GPU"},
         "t_plain_fp64_s": round(t_plain, 3),
         "t_mixed_warm_s": round(t_mix, 3),

@@ -70,6 +70,15 @@ void get_chis_chic(gmod_z* chis, gmod_z* chic, const gmod_z* chi, const double* 
 void get_eig_or_tr_chi(gmod_z* chiqout, const gmod_z* chi, const int64_t* invk, const int64_t* Nkall,
                        const int64_t* Nk, const int64_t* Nchi, const bool* sw_eig);
 
+/* Addition (no single reference entry point; the reference offers chi_s at
+ * one q for all nu, get_chis, and at nu = 0 for all q, get_chis_chic): chi_s =
+ * (I - chi0 S)^-1 chi0 and chi_c = (I + chi0 C)^-1 chi0 at every (q, i nu),
+ * the algebra of libs/src/fchi.f90:319-350 and fself.f90:783-816 applied to
+ * the whole batch.  chi, chis, chic: (Nk,Nw,Nchi,Nchi) like get_chi0's
+ * output; chis or chic may be NULL to skip that susceptibility. */
+void gmod_get_chis_chic_all(gmod_z* chis, gmod_z* chic, const gmod_z* chi, const double* Smat, const double* Cmat,
+                            const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi);
+
 /* ---------------- Green functions ---------------------------------------- */
 /* libs/src/fself.f90:1-41 (bind name gen_green0_) [libs/flibs/_green.py:80-97] */
 void gen_green0_(gmod_z* Gk, const double* eig, const gmod_z* uni, const double* mu, const double* temp,
@@ -187,6 +196,28 @@ int gmod_box_fft_host(double* data, int nx, int ny, int nz, int m, int mask, int
  * for gmod_box_fft_host.  Returns 0, or 1 on an error (gmod_last_error). */
 int gmod_fft_device(void* buf, void* tmp, int nx, int ny, int nz, int m, int mask, int sign, int precision,
                     int reps);
+/* ---------------- learned warm start: the surrogate network --------------
+ * No reference counterpart.  A fully connected network (SiLU hidden layers,
+ * linear output) whose GEMMs run on the matrix cores with BF16 operands and
+ * FP32 accumulation; FP32 master weights and Adam moments.  Handles are
+ * small non-negative integers; functions returning a value return -1 on an
+ * error (gmod_last_error), the others record it only.
+ * Flat parameter / gradient order: W0 (dims[0] x dims[1], row-major), b0,
+ * W1, b1, ...  X: (B, dims[0]) row-major FP32; Y, T: (B, dims[nlayers]). */
+int64_t gmod_nn_create(int64_t nlayers, const int64_t* dims, uint64_t seed);
+void gmod_nn_destroy(int64_t h);
+int64_t gmod_nn_num_params(int64_t h);
+void gmod_nn_get_params(int64_t h, float* out);
+void gmod_nn_set_params(int64_t h, const float* in);
+void gmod_nn_forward(int64_t h, const float* X, int64_t B, float* Y);
+/* mean-squared-error loss of one batch; grad (may be NULL) receives the flat gradient */
+double gmod_nn_grad(int64_t h, const float* X, const float* T, int64_t B, float* grad);
+/* training set resident on the device: n rows of X and T */
+void gmod_nn_set_data(int64_t h, const float* X, const float* T, int64_t n);
+/* one pass over order[0..n_used) (row indices into the resident set) in
+ * mini-batches of `batch`, one Adam step per batch; returns the mean batch loss */
+double gmod_nn_train_epoch(int64_t h, const int64_t* order, int64_t n_used, int64_t batch, double lr);
+
 /* Device memory held by the library's caching pool, in bytes. */
 double gmod_pool_bytes(void);
 void gmod_release_memory(void);

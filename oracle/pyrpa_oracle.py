@@ -214,6 +214,38 @@ def get_ffermi(eig, mu, temp):
     return 0.5 - 0.5 * np.tanh((np.asarray(eig) - mu) * (0.5 / t))
 
 
+def calc_mu(eig, Nk, fill, temp):
+    """mu with sum_k,n f(e - mu) = fill Nk by bisection between the band edges,
+    clamped to the closer edge when the filling lies outside the band range
+    (libs/plibs/_bands.py:50-78, which uses brentq on the same function)."""
+    e = np.asarray(eig, dtype=np.float64).ravel()
+    func = lambda mu: fill * Nk - get_ffermi(e, mu, temp).sum()
+    lo, hi = float(e.min()), float(e.max())
+    flo, fhi = func(lo), func(hi)
+    if flo * fhi > 0:
+        return lo if abs(flo) < abs(fhi) else hi
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        fm = func(mid)
+        if (fm > 0) == (flo > 0):
+            lo, flo = mid, fm
+        else:
+            hi = mid
+        if hi - lo < 1e-15 * max(1.0, abs(mid)):
+            break
+    return 0.5 * (lo + hi)
+
+
+def get_mu(ham_r, rvec, temp, fill, kmesh=40):
+    """Initial mu on a kmesh^3 full mesh (libs/plibs/_calc.py:298-309 with
+    get_emesh, _bands.py:325-344)."""
+    g = np.arange(kmesh) / kmesh
+    kx, ky, kz = np.meshgrid(g, g, g, indexing='ij')
+    kl = np.stack([kx.ravel(), ky.ravel(), kz.ravel()], axis=1)
+    eig, _ = get_eig(gen_ham(kl, ham_r, rvec))
+    return calc_mu(eig, len(kl), fill, temp)
+
+
 def gen_Green0(eig, uni, mu, temp, Nw):
     """G0_{ml}(k,iw_j) = sum_n U_mn U*_ln / (i w_j + mu - e_n), w_j=(2j-1) pi T
     (fself.f90:1-41).  Returns Python Gk[No,No,Nw,Nk]."""
@@ -631,6 +663,20 @@ def get_chis_chic(chi, Smat, Cmat):
     return np.ascontiguousarray(chis.T), np.ascontiguousarray(chic.T)
 
 
+def get_chis_chic_all(chi, Smat, Cmat):
+    """chi_s = (I - chi0 S)^-1 chi0 and chi_c = (I + chi0 C)^-1 chi0 at every
+    (q, i nu): the algebra of get_chis (fchi.f90:319-350) and get_chis_chic
+    (fself.f90:783-816) over the whole batch, on the Fortran matrices
+    chi(k,w,:,:).  chi: Python [Nchi,Nchi,Nw,Nk]; returns the same shape twice."""
+    A = np.asarray(chi).T                      # (Nk,Nw,Nchi,Nchi) = Fortran chi(k,w,:,:)
+    S = np.asarray(Smat).T
+    C = np.asarray(Cmat).T
+    eye = np.eye(S.shape[0])[None, None]
+    chis = np.linalg.solve(eye - A @ S, A)
+    chic = np.linalg.solve(eye + A @ C, A)
+    return np.ascontiguousarray(chis.T), np.ascontiguousarray(chic.T)
+
+
 def get_eig_or_tr_chi(chi, invk, sw_eig):
     """Leading Re eigenvalue or trace per full-BZ k (fself.f90:818-850)."""
     chif = np.asarray(chi).T                       # (Nk,Nchi,Nchi)
@@ -951,6 +997,8 @@ def mkself(Smat, Cmat, kmap, invk, olist, hamk, eig, uni, mu, fill, temp, Nw, Nx
         info['mu'].append(mu)
         Gf = np.asarray(gen_green(np.ascontiguousarray(sigf.T), hamk, mu, temp)).T
         sig0f = sigf.copy()
+    # sigma.bin is written before the final renew_mu (fself.f90:443-446)
+    info['mu_file'], info['mu_old_file'] = mu, mu_old
     mu, mu_old = fl.renew_mu(mu, mu_old, sigf)
     out = np.ascontiguousarray(sigf.T)
     if return_info:
@@ -969,7 +1017,7 @@ def gap_symms(klist, Norb, gap_sym):
     rows = {0: np.ones(len(klist)), 1: np.cos(A * kx) - np.cos(A * ky),
             2: 2 * np.cos(A * kx) * np.cos(A * ky), 3: 2 * np.sin(A * kx) * np.sin(A * ky),
             4: 2 * np.sin(A * kx) * np.sin(A * kz), 5: 2 * np.sin(A * ky) * np.sin(A * kz),
-            -1: 2 * np.sin(A * kx), -2: 2 * np.sin(A * ky)}
+            -1: 2 * np.sin(A * kx), -2: 2 * np.sin(A * ky), -3: 2 * np.sin(A * kx) + 2j * np.sin(A * ky)}
     row = rows.get(gap_sym, np.zeros(len(klist)))
     return np.tile(row, (Norb, 1))
 

@@ -28,6 +28,15 @@ class ConvEngine {
   // the G0 bubble plus the analytic tau reference).  Thresholds at 1e-9.
   void chi0(zd* chi, double temp, bool tail, const double* eig_d, const zd* uni_d, const double* eig_h,
             const zd* uni_h, const int64_t* invk, double mu, hipStream_t s, bool apply_threshold = true);
+  // The tail-corrected chi0 of an FP32 engine: conv[G] - conv[G0] at grid
+  // precision, the analytic tau reference and its sawtooth term through an
+  // FP64 engine (tau_reference), combined in FP64 in (q, i nu)
+  void chi0_split_tail(zd* chi, double temp, const double* eig_d, const zd* uni_d, const double* eig_h,
+                       const zd* uni_h, const int64_t* invk, double mu, hipStream_t s);
+  // chi(pair) += -(1/(T Nkall)) ref[G0](pair) + J(pair) sawc for every
+  // irreducible pair (no partner copies), at this engine's precision
+  void tau_reference(zd* chi, double temp, const double* eig_d, const zd* uni_d, double mu, const std::vector<zd>& J,
+                     const zd* sawc_d, hipStream_t s);
   // sigma(Nk,Nw,No,No) <- (T/Nkall) sum_pairs V (*) G, using Fg; thresholds at 1e-9
   void sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s,
              bool apply_threshold = true);

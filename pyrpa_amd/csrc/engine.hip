@@ -181,6 +181,12 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
   const long Nk = mesh.Nk, Nw = mesh.Nw, M = mesh.M, Nkall = mesh.Nkall;
   const long NkNw = Nk * Nw;
   const double w = temp / (double)Nkall;
+  if (tail && sizeof(R) < sizeof(double)) {
+    // the tail subtraction stays FP64 when the transforms are FP32
+    chi0_split_tail(chi, temp, eig_d, uni_d, eig_h, uni_h, invk, mu, s);
+    if (apply_threshold) threshold(chi, NkNw * Nchi * Nchi, 1e-9, s);
+    return;
+  }
   std::vector<zd> rb;
   DBuf<zd> sawc_d;
   if (tail) {
@@ -258,6 +264,97 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
     if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
   if (apply_threshold) threshold(chi, NkNw * Nchi * Nchi, 1e-9, s);
+}
+
+template <typename R>
+void ConvEngine<R>::tau_reference(zd* chi, double temp, const double* eig_d, const zd* uni_d, double mu,
+                                  const std::vector<zd>& J, const zd* sawc_d, hipStream_t s) {
+  const int No = op.No, Nchi = op.Nchi;
+  const long NkNw = mesh.Nk * (long)mesh.Nw;
+  if (TA.n != (size_t)(No * No) * N) { TA.alloc((size_t)No * No * N); TB.alloc((size_t)No * No * N); }
+  const MeshDev md = mesh_dev(mesh);
+  const double beta = 1.0 / temp, dtau = beta / (double)mesh.M;
+  for (int b = 0; b < No; ++b)
+    for (int a = 0; a < No; ++a)
+      for (int which = 0; which < 2; ++which) {
+        cpx<R>* g = grid(which == 0 ? TA : TB, a, b);
+        p3->exec_fused(TauRefLoad<R>{eig_d, uni_d, a, b, No, which, mu, beta, dtau, md},
+                       GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
+      }
+  const double wref = 1.0 / (temp * (double)mesh.Nkall);
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
+  for (auto& lm : op.irr) {
+    const int l = lm[0], m = lm[1];
+    const int a13 = op.ol[l][0], b13 = op.ol[m][0], a42 = op.ol[m][1], b42 = op.ol[l][1];
+    ProductLoad<R> pl{};
+    pl.m = md;
+    pl.add(grid(TA, a13, b13), grid(TB, a42, b42), (R)(-wref), 0);
+    zd* dst = chi + NkNw * (m + (long)Nchi * l);
+    p4->exec_fused(pl.template fixed<1>(), gather_sf(dst, 1.0, J[l + Nchi * m], sawc_d, 1, s), work.p, tmp.p, +1,
+                   invN, s, true);
+  }
+}
+
+template <typename R>
+void ConvEngine<R>::chi0_split_tail(zd* chi, double temp, const double* eig_d, const zd* uni_d, const double* eig_h,
+                                    const zd* uni_h, const int64_t* invk, double mu, hipStream_t s) {
+  const int No = op.No, Nchi = op.Nchi;
+  const long Nk = mesh.Nk, Nw = mesh.Nw, NkNw = Nk * Nw;
+  const double w = temp / (double)mesh.Nkall;
+  {
+    DBuf<zd> G0((size_t)NkNw * No * No);
+    green0(G0.p, eig_d, uni_d, mu, temp, Nk, (int)Nw, No, s);
+    transform_green(G0.p, true, s);
+    GMOD_HIP(hipStreamSynchronize(s));  // G0 is released at the end of the scope
+  }
+  std::vector<zd> J;
+  DBuf<zd> sawc_d;
+  tail_constants(temp, eig_h, uni_h, invk, mu, J, sawc_d, s);
+  const MeshDev md = mesh_dev(mesh);
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
+  if (!mesh.complete) GMOD_HIP(hipMemsetAsync(chi, 0, NkNw * Nchi * Nchi * sizeof(zd), s));
+  auto is_self = [&](int l, int m) {
+    auto pr = op.partner[m + Nchi * l];
+    return pr[0] == m && pr[1] == l;
+  };
+  // 1) -w conv[G] + w conv[G0] at grid precision (a self-partner pair: -w conv[G] only)
+  for (auto& lm : op.irr) {
+    const int l = lm[0], m = lm[1];
+    const int a13 = op.ol[l][0], b13 = op.ol[m][0], a42 = op.ol[m][1], b42 = op.ol[l][1];
+    ProductLoad<R> pl{};
+    pl.m = md;
+    pl.add(grid(Fg, a13, b13), grid(Fg, a42, b42), (R)(-w), 1);
+    if (!is_self(l, m)) pl.add(grid(F0g, a13, b13), grid(F0g, a42, b42), (R)w, 1);
+    zd* dst = chi + NkNw * (m + (long)Nchi * l);
+    const GatherStore<R> gst = gather_sf(dst, 1.0, zd(0, 0), nullptr, 0, s);
+    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s, true);
+    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s, true);
+  }
+  // 2) + the tau reference and the sawtooth term, FP64
+  {
+    ConvEngine<double> e64(mesh, op);
+    e64.tau_reference(chi, temp, eig_d, uni_d, mu, J, sawc_d.p, s);
+    GMOD_HIP(hipStreamSynchronize(s));  // e64's grids are released at the end of the scope
+  }
+  // 3) the reference's order for a self-partner pair, conj(conv[G] + ref) -
+  //    conv[G0] (fchi.f90:610, 846, 886); partner copies of the others
+  for (auto& lm : op.irr) {
+    const int l = lm[0], m = lm[1];
+    const int a13 = op.ol[l][0], b13 = op.ol[m][0];
+    const int a42 = op.ol[m][1], b42 = op.ol[l][1];
+    zd* dst = chi + NkNw * (m + (long)Nchi * l);
+    if (is_self(l, m)) {
+      conj_copy(dst, dst, NkNw, s);
+      ProductLoad<R> p0{};
+      p0.m = md;
+      p0.add(grid(F0g, a13, b13), grid(F0g, a42, b42), (R)w, 1);
+      p4->exec_fused(p0.template fixed<1>(), gather_sf(dst, 1.0, zd(0, 0), nullptr, 1, s), work.p, tmp.p, +1, invN,
+                     s, true);
+      continue;
+    }
+    auto pr = op.partner[m + Nchi * l];
+    if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
+  }
 }
 
 template <typename R>

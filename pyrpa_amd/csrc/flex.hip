@@ -468,6 +468,9 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
     copy(sig0.p, sig.p, nS, s);
     t_mu += secs(t0);
   }
+  // sigma.bin records the mu that built the last G and its mu_OLD: the
+  // reference writes the file before its final renew_mu (fself.f90:443-446)
+  const double mu_file = mu, mu_old_file = mu_old;
   {
     auto t0 = clk::now();
     renew_mu();
@@ -483,11 +486,12 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
   *mu_out = mu;
   if (A.sw_out) {
     std::ofstream f("sigma.bin", std::ios::binary);
-    write_record(f, &mu, 8);
-    write_record(f, &mu_old, 8);
+    write_record(f, &mu_file, 8);
+    write_record(f, &mu_old_file, 8);
     write_record(f, h.data(), nS * sizeof(zd));
     double meta[4] = {T, (double)Nw, (double)No, (double)Nk};
     write_record(f, meta, 32);
+    if (!f) throw Error("io_sigma: cannot write sigma.bin");
   }
   double st[11] = {(double)iters, esterr, stoner,  mu,       secs(t_start),  t_chi,
                    t_vert,        t_sig,  t_mu,    (double)n_inc, t_solve};
@@ -854,6 +858,41 @@ extern "C" void get_chis_chic(gmod_z* chis, gmod_z* chic, const gmod_z* chi, con
   os.download((zd*)chis, (size_t)nk * n * n, s);
   oc.download((zd*)chic, (size_t)nk * n * n, s);
   GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+// chi_s and chi_c at every (q, i nu): the algebra of get_chis (fchi.f90:319-350)
+// and get_chis_chic (fself.f90:783-816) over the whole (Nk, Nw) batch
+extern "C" void gmod_get_chis_chic_all(gmod_z* chis, gmod_z* chic, const gmod_z* chi, const double* Smat,
+                                       const double* Cmat, const int64_t* Nk, const int64_t* Nw,
+                                       const int64_t* Nchi) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int n = (int)*Nchi;
+  const long NkNw = *Nk * *Nw;
+  const size_t tot = (size_t)NkNw * n * n;
+  if (!chis && !chic) return;
+  Upload up{s};
+  DBuf<zd> a, o(tot);
+  DBuf<double> S, C;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(a, (const zd*)chi, tot);
+  if (chis) {
+    up(S, Smat, (size_t)n * n);
+    rpa_solve(o.p, BatchLayout{1, NkNw}, a.p, BatchLayout{1, NkNw}, NkNw, n, S.p, -1.0, err.p, s);
+    check_err(err, s, "gmod_get_chis_chic_all (chi_s)");
+    o.download((zd*)chis, tot, s);
+    GMOD_HIP(hipStreamSynchronize(s));
+  }
+  if (chic) {
+    up(C, Cmat, (size_t)n * n);
+    rpa_solve(o.p, BatchLayout{1, NkNw}, a.p, BatchLayout{1, NkNw}, NkNw, n, C.p, 1.0, err.p, s);
+    check_err(err, s, "gmod_get_chis_chic_all (chi_c)");
+    o.download((zd*)chic, tot, s);
+    GMOD_HIP(hipStreamSynchronize(s));
+  }
   GMOD_CATCH
 }
 

@@ -338,19 +338,10 @@ static void launch_vertex_reg(zd* chi, long batch, long es, const double* S, con
   GMOD_CHECK_LAUNCH();
 }
 
-// ---------------------------------------------------------------------- wave LU
-// One wavefront per (q, i nu) matrix for Nchi = N <= 16 with sparse vertex
-// matrices (at most NZ nonzeros per row / column, as Kanamori S and C have).
-// Lanes h*2N + c of a 4N-lane group: half h = 0 solves (I - A S) X = A S,
-// h = 1 solves (I + A C) X = A C; c < N holds column c of the matrix, c >= N
-// column c - N of the right-hand side.  The LU with partial pivoting
-// (|Re|+|Im| pivots, as LAPACK's izamax) runs in registers; pivot indices,
-// multipliers and the columns of U move between lanes by cross-lane shuffles,
-// so a wavefront never waits for another.  The sparse products (forming
-// A S, A C, and V = st + a1 S X_s + a2 C X_c - 0.25 W A W) go through the
-// wavefront's own LDS slice.  A block stages VW_MAT consecutive (q, i nu)
-// matrices through LDS so that global loads and stores are coalesced.
-constexpr int VW_MAT = 8;     // matrices (= wavefronts) per block
+// ---------------------------------------------------------------------- sparse vertex matrices
+// Kanamori S and C have at most VW_NZ nonzeros per row / column; the
+// Gauss-Jordan vertex kernel below forms its products A S, A C and W A W
+// from the sparse columns.
 constexpr int VW_NZ = 4;      // nonzeros per row / column of S, C, S + C
 
 struct VSparse {              // column j of K: rows idx[j][t], values val[j][t] (zero padded)
@@ -362,169 +353,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ zd shfl_z(zd v, int src) { return zd(__shfl(v.re, src), __shfl(v.im, src)); }
-
-template <int N>
-__global__ void __launch_bounds__(64 * VW_MAT)
-k_vertex_wave(zd* __restrict__ chi, long batch, long es, const VSparse* __restrict__ spg, const double* __restrict__ Sd,
-              const double* __restrict__ Cd, int mode, int* err) {
-  static_assert(4 * N <= 64, "one wavefront per matrix");
-  constexpr int NN = N * N;
-  __shared__ VSparse sp;
-  __shared__ double2 Abuf_[VW_MAT * NN];   // chi0 per wavefront, later X_s | X_c (first half)
-  __shared__ double2 Tbuf_[VW_MAT * NN];   // A W, later X_c
-  __shared__ double2 Vbuf_[VW_MAT * NN];   // the vertex being built
-  zd* Abuf = reinterpret_cast<zd*>(Abuf_);
-  zd* Tbuf = reinterpret_cast<zd*>(Tbuf_);
-  zd* Vbuf = reinterpret_cast<zd*>(Vbuf_);
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  {
-    const int* src = reinterpret_cast<const int*>(spg);
-    int* dst = reinterpret_cast<int*>(&sp);
-    for (int i = tid; i < (int)(sizeof(VSparse) / sizeof(int)); i += 64 * VW_MAT) dst[i] = src[i];
-  }
-  __syncthreads();
-  const int h = lane / (2 * N), c = lane % (2 * N);  // lanes >= 4N idle in the solve
-  const bool act = lane < 4 * N;
-  const bool is_mat = c < N;
-  const int j = is_mat ? c : c - N;
-  const int kk = h;  // 0: S, 1: C
-  const double a1 = mode == 2 ? -0.5 : 1.5;  // coefficient of S X_s
-  const double a2 = mode == 0 ? 0.5 : -0.5;  // coefficient of C X_c
-  zd* A = Abuf + wave * NN;
-  zd* T = Tbuf + wave * NN;
-  zd* V = Vbuf + wave * NN;
-  for (long b0 = (long)blockIdx.x * VW_MAT; b0 < batch; b0 += (long)gridDim.x * VW_MAT) {
-    // ---- coalesced load of VW_MAT consecutive matrices
-    for (int i = tid; i < VW_MAT * NN; i += 64 * VW_MAT) {
-      const int m = i % VW_MAT, e = i / VW_MAT;
-      const long b = b0 + m;
-      Abuf[m * NN + e] = b < batch ? chi[b + (long)e * es] : zd(0, 0);
-    }
-    __syncthreads();
-    const bool live = b0 + wave < batch;
-    // ---- T = A W (mode 0), then V = st - 0.25 W T  (element per lane, 4 passes)
-    if (mode == 0) {
-      for (int e = lane; e < NN; e += 64) {
-        const int p = e % N, q = e / N;
-        zd acc(0, 0);
-#pragma unroll
-        for (int t = 0; t < VW_NZ; ++t) acc += A[p + N * sp.idx[2][q][t]] * sp.val[2][q][t];
-        T[e] = acc;
-      }
-      wave_sync();
-    }
-    for (int e = lane; e < NN; e += 64) {
-      const int p = e % N, q = e / N;
-      const double sv = Sd[e], cv = Cd[e];
-      double st = mode == 0 ? 1.5 * sv - 0.5 * cv : mode == 1 ? 0.5 * (sv + cv) : 0.5 * (sv - cv);
-      zd v(st, 0);
-      if (mode == 0) {
-        zd acc(0, 0);
-        // W symmetric: row p of W = column p
-#pragma unroll
-        for (int t = 0; t < VW_NZ; ++t) acc += T[sp.idx[2][p][t] + N * q] * sp.val[2][p][t];
-        v -= acc * 0.25;
-      }
-      V[e] = v;
-    }
-    // ---- my column of A K (K = S for h = 0, C for h = 1)
-    zd col[N];
-#pragma unroll
-    for (int p = 0; p < N; ++p) col[p] = zd(0, 0);
-    if (act) {
-#pragma unroll
-      for (int t = 0; t < VW_NZ; ++t) {
-        const int r = sp.idx[kk][j][t];
-        const double kv = sp.val[kk][j][t];
-#pragma unroll
-        for (int p = 0; p < N; ++p) col[p] += A[p + N * r] * kv;
-      }
-      if (is_mat) {
-#pragma unroll
-        for (int p = 0; p < N; ++p) {
-          if (kk == 0) col[p] = zd(0, 0) - col[p];
-          if (p == j) col[p] += zd(1.0, 0);
-        }
-      }
-    }
-    // ---- LU with partial pivoting; the RHS lanes are eliminated alongside
-    const int base = (lane / (4 * N)) * 4 * N + h * 2 * N;  // first lane of my half
-    bool singular = false;
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      int pl = k;
-      double best = abs1(col[k]);
-#pragma unroll
-      for (int r = k + 1; r < N; ++r) {
-        const double v = abs1(col[r]);
-        if (v > best) { best = v; pl = r; }
-      }
-      const int piv = __shfl(pl, base + k);
-      const double pbest = __shfl(best, base + k);
-      if (pbest == 0.0) singular = true;
-#pragma unroll
-      for (int r = k + 1; r < N; ++r)
-        if (r == piv) { const zd t = col[k]; col[k] = col[r]; col[r] = t; }
-      // pivot lane: multipliers below the diagonal
-      if (c == k) {
-        const zd d = col[k];
-        const zd inv = (d.re == 0.0 && d.im == 0.0) ? zd(0, 0) : zd(1.0, 0) / d;
-#pragma unroll
-        for (int r = k + 1; r < N; ++r) col[r] = col[r] * inv;
-      }
-      const zd ck = col[k];
-#pragma unroll
-      for (int r = k + 1; r < N; ++r) {
-        const zd m = shfl_z(col[r], base + k);
-        if (c > k) col[r] -= m * ck;
-      }
-    }
-    if (singular && act && lane == base && live) atomicOr(err, 1);
-    // ---- back substitution on the RHS lanes with U (columns from the matrix lanes)
-#pragma unroll
-    for (int k = N - 1; k >= 0; --k) {
-      const zd ukk = shfl_z(col[k], base + k);
-      zd xk(0, 0);
-      if (!is_mat) {
-        xk = (ukk.re == 0.0 && ukk.im == 0.0) ? zd(0, 0) : col[k] / ukk;
-        col[k] = xk;
-      }
-#pragma unroll
-      for (int r = 0; r < k; ++r) {
-        const zd u = shfl_z(col[r], base + k);
-        if (!is_mat) col[r] -= u * xk;
-      }
-    }
-    wave_sync();  // all reads of A (forming) and T are done
-    // ---- X_s -> A, X_c -> T (column j of the half's solution)
-    if (act && !is_mat) {
-      zd* X = kk == 0 ? A : T;
-#pragma unroll
-      for (int p = 0; p < N; ++p) X[p + N * j] = col[p];
-    }
-    wave_sync();
-    // ---- V += a1 S X_s + a2 C X_c
-    for (int e = lane; e < NN; e += 64) {
-      const int p = e % N, q = e / N;
-      zd acc1(0, 0), acc2(0, 0);
-#pragma unroll
-      for (int t = 0; t < VW_NZ; ++t) {
-        acc1 += A[sp.idx[0][p][t] + N * q] * sp.val[0][p][t];
-        acc2 += T[sp.idx[1][p][t] + N * q] * sp.val[1][p][t];
-      }
-      V[e] += acc1 * a1 + acc2 * a2;
-    }
-    __syncthreads();
-    for (int i = tid; i < VW_MAT * NN; i += 64 * VW_MAT) {
-      const int m = i % VW_MAT, e = i / VW_MAT;
-      const long b = b0 + m;
-      if (b < batch) chi[b + (long)e * es] = Vbuf[m * NN + e];
-    }
-    __syncthreads();
-  }
 }
 
 // Gauss-Jordan variant, one matrix row per lane: a 16-lane group solves one
@@ -817,9 +645,7 @@ static void check_lds(size_t bytes) {
 // when it does not apply (the caller uses the general kernels)
 static bool vertex_gj(const zd* in, zd* out, long batch, BatchLayout L, int n, const double* S, const double* C,
                       int mode, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s) {
-  if (!(L.bs == 1 && n == 16) || getenv("GMOD_VERTEX_LDS") || getenv("GMOD_VERTEX_REG") ||
-      getenv("GMOD_VERTEX_WAVE"))
-    return false;
+  if (!(L.bs == 1 && n == 16)) return false;
   VSparse sp;
   if (!vertex_sparse(S, C, n, sp, s)) return false;
   DBuf<VSparse> spd(1);
@@ -849,21 +675,7 @@ void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, cons
                 hipStream_t s) {
   if (batch <= 0) return;
   if (vertex_gj(chi, chi, batch, L, n, S, C, mode, 0.0, nullptr, nullptr, err, s)) return;
-  if (L.bs == 1 && n == 16 && !getenv("GMOD_VERTEX_LDS") && !getenv("GMOD_VERTEX_REG")) {
-    VSparse sp;
-    if (vertex_sparse(S, C, n, sp, s)) {
-      DBuf<VSparse> spd(1);
-      spd.upload(&sp, 1, s);
-      const long nb = (batch + VW_MAT - 1) / VW_MAT;
-      const unsigned grid = (unsigned)std::min<long>(nb, 256L * 16);
-      hipLaunchKernelGGL(k_vertex_wave<16>, dim3(grid), dim3(64 * VW_MAT), 0, s, chi, batch, L.es, spd.p, S, C,
-                         mode, err);
-      GMOD_CHECK_LAUNCH();
-      GMOD_HIP(hipStreamSynchronize(s));  // spd is released at return
-      return;
-    }
-  }
-  if (L.bs == 1 && !getenv("GMOD_VERTEX_LDS")) {
+  if (L.bs == 1) {
     switch (n) {
       case 1: launch_vertex_reg<1>(chi, batch, L.es, S, C, mode, err, s); return;
       case 4: launch_vertex_reg<4>(chi, batch, L.es, S, C, mode, err, s); return;

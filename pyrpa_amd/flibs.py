@@ -34,6 +34,8 @@ _SIGS = {
     "get_chis": [_z, _z, _d, _pi, _pi],
     "get_chis_chic": [_z, _z, _z, _d, _d, _pi, _pi, _pi],
     "get_eig_or_tr_chi": [_z, _z, _i, _pi, _pi, _pi, _pb],
+    # addition: chi_s / chi_c at every (q, i nu)
+    "gmod_get_chis_chic_all": [c_void_p, c_void_p, _z, _d, _d, _pi, _pi, _pi],
     # libs/flibs/_green.py:7-27
     "gen_green0_": [_z, _d, _z, _pd, _pd, _pi, _pi, _pi],
     "gen_green_inv_": [_z, _z, _z, _pd, _pd, _pi, _pi, _pi],
@@ -288,6 +290,20 @@ def get_chis_chic(chi, Smat, Cmat):
     chis = np.zeros((Nchi, Nchi, Nk), dtype=np.complex128)
     chic = np.zeros((Nchi, Nchi, Nk), dtype=np.complex128)
     _call("get_chis_chic", chis, chic, np.ascontiguousarray(chi), Smat, Cmat, i64(Nk), i64(Nw), i64(Nchi))
+    return chis, chic
+
+
+def get_chis_chic_all(chi, Smat, Cmat, spin=True, charge=True):
+    """RPA chi_s and chi_c at every (q, i nu), [Nchi,Nchi,Nw,Nk] each like chi
+    (None for a susceptibility not asked for): the algebra of get_chis /
+    get_chis_chic applied to the whole batch (addition; no single reference
+    wrapper)"""
+    Nk, Nw, Nchi = chi.shape[3], chi.shape[2], len(Smat)
+    chis = np.zeros((Nchi, Nchi, Nw, Nk), dtype=np.complex128) if spin else None
+    chic = np.zeros((Nchi, Nchi, Nw, Nk), dtype=np.complex128) if charge else None
+    _call("gmod_get_chis_chic_all", None if chis is None else chis.ctypes.data,
+          None if chic is None else chic.ctypes.data, np.ascontiguousarray(chi), np.ascontiguousarray(Smat),
+          np.ascontiguousarray(Cmat), i64(Nk), i64(Nw), i64(Nchi))
     return chis, chic
 
 

@@ -67,20 +67,43 @@ def synthetic_lattice(norb, in_plane, amp, onsite, seed, inter=(), inter_scale=1
 
 # --------------------------------------------------------------------------- orbital pairs, gaps
 def get_chi_orb_list(Norb, site_prof=None):
-    """olist (Nchi,2), F-contiguous like the reference (_response.py:391-395)."""
-    tmp = np.arange(Norb) + 1
-    o1, o2 = np.meshgrid(tmp, tmp)
-    chiolist = np.array([o1.flatten(), o2.flatten()]).T
-    return chiolist.astype(np.int64), np.ones(len(chiolist), dtype=np.int64)
+    """(olist (Nchi,2), site (Nchi,)) of 1-based orbital pairs, olist[:,0]
+    fastest (libs/plibs/_response.py:382-413).  One site (site_prof None or of
+    length 1): the F-contiguous transposed view the reference returns, whose
+    memory is the Fortran olist(Nchi,2).  Several sites: the reference's
+    C-contiguous stack of per-site blocks, site label = first orbital of the
+    site (the Fortran side reads that layout transposed, as in the reference)."""
+    if site_prof is None or len(site_prof) == 1:
+        tmp = np.arange(Norb) + 1
+        o1, o2 = np.meshgrid(tmp, tmp)
+        chiolist = np.array([o1.flatten(), o2.flatten()]).T.astype(np.int64)
+        return chiolist, np.ones(len(chiolist), dtype=np.int64)
+    if Norb != sum(site_prof):
+        raise ValueError("site_prof doesn't correspond to Hamiltonian")
+    rows, site, n0 = [], [], 1
+    for ns in site_prof:
+        tmp = np.arange(ns) + n0
+        o1, o2 = np.meshgrid(tmp, tmp)
+        rows += list(np.array([o1.flatten(), o2.flatten()]).T)
+        site += [n0] * o1.size
+        n0 += ns
+    return np.array(rows, dtype=np.int64), np.array(site, dtype=np.int64)
 
 
 def gap_symms(klist, Norb, gap_sym):
+    """Initial-gap form factor of symmetry gap_sym on fractional k, tiled over
+    the orbitals (libs/plibs/_response.py:415-445): 0 s, 1 d_{x2-y2}, 2 s+-,
+    3 d_xy, 4 d_xz, 5 d_yz, -1 p_x, -2 p_y, -3 p_x + i p_y (complex); other
+    values give zeros."""
     A = 2 * np.pi
     kx, ky, kz = klist[:, 0], klist[:, 1], klist[:, 2]
-    rows = {0: np.ones(len(klist)), 1: np.cos(A * kx) - np.cos(A * ky), 2: 2 * np.cos(A * kx) * np.cos(A * ky),
-            3: 2 * np.sin(A * kx) * np.sin(A * ky), 4: 2 * np.sin(A * kx) * np.sin(A * kz),
-            5: 2 * np.sin(A * ky) * np.sin(A * kz), -1: 2 * np.sin(A * kx), -2: 2 * np.sin(A * ky)}
-    return np.tile(rows.get(gap_sym, np.zeros(len(klist))), (Norb, 1))
+    rows = {0: lambda: np.ones(len(klist)), 1: lambda: np.cos(A * kx) - np.cos(A * ky),
+            2: lambda: 2 * np.cos(A * kx) * np.cos(A * ky), 3: lambda: 2 * np.sin(A * kx) * np.sin(A * ky),
+            4: lambda: 2 * np.sin(A * kx) * np.sin(A * kz), 5: lambda: 2 * np.sin(A * ky) * np.sin(A * kz),
+            -1: lambda: 2 * np.sin(A * kx), -2: lambda: 2 * np.sin(A * ky),
+            -3: lambda: 2 * np.sin(A * kx) + 2j * np.sin(A * ky)}
+    row = rows[gap_sym]() if gap_sym in rows else np.zeros(len(klist))
+    return np.tile(row, (Norb, 1))
 
 
 get_initial_gap = gap_symms
@@ -88,12 +111,20 @@ get_initial_gap = gap_symms
 
 # --------------------------------------------------------------------------- chemical potential
 def calc_mu(eig, Nk, fill, temp):
-    """brentq on sum_k f(e_k - mu) = fill * Nk (libs/plibs/_bands.py:63-78)."""
+    """brentq on sum_k f(e_k - mu) = fill * Nk between the band edges
+    (libs/plibs/_bands.py:63-78); a filling outside the band range is clamped
+    to the closer edge, as the reference's _solve_mu does (_bands.py:50-61)."""
     from scipy.optimize import brentq
+    if temp <= 0:
+        raise ValueError(f"calc_mu requires temp > 0, got {temp}")
 
     def func(mu):
         return fill * Nk - F.get_ffermi(eig, mu, temp).sum()
-    return brentq(func, eig.min(), eig.max(), xtol=1e-14)
+    emin, emax = float(eig.min()), float(eig.max())
+    fa, fb = func(emin), func(emax)
+    if fa * fb > 0:
+        return emin if abs(fa) < abs(fb) else emax
+    return brentq(func, emin, emax)
 
 
 def get_mu(ham_r, rvec, temp, fill, kmesh=40):

@@ -32,7 +32,7 @@ def coarse_mesh(st, factor=2, nw_coarse=None):
     return cut(st['Nx']), cut(st['Ny']), cut(st['Nz']), nw
 
 
-def coarse_seed(st, factor=2, nw_coarse=None, eps=1e-3, precision=1, scf_loop=60):
+def coarse_seed(st, factor=2, nw_coarse=None, eps=1e-3, precision=1, scf_loop=60, sw_tail=False):
     """Solve FLEX on the coarse mesh and re-grid Sigma onto st's mesh.
     Returns ((sigma, mu, mu_old), info).  The seed's distance from the fine
     solution is the coarse mesh's discretisation error (~1e-2 at factor 2),
@@ -44,13 +44,14 @@ def coarse_seed(st, factor=2, nw_coarse=None, eps=1e-3, precision=1, scf_loop=60
     old = F.get_precision()
     try:
         F.set_precision(precision)
-        sig_c, mu_c = L.flex(sc, eps=eps, scf_loop=scf_loop, sw_rescale=True)
+        sig_c, mu_c = L.flex(sc, eps=eps, scf_loop=scf_loop, sw_rescale=True, sw_tail=sw_tail)
         stats = F.flex_stats()
     finally:
         F.set_precision(old)
     sig_f = F.regrid_sigma(sig_c, sc['kmap'], sc['invk'], (Nxc, Nyc, Nzc, Nwc), st['kmap'], st['invk'],
                            (st['Nx'], st['Ny'], st['Nz'], st['Nw']), st['temp'])
-    info = {"kind": f"coarse FLEX {Nxc}x{Nyc}x{Nzc} x {Nwc} Matsubara (tol {eps:g}), re-gridded",
+    info = {"kind": f"coarse FLEX {Nxc}x{Nyc}x{Nzc} x {Nwc} Matsubara (tol {eps:g}"
+                    f"{', tail-corrected chi0' if sw_tail else ''}), re-gridded",
             "coarse_iterations": int(stats['iterations']), "seconds": time.perf_counter() - t0}
     return (sig_f, mu_c, mu_c), info
 

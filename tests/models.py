@@ -23,18 +23,26 @@ def two_orbital(Nx=4, Ny=4, Nz=1, Nw=16, temp=0.1, mu=0.0, t1=1.0, t2=0.5, de=0.
     return _bundle(ham_r, RVEC_NN, 2, Nx, Ny, Nz, Nw, temp, mu, U, J)
 
 
-def random_model(Norb, Nx, Ny, Nz, Nw, temp, mu, U, J, seed=0, amp=0.3, interlayer=0.2):
-    """Seeded Hermitian hoppings to first neighbours (+ z hopping if Nz>1)."""
+def random_model(Norb, Nx, Ny, Nz, Nw, temp, mu, U, J, seed=0, amp=0.3, interlayer=0.2, real=False):
+    """Seeded Hermitian hoppings to first neighbours (+ z hopping if Nz>1).
+    Complex by default (time reversal broken: the library and the oracle run
+    the reference's algorithm on it all the same); real=True keeps the
+    hoppings real, i.e. time-reversal symmetric as the reference's k/-k
+    unfolding assumes physically."""
     rng = np.random.default_rng(seed)
     Rs = [(1, 0, 0), (0, 1, 0), (1, 1, 0), (1, -1, 0)]
     if Nz > 1:
         Rs += [(0, 0, 1)]
     rvec = [(0, 0, 0)]
     onsite = rng.uniform(-0.2, 0.2, size=(Norb, Norb)) + 1j * rng.uniform(-0.05, 0.05, size=(Norb, Norb))
+    if real:
+        onsite = onsite.real
     onsite = 0.5 * (onsite + onsite.conj().T)
     hams = [onsite]
     for R in Rs:
         h = rng.uniform(-amp, amp, size=(Norb, Norb)) + 1j * rng.uniform(-amp, amp, size=(Norb, Norb)) * 0.3
+        if real:
+            h = h.real
         if R[2] != 0:
             h = h * interlayer
         rvec += [R, tuple(-x for x in R)]
@@ -81,6 +89,42 @@ def lindhard_exact(st, mlist):
             deg = (m == 0) & (np.abs(epsg - eq) < 1e-9)
             safe = np.where(np.abs(de) < 1e-30, 1.0, de)
             out[iq, jm] = np.where(deg, fg * (1 - fg) / temp, (fq - fg) / safe).mean()
+    return out
+
+
+def bubble_exact(st, mlist):
+    """Exact Matsubara bubble of a multi-orbital G0 from a band sum over the
+    full mesh, in the layout of chi0 (Python [Nchi, Nchi, len(mlist), Nk]):
+    chi_{(l1 l2)(m1 m2)}(q, i nu) = -(T/N) sum_k sum_n G_{l1 m1}(k+q, i w + i nu) G_{m2 l2}(k, i w)
+      = (1/N) sum_k sum_ab U_{l1 a}(k+q) U*_{m1 a}(k+q) U_{m2 b}(k) U*_{l2 b}(k)
+        [f(e_a(k+q)) - f(e_b(k))] / (i nu + e_b(k) - e_a(k+q)),
+    -f'(e) = f (1 - f)/T at equal energies and nu = 0."""
+    Nx, Ny, Nz = st['Nx'], st['Ny'], st['Nz']
+    temp, mu = st['temp'], st['mu']
+    ix, iy, iz = np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing='ij')
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+    kf = np.stack([ix / Nx, iy / Ny, iz / Nz], axis=1)
+    e, u = O.get_eig(O.gen_ham(kf, st['ham_r'], st['rvec']))
+    U = np.transpose(u, (0, 2, 1))                      # U[k, orb, band]
+    f = fermi(e, temp, mu)
+    ol = np.asarray(st['olist']) - 1
+    Nchi = len(ol)
+    out = np.zeros((Nchi, Nchi, len(mlist), len(st['klist'])), dtype=complex)
+    for iq, q in enumerate(st['klist']):
+        qx, qy, qz = (int(round(q[0] * Nx)) % Nx, int(round(q[1] * Ny)) % Ny, int(round(q[2] * Nz)) % Nz)
+        kq = ((ix + qx) % Nx) * Ny * Nz + ((iy + qy) % Ny) * Nz + (iz + qz) % Nz
+        ea, fa, Ua = e[kq], f[kq], U[kq]
+        A = Ua[:, :, None, :] * np.conj(Ua[:, None, :, :])     # [k, l1, m1, a]
+        B = U[:, :, None, :] * np.conj(U[:, None, :, :])       # [k, m2, l2, b]
+        for jm, m in enumerate(mlist):
+            de = 2j * np.pi * m * temp + e[:, None, :] - ea[:, :, None]      # [k, a, b]
+            deg = (m == 0) & (np.abs(e[:, None, :] - ea[:, :, None]) < 1e-9)
+            safe = np.where(np.abs(de) < 1e-30, 1.0, de)
+            r = np.where(deg, (f * (1 - f))[:, None, :] / temp, (fa[:, :, None] - f[:, None, :]) / safe)
+            X = np.einsum('kxya,kzwb,kab->xyzw', A, B, r) / len(kf)   # [l1, m1, m2, l2]
+            for l in range(Nchi):
+                for mm in range(Nchi):
+                    out[l, mm, jm, iq] = X[ol[l, 0], ol[mm, 0], ol[mm, 1], ol[l, 1]]
     return out
 
 

@@ -1,6 +1,9 @@
-"""CPU checks of the C-ABI library: it loads, exports every symbol declared in
-include/gmod.h, and its host-side setup entry points (no GPU involved) match
-the oracle."""
+"""Checks of the C-ABI library: it loads, exports every symbol declared in
+include/gmod.h, and its host-side setup entry points match the oracle.
+
+The setup entry points run on the host (no GPU call), so their tests run in
+the CPU suite and again, through the RUN parameter's gpu-marked case, in the
+GPU suite on the machine the library is measured on."""
 import ctypes
 import os
 import re
@@ -17,7 +20,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "gmod.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:void|int|double)\s+(\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:void|int|int64_t|double)\s+(\w+)\s*\(", txt, flags=re.M)))
+
+
+# every host-side setup test runs once in each suite
+RUN = pytest.mark.parametrize("run", [pytest.param("cpu"), pytest.param("gpu", marks=pytest.mark.gpu)])
 
 
 def test_header_declares_the_hot_path_entry_points():
@@ -26,7 +33,8 @@ def test_header_declares_the_hot_path_entry_points():
                  "get_chis", "get_chis_chic", "get_eig_or_tr_chi", "gen_green0_", "gen_green_inv_", "getinv_",
                  "lin_eliash", "mkfk_trs_nsoc_", "get_initial_delta_", "conv_delta_orb_to_band", "remap_delta_",
                  "get_scmat", "get_scmat_orb", "generate_irr_kpoint_inv", "gen_ham", "get_eig", "get_ffermi",
-                 "openmp_params", "gmod_last_error"]:
+                 "openmp_params", "gmod_last_error", "gmod_get_chis_chic_all", "gmod_nn_create",
+                 "gmod_nn_train_epoch"]:
         assert name in syms
 
 
@@ -37,9 +45,10 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert not missing, missing
 
 
+@RUN
 @pytest.mark.parametrize("mesh", [(2, 2, 1), (4, 4, 1), (8, 8, 1), (4, 4, 4), (3, 3, 1), (4, 3, 1),
                                   (6, 4, 2), (5, 5, 3), (4, 4, 3)])
-def test_irreducible_mesh_matches_oracle(mesh):
+def test_irreducible_mesh_matches_oracle(mesh, run):
     import pyrpa_amd.flibs as F
     a = F.gen_irr_k_TRS(*mesh)
     b = O.gen_irr_k_TRS(*mesh)
@@ -47,7 +56,8 @@ def test_irreducible_mesh_matches_oracle(mesh):
         assert np.array_equal(x, y)
 
 
-def test_irreducible_mesh_full_size_consistent():
+@RUN
+def test_irreducible_mesh_full_size_consistent(run):
     """32x32x16 (the 4-orbital 3D mesh): pairing invariants at full size."""
     import pyrpa_amd.flibs as F
     klist, kmap, invk = F.gen_irr_k_TRS(32, 32, 16)
@@ -62,7 +72,8 @@ def test_irreducible_mesh_full_size_consistent():
     assert np.bincount(irr, minlength=len(klist)).sum() == 32 * 32 * 16
 
 
-def test_setup_host_entry_points_match_oracle():
+@RUN
+def test_setup_host_entry_points_match_oracle(run):
     import pyrpa_amd.flibs as F
     st = MD.random_model(4, 4, 4, 2, 4, 0.1, 0.0, 1.0, 0.2, seed=1)
     hk = F.gen_ham(st['klist'], st['ham_r'], st['rvec'])
@@ -84,7 +95,8 @@ def test_setup_host_entry_points_match_oracle():
     assert np.array_equal(w, O.kweight(st['invk'], len(st['klist'])))
 
 
-def test_gap_output_formatting_matches_oracle():
+@RUN
+def test_gap_output_formatting_matches_oracle(run):
     """conv_delta_orb_to_band / remap_gap are host-side output formatting."""
     import pyrpa_amd.flibs as F
     st = MD.random_model(3, 4, 4, 1, 3, 0.1, 0.0, 1.0, 0.2, seed=2)
