@@ -25,10 +25,11 @@ read or written counts N b (N grid elements, b bytes each), an irreducible
 (k, w) slice counts its own size, so a plain pass is 2 N b and each fused
 grid operand adds N b.
 
-The CPU baseline is the numpy restatement of the reference (oracle/) timed
-on a bounded sample -- one orbital-pair FFT convolution of the reference
-algorithm on this workload's grid -- and extrapolated to one reference FLEX
-iteration and to the plain arm's iteration count.
+The CPU baseline is the numpy/scipy restatement of the reference (oracle/)
+on the host cores: every component of a reference FLEX iteration timed on a
+bounded sample of this workload and scaled to one iteration, times the plain
+arm's iteration count; beside it, one whole oracle iteration of the 1-orbital
+64x64 x 1024 workload, measured.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 3]
 """
@@ -87,41 +88,127 @@ def solve(F, L, st, precision, eps, scf_loop=300, seed=None, profile=False):
     return sig, mu, stats, prof
 
 
-def cpu_baseline(st, iters_plain):
-    """Oracle timing on host cores: one orbital-pair bubble of the reference
-    algorithm (scatter, 2 forward 4D FFTs, product with (-r,-tau) reversal,
-    inverse FFT, gather; libs/src/fchi.f90:559-611) on the full grid, complex128.
-    A reference FLEX iteration does P such units for chi0 and Nchi^2 for Sigma
-    (libs/src/fself.f90:199-269); vertex, Dyson and mu search are left out, so
-    the extrapolation is a lower bound."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(st, iters_plain, threads):
+    """The oracle (numpy/scipy restatement of the reference, oracle/) timed on
+    the host cores on bounded samples of this workload, every component of a
+    reference FLEX iteration with the tail correction (libs/src/fself.f90:361-430):
+      * the bubble unit of fchi.f90:559-611 (scatter, 2 forward 4D FFTs,
+        product with the (-r,-tau) reversal, inverse FFT, gather) on the full
+        complex128 grid; an iteration runs 3 P of them for chi0 (conv[G],
+        conv[G0] and the tau reference, fchi.f90:852-892) and Nchi^2 for
+        Sigma (fself.f90:199-269);
+      * the vertex (fUJ.f90:318-365) on a sample of the (q, i nu) points,
+        scaled to all of them;
+      * Dyson (fself.f90:43-157) and the eigenvalues of the mu search
+        (fself.f90:716-740) on a sample of k points with all frequencies,
+        scaled to all k, plus 30 filling evaluations (the Brent search's
+        typical count) likewise scaled;
+    and, measured whole, one oracle FLEX iteration (the final mu search
+    included) of the 1-orbital 64x64 x 1024 workload."""
     import oracle.pyrpa_oracle as O
+    O.FFT_WORKERS = threads
     Nx, Ny, Nz, Nw = st["Nx"], st["Ny"], st["Nz"], st["Nw"]
     No = st["Norb"]
     Nchi = No * No
     P = Nchi * (Nchi + 1) // 2
-    # G0_00(k, i w_n) only (the one orbital element the timed unit reads),
-    # as gen_green0 builds it (libs/src/fself.f90:1-41)
     eig, uni = st["eig"], st["uni"]  # (Nk, No), (Nk, No, No) C views of Fortran eig(No,Nk), uni(No,No,Nk)
     Nk = eig.shape[0]
+    parts = {}
+    # bubble unit (G0_00 as gen_green0 builds it, fself.f90:1-41)
     wn = (2.0 * np.arange(Nw) + 1.0) * np.pi * st["temp"]
-    wts = np.abs(uni[:, :, 0]) ** 2  # |U_{0n}(k)|^2, Python uni[k, n, orb]
+    wts = np.abs(uni[:, :, 0]) ** 2
     g00 = (wts[:, None, :] / (1j * wn[None, :, None] + st["mu"] - eig[:, None, :])).sum(axis=2)
     Gf = g00[:, :, None, None]
     t0 = time.perf_counter()
     g13 = O.fft4(O._full_green(Gf, 0, 0, st["kmap"], st["invk"], Nx, Ny, Nz))
     g42 = O.fft4(O._full_green(Gf, 0, 0, st["kmap"], st["invk"], Nx, Ny, Nz))
     tmp = O.ifft4(-g13 * O._reverse(g42))
-    out = O._gather(tmp, st["kmap"], st["invk"], Nk, Nw)
-    t_pair = time.perf_counter() - t0
-    del g13, g42, tmp, out, Gf, g00
-    t_iter = (P + Nchi * Nchi) * t_pair
-    return {"value": round(t_iter * iters_plain, 1), "unit": "s to converge (extrapolated)", "cores": 1,
-            "kind": "port",
-            "sample": (f"one orbital-pair FFT bubble of the reference algorithm on the {Nx}x{Ny}x{Nz}x{2 * Nw} "
-                       f"complex128 grid took {t_pair:.2f} s (numpy pocketfft, 1 thread); x (P + Nchi^2) = "
-                       f"{P + Nchi * Nchi} units = {t_iter:.0f} s per FLEX iteration (lower bound), "
-                       f"x {iters_plain} iterations of the plain arm"),
-            "s_per_iteration": round(t_iter, 1)}
+    O._gather(tmp, st["kmap"], st["invk"], Nk, Nw)
+    t_unit = time.perf_counter() - t0
+    del g13, g42, tmp, Gf, g00
+    parts["bubble_units"] = (3 * P + Nchi * Nchi) * t_unit
+    # vertex on a sample of (q, i nu) points
+    rng = np.random.default_rng(1)
+    ns = min(Nk * Nw, 16384)
+    chi = 0.05 * (rng.standard_normal((Nchi, Nchi, 1, ns)) + 1j * rng.standard_normal((Nchi, Nchi, 1, ns)))
+    t0 = time.perf_counter()
+    O.get_Vsigma_nosoc_flex(chi, st["Smat"], st["Cmat"])
+    parts["vertex"] = (time.perf_counter() - t0) * (Nk * Nw) / ns
+    # Dyson and the mu search on a sample of k points
+    ks = min(Nk, 64)
+    sig = 0.05 * (rng.standard_normal((No, No, Nw, ks)) + 1j * rng.standard_normal((No, No, Nw, ks)))
+    t0 = time.perf_counter()
+    O.gen_green(sig, st["hamk"][:ks], st["mu"], st["temp"])
+    fl = O._Flex(st["hamk"][:ks], eig[:ks], st["fill"], st["temp"], 1, No, ks, Nw)
+    fl.precompute(np.asarray(sig).T)
+    for i in range(30):
+        fl.get_rn(st["mu"] + 1e-3 * i)
+    parts["dyson_and_mu"] = (time.perf_counter() - t0) * Nk / ks
+    t_iter = sum(parts.values())
+    # one whole oracle FLEX iteration of the 1-orbital 64x64 x 1024 workload
+    from pyrpa_amd import lattice as L
+    c1 = L.config(1)
+    t0 = time.perf_counter()
+    O.mkself(c1["Smat"], c1["Cmat"], c1["kmap"], c1["invk"], c1["olist"], c1["hamk"], c1["eig"], c1["uni"],
+             c1["mu"], c1["fill"], c1["temp"], c1["Nw"], c1["Nx"], c1["Ny"], c1["Nz"], False, False, eps=0.0,
+             pp=0.5, m_diis=1, scf_loop=1, sw_rescale=True, sw_tail=SW_TAIL)
+    t_c1 = time.perf_counter() - t0
+    O.FFT_WORKERS = 1
+    return {"value": round(t_iter * iters_plain, 1), "unit": "s to converge (extrapolated)", "cores": threads,
+            "kind": "port", "cpu": _cpu_model(),
+            "sample": (f"oracle on {threads} host threads ({_cpu_model()}): one bubble unit of the reference "
+                       f"algorithm on the {Nx}x{Ny}x{Nz}x{2 * Nw} complex128 grid {t_unit:.2f} s, x (3P + Nchi^2) = "
+                       f"{3 * P + Nchi * Nchi} units; vertex on {ns} of {Nk * Nw} (q, i nu) points; Dyson and the "
+                       f"mu search on {ks} of {Nk} k points; {t_iter:.0f} s per FLEX iteration x {iters_plain} "
+                       f"iterations of the plain arm. Measured whole: one oracle FLEX iteration of the 1-orbital "
+                       f"64x64 x 1024 workload {t_c1:.1f} s"),
+            "s_per_iteration": round(t_iter, 1), "s_per_iteration_parts": {k: round(v, 1) for k, v in parts.items()},
+            "one_orbital_64x64_1024_s_per_iteration_measured": round(t_c1, 2)}
+
+
+def fft_engine(st, reps):
+    """The transform engine alone (gmod_fft_device) on the workload's full
+    (k-mesh x 2 Nw) grid: `reps` in-place forward 4D transforms per precision,
+    timed with events on the stream the library launches on (synchronised by
+    the entry point).  One pass moves 2 N b bytes (N grid elements of b bytes)."""
+    from pyrpa_amd import flibs as F
+    dims = (st["Nx"], st["Ny"], st["Nz"], 2 * st["Nw"])
+    n = int(np.prod(dims))
+    out = {}
+    rng = np.random.default_rng(0)
+    for name, dt, b in (("fp32", np.complex64, 8), ("fp64", np.complex128, 16)):
+        buf = F.DeviceArray.from_host((rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(dt))
+        tmp = F.DeviceArray(n, dt)
+        F.profile(True)
+        F.fft_device(buf, tmp, dims, sign=-1, reps=1)       # plan + warm-up
+        F.profile(False)
+        F.profile(True)
+        t0 = time.perf_counter()
+        F.fft_device(buf, tmp, dims, sign=-1, reps=reps)
+        wall = time.perf_counter() - t0
+        prof = F.profile_read()
+        F.profile(False)
+        pc = prof["fft_pass_fp32" if b == 8 else "fft_pass"]
+        passes = pc["launches"] // reps
+        gbs = pc["bytes"] / (pc["ms"] * 1e-3) / 1e9
+        out[name] = {"passes_per_transform": int(passes), "ms_per_transform": round(1e3 * wall / reps, 4),
+                     "us_per_pass": round(1e3 * pc["ms"] / max(pc["launches"], 1), 2),
+                     "algorithmic_gbs": round(gbs, 1), "frac_of_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"fft {name}: {out[name]}")
+        buf.free()
+        tmp.free()
+    return {"metric": "4D FFT pass throughput (algorithmic bytes / pass time)", "grid": list(dims),
+            "reps": reps, **out}
 
 
 def main():
@@ -132,7 +219,11 @@ def main():
     ap.add_argument("--config", type=int, default=3, help="workload (1..4); the metric is quoted on 3")
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads of the CPU baseline (the GPU box's CPU share is 16)")
     ap.add_argument("--warm", default="coarse", choices=["coarse", "none"], help="warm start of the mixed arm")
+    ap.add_argument("--fft", type=int, default=0, metavar="REPS",
+                    help="time only the transform engine on the workload's grid (REPS transforms per precision)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -147,6 +238,9 @@ def main():
 
     t0 = time.perf_counter()
     st = L.config(args.config)
+    if args.fft:
+        print(json.dumps(fft_engine(st, args.fft)), flush=True)
+        return
     log(f"workload {args.config} built in {time.perf_counter() - t0:.1f} s: mesh {st['Nx']}x{st['Ny']}x{st['Nz']} "
         f"Nw={st['Nw']} Norb={st['Norb']} Nk={len(st['klist'])} mu0={st['mu']:.6f}")
 
@@ -194,7 +288,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(st, int(best["s64"]["iterations"]))
+            cpu = cpu_baseline(st, int(best["s64"]["iterations"]), args.cpu_threads)
             log(f"cpu baseline: {cpu['sample']}")
         except MemoryError:
             cpu = None

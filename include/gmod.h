@@ -16,6 +16,7 @@
 #ifndef GMOD_H
 #define GMOD_H
 #include <stdbool.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -217,6 +218,14 @@ void gmod_nn_set_data(int64_t h, const float* X, const float* T, int64_t n);
 /* one pass over order[0..n_used) (row indices into the resident set) in
  * mini-batches of `batch`, one Adam step per batch; returns the mean batch loss */
 double gmod_nn_train_epoch(int64_t h, const int64_t* order, int64_t n_used, int64_t batch, double lr);
+
+/* Device buffers for the device-pointer entry points (gmod_fft_device and
+ * the allocation-free variant below), so that a caller needs no second HIP
+ * runtime in its process.  gmod_memcpy kind: 1 host to device, 2 device to
+ * host, 3 device to device; synchronous; returns 0, or 1 on an error. */
+void* gmod_device_alloc(size_t bytes);
+void gmod_device_free(void* p);
+int gmod_memcpy(void* dst, const void* src, size_t bytes, int kind);
 
 /* Device memory held by the library's caching pool, in bytes. */
 double gmod_pool_bytes(void);

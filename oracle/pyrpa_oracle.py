@@ -387,14 +387,25 @@ def get_chi_map(olist):
 # --------------------------------------------------------------------------- #
 #  full-grid scatter / FFT conventions (fmod.f90:67-160, fchi.f90:563-583)
 # --------------------------------------------------------------------------- #
+# host threads of the oracle's transforms (scipy.fft workers; pocketfft, the
+# library the reference vendors); 1 unless a timing harness raises it
+FFT_WORKERS = 1
+
+
+def _fftn(a, axes):
+    import scipy.fft
+    return scipy.fft.fftn(a, axes=axes, workers=FFT_WORKERS)
+
+
 def fft4(a):
     """FFT(.true.): 4D, sign -1, unnormalised (fmod.f90:67-116)."""
-    return np.fft.fftn(a, axes=(0, 1, 2, 3))
+    return _fftn(a, (0, 1, 2, 3))
 
 
 def ifft4(a):
     """FFT(.false.): 4D, sign +1, divided by the grid size."""
-    return np.fft.ifftn(a, axes=(0, 1, 2, 3))
+    import scipy.fft
+    return scipy.fft.ifftn(a, axes=(0, 1, 2, 3), workers=FFT_WORKERS)
 
 
 def _spatial(kmap, Nx, Ny, Nz):
@@ -577,8 +588,8 @@ def _chi0_ref_tau(chif, eig, uni, mu, kmap, invk, olist, temp, Nx, Ny, Nz, Nw):
         B[sp] = zB
         A = A.reshape(Nx, Ny, Nz, M)
         B = B.reshape(Nx, Ny, Nz, M)
-        A = np.fft.fftn(A, axes=(0, 1, 2))
-        B = np.fft.fftn(B, axes=(0, 1, 2))
+        A = _fftn(A, (0, 1, 2))
+        B = _fftn(B, (0, 1, 2))
         idx = [(-np.arange(n)) % n for n in (Nx, Ny, Nz)]
         Brev = B[np.ix_(idx[0], idx[1], idx[2], np.arange(M))]
         tmp = ifft4(-A * Brev)

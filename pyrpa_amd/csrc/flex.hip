@@ -937,6 +937,78 @@ extern "C" void get_eig_or_tr_chi(gmod_z* chiqout, const gmod_z* chi, const int6
   GMOD_CATCH
 }
 
+// ---------------------------------------------------------------------- allocation-free variant
+// A file-scope device array holds the workspace of gmod_eig_or_tr_chi_noalloc
+// (its error flag, the per-k maxima and the interleaved Hessenberg matrices of
+// k_max_re_eig).  In host code `scratch` names a host-side stand-in,
+// so the kernels receive the device address that hipGetSymbolAddress returns,
+// looked up once per device and kept.  Every call on a device shares that
+// device's array: calls that overlap on one device race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static unsigned char* noalloc_scratch_ptr() {
+  static std::mutex mu;
+  static std::vector<unsigned char*> per_dev;
+  int dev = 0;
+  GMOD_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(mu);
+  if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
+  if (!per_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    per_dev[dev] = static_cast<unsigned char*>(p);
+  }
+  return per_dev[dev];
+}
+
+__global__ void k_zero_int(int* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+extern "C" int gmod_eig_or_tr_chi_noalloc(gmod_z* chiqout, const gmod_z* chi, const int64_t* invk, int64_t Nkall,
+                                          int64_t Nk, int64_t Nchi, int sw_eig) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int n = (int)Nchi;
+  if (Nk < 1 || Nkall < 1 || n < 1) throw Error("gmod_eig_or_tr_chi_noalloc: empty batch");
+  if (!chiqout || !chi || !invk) throw Error("gmod_eig_or_tr_chi_noalloc: null device pointer");
+  // workspace: error flag | per-k maxima (Nk doubles) | Nk (n^2 + n) complex
+  const size_t off_v = 256, off_h = off_v + (((size_t)Nk * sizeof(double) + 255) / 256) * 256;
+  const size_t need = sw_eig ? off_h + (size_t)Nk * (size_t)(n * n + n) * sizeof(zd) : off_v;
+  if (need > NOALLOC_SCRATCH_BYTES)
+    throw Error("gmod_eig_or_tr_chi_noalloc: Nk = " + std::to_string(Nk) + ", Nchi = " + std::to_string(n) +
+                " needs " + std::to_string(need) + " bytes of workspace, more than its " +
+                std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte maximum");
+  unsigned char* ws = noalloc_scratch_ptr();
+  int* err = reinterpret_cast<int*>(ws);
+  if (sw_eig) {
+    double* v = reinterpret_cast<double*>(ws + off_v);
+    zd* h = reinterpret_cast<zd*>(ws + off_h);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, err);
+    GMOD_CHECK_LAUNCH();
+    max_re_eig_ws(v, (const zd*)chi, BatchLayout{1, Nk}, Nk, n, nullptr, 1.0, h, err, s);
+    hipLaunchKernelGGL(k_eig_map, dim3(256), dim3(256), 0, s, (zd*)chiqout, v, invk, (long)Nkall);
+    GMOD_CHECK_LAUNCH();
+    int herr = 0;
+    GMOD_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    GMOD_HIP(hipStreamSynchronize(s));
+    if (herr) throw Error("gmod_eig_or_tr_chi_noalloc: eigenvalue iteration did not converge");
+  } else {
+    hipLaunchKernelGGL(k_trace_map, dim3(256), dim3(256), 0, s, (zd*)chiqout, (const zd*)chi, invk, (long)Nkall,
+                       (long)Nk, n);
+    GMOD_CHECK_LAUNCH();
+    GMOD_HIP(hipStreamSynchronize(s));
+  }
+  return 0;
+  }
+  catch (std::exception& e) { set_error(e.what()); return 1; }
+  catch (...) { set_error("unknown error"); return 1; }
+}
+
 // ====================================================================== Green functions
 extern "C" void gen_green0_(gmod_z* Gk, const double* eig, const gmod_z* uni, const double* mu, const double* temp,
                             const int64_t* Nk, const int64_t* Nw, const int64_t* Norb) {

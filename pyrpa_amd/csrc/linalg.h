@@ -30,6 +30,10 @@ void rpa_solve(zd* out, BatchLayout Lo, const zd* A, BatchLayout La, long batch,
 // max Re eig(alpha * A * W) per matrix (W = nullptr: identity)  (fchi.f90:373-389)
 void max_re_eig(double* out, const zd* A, BatchLayout La, long batch, int n, const double* W, double alpha,
                 int* err, hipStream_t s);
+// the same with a caller-provided workspace of batch (n^2 + n) elements; no
+// allocation, no synchronisation
+void max_re_eig_ws(double* out, const zd* A, BatchLayout La, long batch, int n, const double* W, double alpha,
+                   zd* work, int* err, hipStream_t s);
 
 // Dyson: G = [(i w_j + mu) I - H(k) - Sigma(k, i w_j)]^-1   (fself.f90:43-83, 131-157)
 void dyson(zd* G, const zd* sigma, const zd* hamk, double mu, double temp, long Nk, int Nw, int No, int* err,

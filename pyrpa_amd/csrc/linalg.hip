@@ -873,13 +873,19 @@ __global__ void k_max_re_eig(double* out, const zd* A, BatchLayout La, long batc
   }
 }
 
+void max_re_eig_ws(double* out, const zd* A, BatchLayout La, long batch, int n, const double* W, double alpha,
+                   zd* work, int* err, hipStream_t s) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(k_max_re_eig, dim3(nblk(batch, 64)), dim3(64), 0, s, out, A, La, batch, n, W, alpha, work,
+                     err);
+  GMOD_CHECK_LAUNCH();
+}
+
 void max_re_eig(double* out, const zd* A, BatchLayout La, long batch, int n, const double* W, double alpha,
                 int* err, hipStream_t s) {
   if (batch <= 0) return;
   DBuf<zd> scratch(batch * (long)(n * n + n));
-  hipLaunchKernelGGL(k_max_re_eig, dim3(nblk(batch, 64)), dim3(64), 0, s, out, A, La, batch, n, W, alpha,
-                     scratch.p, err);
-  GMOD_CHECK_LAUNCH();
+  max_re_eig_ws(out, A, La, batch, n, W, alpha, scratch.p, err, s);
   GMOD_HIP(hipStreamSynchronize(s));
 }
 

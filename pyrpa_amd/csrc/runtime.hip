@@ -196,6 +196,44 @@ extern "C" void gmod_release_memory(void) {
   try { pool().trim(); } catch (std::exception& e) { set_error(e.what()); }
 }
 
+// Device buffers for callers of the device-pointer entry points that hold no
+// other HIP runtime (a process that loads a second HIP runtime, e.g. one
+// bundled with a Python package, ends up with two runtimes sharing one GPU).
+extern "C" void* gmod_device_alloc(size_t bytes) {
+  try {
+    std::lock_guard<std::mutex> g(ctx().lock);
+    void* p = nullptr;
+    GMOD_HIP(hipMalloc(&p, bytes ? bytes : 1));
+    return p;
+  } catch (std::exception& e) {
+    set_error(e.what());
+    return nullptr;
+  }
+}
+extern "C" void gmod_device_free(void* p) {
+  try {
+    std::lock_guard<std::mutex> g(ctx().lock);
+    if (p) GMOD_HIP(hipFree(p));
+  } catch (std::exception& e) { set_error(e.what()); }
+}
+// kind 1: host to device, 2: device to host, 3: device to device; synchronous
+extern "C" int gmod_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  try {
+    std::lock_guard<std::mutex> g(ctx().lock);
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                            : kind == 2 ? hipMemcpyDeviceToHost
+                            : kind == 3 ? hipMemcpyDeviceToDevice
+                                        : throw Error("gmod_memcpy: kind must be 1, 2 or 3");
+    hipStream_t s = ctx().stream;
+    GMOD_HIP(hipMemcpyAsync(dst, src, bytes, k, s));
+    GMOD_HIP(hipStreamSynchronize(s));
+    return 0;
+  } catch (std::exception& e) {
+    set_error(e.what());
+    return 1;
+  }
+}
+
 // ====================================================================== C ABI: setup (host)
 // These run on the host exactly like the reference's own setup routines: they
 // produce the k-mesh tables and band data once per run.

@@ -6,7 +6,7 @@ Fortran shapes), so a caller of ``libs.flibs`` can switch by changing the
 import.  Differences: errors raise ``RuntimeError`` instead of terminating the
 process (the reference prints and STOPs), and the GPU library is required.
 """
-from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p
+from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_void_p
 
 import numpy as np
 
@@ -86,6 +86,14 @@ def _L():
         L.gmod_set_verbose.argtypes = [c_int]
         L.gmod_pool_bytes.restype = c_double
         L.gmod_fft_device.argtypes = [c_void_p, c_void_p] + [c_int] * 8
+        L.gmod_device_alloc.argtypes = [c_size_t]
+        L.gmod_device_alloc.restype = c_void_p
+        L.gmod_device_free.argtypes = [c_void_p]
+        L.gmod_device_free.restype = None
+        L.gmod_memcpy.argtypes = [c_void_p, c_void_p, c_size_t, c_int]
+        L.gmod_memcpy.restype = c_int
+        L.gmod_eig_or_tr_chi_noalloc.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int]
+        L.gmod_eig_or_tr_chi_noalloc.restype = c_int
         L.gmod_fft_device.restype = c_int
         _bound = True
     return L
@@ -447,20 +455,63 @@ def release_memory():
     check()
 
 
+class DeviceArray:
+    """A device buffer of the library (gmod_device_alloc) holding `n` elements
+    of numpy dtype `dtype`, for the device-pointer entry points.  It keeps the
+    process on the library's own HIP runtime."""
+
+    def __init__(self, n, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = _L().gmod_device_alloc(max(self.nbytes, 1))
+        check()
+        if not self.ptr:
+            raise RuntimeError("gmod_device_alloc failed")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.n:
+            raise ValueError("DeviceArray.upload: size mismatch")
+        if _L().gmod_memcpy(self.ptr, a.ctypes.data, self.nbytes, 1) != 0:
+            check()
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        if _L().gmod_memcpy(out.ctypes.data, self.ptr, self.nbytes, 2) != 0:
+            check()
+        return out
+
+    def free(self):
+        if self.ptr:
+            _L().gmod_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def fft_device(buf, tmp, dims, sign=-1, mask=15, reps=1):
     """In-place transform of a device grid (gmod_fft_device): buf and tmp are
-    contiguous torch complex64 (FP32 arithmetic) or complex128 (FP64) device
-    tensors of nx*ny*nz*m elements, x fastest; dims = (nx, ny, nz, m)."""
-    import torch
-    if buf.dtype not in (torch.complex64, torch.complex128) or tmp.dtype != buf.dtype:
-        raise TypeError("fft_device: complex64 or complex128 tensors of one dtype")
-    if not (buf.is_cuda and tmp.is_cuda and buf.is_contiguous() and tmp.is_contiguous()):
-        raise ValueError("fft_device: contiguous device tensors")
+    DeviceArray of complex64 (FP32 arithmetic) or complex128 (FP64) with
+    nx*ny*nz*m elements, x fastest; dims = (nx, ny, nz, m)."""
+    if buf.dtype not in (np.complex64, np.complex128) or tmp.dtype != buf.dtype:
+        raise TypeError("fft_device: complex64 or complex128 buffers of one dtype")
     nx, ny, nz, m = (int(v) for v in dims)
-    if buf.numel() != nx * ny * nz * m or tmp.numel() != buf.numel():
-        raise ValueError("fft_device: tensor size does not match dims")
-    prec = 1 if buf.dtype == torch.complex64 else 0
-    rc = _L().gmod_fft_device(buf.data_ptr(), tmp.data_ptr(), nx, ny, nz, m, int(mask), int(sign), prec, int(reps))
+    if buf.n != nx * ny * nz * m or tmp.n != buf.n:
+        raise ValueError("fft_device: buffer size does not match dims")
+    prec = 1 if buf.dtype == np.complex64 else 0
+    rc = _L().gmod_fft_device(buf.ptr, tmp.ptr, nx, ny, nz, m, int(mask), int(sign), prec, int(reps))
     check()
     if rc != 0:
         raise RuntimeError("gmod_fft_device failed")

@@ -1,0 +1,142 @@
+"""The benchmark configurations at their full sizes on the GPU, checked by
+properties that hold at any size (the oracle's cost grows past a test's
+budget there; its parity with the library is established on the small grids
+of the other test files, on the same code paths).
+
+  1-orbital FLEX, 64x64 x 1024 Matsubara (T = 0.02, n = 0.85, U = 3):
+      convergence to esterr < 1e-8, the filling of the returned mu by the
+      resolvent trace, causality Im Sigma <= 0, Sigma(kx,ky) = Sigma(ky,kx).
+  5-orbital RPA, 64x64 x 512 bosonic frequencies (T = 0.01, U = 1.2, J = 0.3):
+      chi0 with FP32 and with FP64 transforms, chi_s and chi_c at every
+      (q, i nu) with the FP64 denominator; the FP32 chi_s within 1e-4 of the
+      FP64 one (max|d chi_s| / max|chi_s|), chi0 Hermitian-symmetric.
+  1-orbital linearized Eliashberg on FLEX output, 128x128 x 2048 (T = 0.01,
+      n = 0.88, U = 3.5), random initial gap: lambda with FP32 convolutions
+      within 1e-5 of lambda with FP64 ones, the gap d-wave:
+      Delta(kx,ky) = -Delta(ky,kx).
+Each test prints its figures (run with -s to see them)."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def F():
+    import pyrpa_amd.flibs as F
+    F.set_precision(0)
+    yield F
+    F.set_precision(0)
+
+
+def swap_partner(klist, N):
+    """index of (ky, kx) (or its -k image) for every irreducible k of a 2D mesh"""
+    kc = np.rint(klist[:, :2] * N).astype(int) % N
+    pos = {tuple(k): i for i, k in enumerate(kc)}
+    out = np.empty(len(kc), dtype=int)
+    for i, (x, y) in enumerate(kc):
+        j = pos.get((y, x))
+        if j is None:
+            j = pos[((-y) % N, (-x) % N)]
+        out[i] = j
+    return out
+
+
+def test_config_flex_one_orbital_64x64_1024(F):
+    from pyrpa_amd import lattice as L
+    st = L.config(1)
+    t0 = time.perf_counter()
+    sig, mu = L.flex(st, eps=1e-8, pp=0.5, m_diis=1, scf_loop=300, sw_tail=True, sw_rescale=True)
+    stats = F.flex_stats()
+    print(f"\n[C1] {int(stats['iterations'])} iterations, {time.perf_counter() - t0:.2f} s, esterr "
+          f"{stats['esterr']:.2e}, mu {mu:.8f}, Stoner {stats['stoner']:.6f}")
+    assert stats['esterr'] < 1e-8
+    T, Nw = st['temp'], st['Nw']
+    s = sig[0, 0]                               # [Nw, Nk]
+    eps_k = st['eig'][:, 0]
+    wn = (2 * np.arange(Nw) + 1) * np.pi * T
+    G = 1.0 / (1j * wn[:, None] + mu - eps_k[None, :] - s)
+    G0 = 1.0 / (1j * wn[:, None] + mu - eps_k[None, :])
+    f = 0.5 - 0.5 * np.tanh((eps_k - mu) / (2 * T))
+    n = (f.sum() + 2 * T * (G - G0).real.sum()) / len(eps_k)  # fself.f90:687-714, irreducible average
+    print(f"[C1] filling by the resolvent trace {n:.12f} (target {st['fill']})")
+    assert abs(n - st['fill']) < 1e-7
+    assert s.imag.max() <= 1e-12                # causality at w_n > 0
+    p = swap_partner(st['klist'], 64)
+    assert np.abs(s - s[:, p]).max() < 1e-9 * np.abs(s).max()
+
+
+def test_config_rpa_five_orbital_64x64_512(F):
+    from pyrpa_amd import lattice as L
+    st = L.config(2)
+    Gk = F.gen_Green0(st['eig'], st['uni'], st['mu'], st['temp'], st['Nw'])
+    args = (st['Smat'], st['Cmat'], Gk, st['olist'], st['kmap'], st['invk'], st['temp'], 64, 64, 1)
+    t0 = time.perf_counter()
+    chi64, s64 = F.get_chi0(*args)
+    t64 = time.perf_counter() - t0
+    try:
+        F.set_precision(1)
+        t0 = time.perf_counter()
+        chi32, s32 = F.get_chi0(*args)
+        t32 = time.perf_counter() - t0
+    finally:
+        F.set_precision(0)
+    e_chi0 = float(np.abs(chi32 - chi64).max() / np.abs(chi64).max())
+    print(f"\n[C2] chi0 FP64 {t64:.2f} s, FP32 {t32:.2f} s (with the host copies); Stoner FP64 {s64:.8f} "
+          f"FP32 {s32:.8f}; chi0 max|d|/max {e_chi0:.2e}")
+    assert e_chi0 < 1e-5
+    # chi_{l,m}(q, i nu) = chi*_{m',l'} partners: the static slice is Hermitian per q
+    c0 = chi64[:, :, 0, :]
+    assert np.abs(c0 - np.conj(np.swapaxes(c0, 0, 1))).max() < 1e-12 * np.abs(c0).max()
+    if s64 >= 1.0:
+        pytest.skip(f"Stoner factor {s64:.4f} >= 1: chi_s is not defined")
+    del c0
+    t0 = time.perf_counter()
+    xs64, _ = F.get_chis_chic_all(chi64, st['Smat'], st['Cmat'], charge=False)
+    ts = time.perf_counter() - t0
+    del chi64
+    xs32, _ = F.get_chis_chic_all(chi32, st['Smat'], st['Cmat'], charge=False)
+    del chi32
+    e_chis = float(np.abs(xs32 - xs64).max() / np.abs(xs64).max())
+    print(f"[C2] chi_s at all (q, i nu): {ts:.2f} s; FP32-chi0 chi_s max|d|/max|chi_s| {e_chis:.2e}")
+    assert e_chis < 1e-4
+
+
+def test_config_eliashberg_one_orbital_128x128_2048(F):
+    from pyrpa_amd import lattice as L
+    st = L.config(4)
+    t0 = time.perf_counter()
+    sig, mu = L.flex(st, eps=1e-8, pp=0.5, m_diis=1, scf_loop=300, sw_tail=True, sw_rescale=True)
+    stats = F.flex_stats()
+    print(f"\n[C4] FLEX {int(stats['iterations'])} iterations, {time.perf_counter() - t0:.2f} s, esterr "
+          f"{stats['esterr']:.2e}, Stoner {stats['stoner']:.6f}")
+    assert stats['esterr'] < 1e-8
+    G = F.gen_green(sig, st['hamk'], mu, st['temp'])
+    del sig
+    chi, _ = F.get_chi0_tail(st['Smat'], st['Cmat'], G, st['eig'], st['uni'], st['olist'], st['kmap'], st['invk'],
+                             mu, st['temp'], 128, 128, 1)
+    rng = np.random.default_rng(20261202 + 4)
+    init = rng.uniform(-1.0, 1.0, (1, len(st['klist'])))
+    args = (G, st['uni'], init, st['Smat'], st['Cmat'], st['olist'], st['plist'], st['kmap'], st['invk'], 128, 128,
+            1, st['temp'], 1)
+    res = {}
+    for prec in (0, 1):
+        try:
+            F.set_precision(prec)
+            t0 = time.perf_counter()
+            gap, lam = F.linearized_eliashberg(chi.copy(), *args, eps=1e-7, itemax=400, arnoldi_m=0)
+            res[prec] = (gap, lam, time.perf_counter() - t0)
+        finally:
+            F.set_precision(0)
+    l64, l32 = res[0][1], res[1][1]
+    print(f"[C4] lambda FP64 {l64:.10f} ({res[0][2]:.2f} s), FP32 convolutions {l32:.10f} ({res[1][2]:.2f} s), "
+          f"|d lambda| {abs(l32 - l64):.2e}")
+    assert abs(l32 - l64) < 1e-5
+    p = swap_partner(st['klist'], 128)
+    for prec in (0, 1):
+        g = res[prec][0][0, 0, 0]               # w_0 slice over the irreducible k
+        anti = np.abs(g + g[p]).max() / np.abs(g).max()
+        print(f"[C4] precision {prec}: max|Delta(kx,ky) + Delta(ky,kx)| / max|Delta| = {anti:.2e}")
+        assert anti < 1e-4
