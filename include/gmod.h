@@ -80,6 +80,19 @@ void get_eig_or_tr_chi(gmod_z* chiqout, const gmod_z* chi, const int64_t* invk, 
 void gmod_get_chis_chic_all(gmod_z* chis, gmod_z* chic, const gmod_z* chi, const double* Smat, const double* Cmat,
                             const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi);
 
+/* No counterpart in the reference: an allocation-free variant of
+ * get_eig_or_tr_chi for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  chiqout (Nkall), chi (Nk,Nchi,Nchi) and invk (3,Nkall) are
+ * device pointers; sw_eig nonzero: leading real eigenvalue per k, else the
+ * trace.  The workspace (error flag, per-k maxima, interleaved Hessenberg
+ * matrices: about Nk (Nchi^2 + Nchi) 16 bytes) is a fixed 64 MiB device
+ * array of the library, one per GPU, held until the process ends; a batch
+ * needing more is refused.  Calls that overlap on one GPU race.  Returns 0,
+ * or 1 on an error (gmod_last_error). */
+int gmod_eig_or_tr_chi_noalloc(gmod_z* chiqout, const gmod_z* chi, const int64_t* invk, int64_t Nkall, int64_t Nk,
+                               int64_t Nchi, int sw_eig);
+
 /* ---------------- Green functions ---------------------------------------- */
 /* libs/src/fself.f90:1-41 (bind name gen_green0_) [libs/flibs/_green.py:80-97] */
 void gen_green0_(gmod_z* Gk, const double* eig, const gmod_z* uni, const double* mu, const double* temp,
