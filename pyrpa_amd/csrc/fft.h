@@ -341,8 +341,11 @@ void FFTPlan<R>::launch_box(int sg, int i, LD ld, ST st, const cpx<R>* tw, R sc,
   // tiled passes: persistent work-groups, one per CU, a multiple of 8
   long grid = a.blocks();
   if (a.tiled && grid > persist_) grid = persist_;
+  // the pool a pass reads its tables from is the sign's; the power-of-two
+  // sub-passes take the sign itself at compile time
+  const int sign = tw == twb_[0] ? -1 : 1;
   hipLaunchKernelGGL((box_fft_kernel<R, LD, ST>), dim3((unsigned)grid), dim3(BOX_THREADS), shm, s, ld, st,
-                     (const BoxArgs*)(argsd_[sg] + i), tw, (const int*)tabs_, sc);
+                     (const BoxArgs*)(argsd_[sg] + i), tw, (const int*)tabs_, sc, sign);
   GMOD_CHECK_LAUNCH();
 }
 
@@ -430,7 +433,7 @@ void FFTPlan<R>::exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac
   long grid = a.blocks();
   if (grid > persist_) grid = persist_;
   hipLaunchKernelGGL((box_accum_kernel<R, 16, AC>), dim3((unsigned)grid), dim3(BOX_THREADS), shm, s, src, stride,
-                     nsrc, ac, (const BoxArgs*)(argsd_[sg] + i), twb_[sg], (const int*)tabs_, scale);
+                     nsrc, ac, (const BoxArgs*)(argsd_[sg] + i), twb_[sg], (const int*)tabs_, scale, sg == 0 ? -1 : 1);
   GMOD_CHECK_LAUNCH();
 }
 

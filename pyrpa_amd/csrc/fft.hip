@@ -674,7 +674,7 @@ int box_host_run(double* data, int nx, int ny, int nz, int m, int mask, int sign
       const BoxOrigin O = box_origin(a, lb);
       box_twiddles_to_lds<R>(lds.data(), a, tw, 0, 1);
       box_load<R>(lds.data(), HostGridLoad<R>{in, S}, a, O, 0, 1, plan.tab_host.data());
-      for (int j = 0; j < a.nsub; ++j) box_dispatch<R>(lds.data(), a, a.sub[j], tw, 0, 1);
+      for (int j = 0; j < a.nsub; ++j) box_dispatch<R>(lds.data(), a, a.sub[j], tw, 0, 1, sign);
       box_store<R>(lds.data(), HostGridStore<R>{out, S}, a, O, scale, tw, 0, 1, plan.tab_host.data());
     }
   }

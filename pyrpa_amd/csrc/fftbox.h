@@ -154,10 +154,47 @@ __host__ __device__ constexpr int cbrev(int i) {
   return r;
 }
 
-// In-register DFT of length RAD, natural order in and out; wr[m] = w_RAD^m.
-template <typename R, int RAD>
+// cos(2 pi m / 32) for m = 0..8; the rest of the period by symmetry
+__host__ __device__ constexpr double box_cos32_q(int m) {
+  return m == 0 ? 1.0
+       : m == 1 ? 0.98078528040323044912618223613424
+       : m == 2 ? 0.92387953251128675612818318939679
+       : m == 3 ? 0.83146961230254523707878837761791
+       : m == 4 ? 0.70710678118654752440084436210485
+       : m == 5 ? 0.55557023301960222474283081394853
+       : m == 6 ? 0.38268343236508977172845998403040
+       : m == 7 ? 0.19509032201612826784828486847702
+                : 0.0;
+}
+__host__ __device__ constexpr double box_cos32(int m) {
+  m &= 31;
+  if (m > 16) m = 32 - m;                                  // cos(2 pi - x) = cos x
+  return m <= 8 ? box_cos32_q(m) : -box_cos32_q(16 - m);   // cos(pi - x) = -cos x
+}
+__host__ __device__ constexpr double box_sin32(int m) { return box_cos32(8 - m); }
+
+// b w^m for w = exp(S 2 pi i / 32): the twiddle of a power-of-two DFT of
+// length <= 32 with compile-time values (m is a constant once the caller's
+// loops are unrolled), so a factor 1, -1 or +-i costs no arithmetic and a
+// factor (+-1 +- i)/sqrt(2) two multiplies
+template <typename R, int S>
+__host__ __device__ __forceinline__ cpx<R> box_twiddle32(cpx<R> b, int m) {
+  m &= 31;
+  const R c = (R)box_cos32(m), s = (R)(S * box_sin32(m));
+  if (m == 0) return b;
+  if (m == 16) return cpx<R>(-b.re, -b.im);
+  if (m == 8 || m == 24) return cpx<R>(-s * b.im, s * b.re);                   // s = +-1
+  if ((m & 7) == 4) return cpx<R>(c * (b.re - (s / c) * b.im), c * (b.im + (s / c) * b.re));  // s/c = +-1
+  return cpx<R>(c * b.re - s * b.im, c * b.im + s * b.re);
+}
+
+// In-register DFT of length RAD, natural order in and out, exponent sign S
+// (-1 forward, +1 backward).  Powers of two up to 32: radix-2 steps with
+// compile-time twiddles; other radices: wr[m] = w_RAD^m from the plan's table.
+template <typename R, int RAD, int S>
 __host__ __device__ __forceinline__ void box_dft(cpx<R> (&v)[RAD], const cpx<R>* __restrict__ wr) {
   if constexpr ((RAD & (RAD - 1)) == 0) {
+    static_assert(RAD <= 32, "compile-time twiddles cover lengths up to 32");
     cpx<R> u[RAD];
 #pragma unroll
     for (int i = 0; i < RAD; ++i) u[cbrev<RAD>(i)] = v[i];
@@ -167,8 +204,8 @@ __host__ __device__ __forceinline__ void box_dft(cpx<R> (&v)[RAD], const cpx<R>*
       for (int i = 0; i < RAD; i += len) {
 #pragma unroll
         for (int j = 0; j < len / 2; ++j) {
-          cpx<R> a = u[i + j], b = u[i + j + len / 2];
-          if (j != 0) b = b * wr[j * (RAD / len)];
+          const cpx<R> a = u[i + j];
+          const cpx<R> b = box_twiddle32<R, S>(u[i + j + len / 2], j * (32 / len));
           u[i + j] = a + b;
           u[i + j + len / 2] = a - b;
         }
@@ -297,7 +334,7 @@ __host__ __device__ __forceinline__ void box_load(cpx<R>* lds, const LD& ld, con
 }
 
 // ---- phase 2: one radix-RAD sub-pass over the whole box (in place in LDS)
-template <typename R, int RAD>
+template <typename R, int RAD, int S>
 __host__ __device__ __forceinline__ void box_subpass(cpx<R>* lds, const BoxArgs& a, const BoxSub& sp,
                                                      const cpx<R>* __restrict__ tw, int tid, int nthr) {
   const int n = sp.n;
@@ -307,37 +344,49 @@ __host__ __device__ __forceinline__ void box_subpass(cpx<R>* lds, const BoxArgs&
   const int ld = sp.ld;
   const cpx<R>* wr = tw + sp.wr_off;
   const cpx<R>* wl = tw + sp.tw_off;
+  // LDS distance of consecutive elements of a group (below 2^24: one 24-bit multiply)
+  const unsigned str = mul24((unsigned)s, (unsigned)ld);
   for (int g = tid; g < groups; g += nthr) {
     const unsigned gl = fdiv((unsigned)g, sp.flines);
     const unsigned line = (unsigned)g - mul24(gl, (unsigned)lines);
     const unsigned r1 = fdiv(line, sp.foe[0]), c2 = fdiv(r1, sp.foe[1]);
     const unsigned c0 = line - mul24(r1, sp.foe[0].d), c1 = r1 - mul24(c2, sp.foe[1].d);
-    const int base = (int)(mul24(c0, (unsigned)sp.os[0]) + mul24(c1, (unsigned)sp.os[1]) + mul24(c2, (unsigned)sp.os[2]));
+    const unsigned base = mul24(c0, (unsigned)sp.os[0]) + mul24(c1, (unsigned)sp.os[1]) + mul24(c2, (unsigned)sp.os[2]);
     const unsigned blk = fdiv(gl, sp.fs);
     const int jp = (int)(gl - mul24(blk, (unsigned)s));
     const int p0 = (int)mul24(blk, (unsigned)L) + jp;
     GMOD_BOX_CHECK(line < (unsigned)lines && p0 + s * (RAD - 1) < n, "sub-pass position");
-    GMOD_BOX_CHECK(base + (p0 + s * (RAD - 1)) * ld < a.lds_elems, "sub-pass lds");
+    GMOD_BOX_CHECK((int)base + (p0 + s * (RAD - 1)) * ld < a.lds_elems, "sub-pass lds");
+    // element j of the group at a0 + j str (additions only)
+    const unsigned a0 = base + mul24((unsigned)p0, (unsigned)ld);
     cpx<R> v[RAD];
+    {
+      unsigned ad = a0;
 #pragma unroll
-    for (int j = 0; j < RAD; ++j) v[j] = lds[base + (p0 + s * j) * ld];
-    box_dft<R, RAD>(v, wr);
+      for (int j = 0; j < RAD; ++j, ad += str) v[j] = lds[ad];
+    }
+    box_dft<R, RAD, S>(v, wr);
     if (s > 1) {
       // w_L^(jp k) = w_n^(jp k n / L): the dimension's table, in LDS when
       // the plan placed it there (sp.twl >= 0), else in global memory
-      const int step = jp * sp.tws;
+      const unsigned step = mul24((unsigned)jp, (unsigned)sp.tws);
       GMOD_BOX_CHECK(jp * (RAD - 1) < L, "sub-pass twiddle");
       if (sp.twl >= 0) {
         const cpx<R>* wt = lds + sp.twl;
+        unsigned ti = step;
 #pragma unroll
-        for (int k = 1; k < RAD; ++k) v[k] = v[k] * wt[step * k];
+        for (int k = 1; k < RAD; ++k, ti += step) v[k] = v[k] * wt[ti];
       } else {
+        unsigned ti = (unsigned)jp;
 #pragma unroll
-        for (int k = 1; k < RAD; ++k) v[k] = v[k] * wl[jp * k];
+        for (int k = 1; k < RAD; ++k, ti += (unsigned)jp) v[k] = v[k] * wl[ti];
       }
     }
+    {
+      unsigned ad = a0;
 #pragma unroll
-    for (int k = 0; k < RAD; ++k) lds[base + (p0 + s * k) * ld] = v[k];
+      for (int k = 0; k < RAD; ++k, ad += str) lds[ad] = v[k];
+    }
   }
 }
 
@@ -362,23 +411,37 @@ __host__ __device__ __forceinline__ BoxSub box_sub_at(const BoxArgs& a, int i) {
 
 // MAXR: the largest radix compiled in (a kernel that never runs radix 32
 // then does not reserve its registers)
-template <typename R, int MAXR = 32>
-__host__ __device__ __forceinline__ void box_dispatch(cpx<R>* lds, const BoxArgs& a, const BoxSub& sp,
-                                                      const cpx<R>* __restrict__ tw, int tid, int nthr) {
+template <typename R, int MAXR, int S>
+__host__ __device__ __forceinline__ void box_dispatch_pow2(cpx<R>* lds, const BoxArgs& a, const BoxSub& sp,
+                                                           const cpx<R>* __restrict__ tw, int tid, int nthr) {
   switch (sp.radix) {
-    case 2: box_subpass<R, 2>(lds, a, sp, tw, tid, nthr); break;
-    case 3: box_subpass<R, 3>(lds, a, sp, tw, tid, nthr); break;
-    case 4: box_subpass<R, 4>(lds, a, sp, tw, tid, nthr); break;
-    case 5: box_subpass<R, 5>(lds, a, sp, tw, tid, nthr); break;
-    case 7: box_subpass<R, 7>(lds, a, sp, tw, tid, nthr); break;
-    case 8: box_subpass<R, 8>(lds, a, sp, tw, tid, nthr); break;
+    case 2: box_subpass<R, 2, S>(lds, a, sp, tw, tid, nthr); break;
+    case 4: box_subpass<R, 4, S>(lds, a, sp, tw, tid, nthr); break;
+    case 8: box_subpass<R, 8, S>(lds, a, sp, tw, tid, nthr); break;
     case 16:
-      if constexpr (MAXR >= 16) box_subpass<R, 16>(lds, a, sp, tw, tid, nthr);
+      if constexpr (MAXR >= 16) box_subpass<R, 16, S>(lds, a, sp, tw, tid, nthr);
       break;
     case 32:
-      if constexpr (sizeof(R) == 4 && MAXR >= 32) box_subpass<R, 32>(lds, a, sp, tw, tid, nthr);
+      if constexpr (sizeof(R) == 4 && MAXR >= 32) box_subpass<R, 32, S>(lds, a, sp, tw, tid, nthr);
       break;
     default: break;
+  }
+}
+
+// sign: the exponent sign of the transform (-1 forward, +1 backward); the
+// power-of-two radices take it at compile time, the odd ones from the
+// sign's twiddle table tw
+template <typename R, int MAXR = 32>
+__host__ __device__ __forceinline__ void box_dispatch(cpx<R>* lds, const BoxArgs& a, const BoxSub& sp,
+                                                      const cpx<R>* __restrict__ tw, int tid, int nthr, int sign) {
+  switch (sp.radix) {
+    case 3: box_subpass<R, 3, 0>(lds, a, sp, tw, tid, nthr); break;
+    case 5: box_subpass<R, 5, 0>(lds, a, sp, tw, tid, nthr); break;
+    case 7: box_subpass<R, 7, 0>(lds, a, sp, tw, tid, nthr); break;
+    default:
+      if (sign < 0) box_dispatch_pow2<R, MAXR, -1>(lds, a, sp, tw, tid, nthr);
+      else box_dispatch_pow2<R, MAXR, 1>(lds, a, sp, tw, tid, nthr);
+      break;
   }
 }
 
@@ -617,7 +680,7 @@ __device__ __forceinline__ void box_fetch(cpx<R> (&v)[TM], const LD& ld, const B
 template <typename R, class LD, class ST>
 __device__ __forceinline__ void box_persistent(cpx<R>* lds, const LD& ld, const ST& st, const BoxArgs& a,
                                                const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale,
-                                               int tid) {
+                                               int tid, int sign) {
   constexpr int TM = BOX_TILE_REGS<R>;
   const unsigned nb = (unsigned)a.blocks();
   unsigned c[4];
@@ -645,7 +708,7 @@ __device__ __forceinline__ void box_persistent(cpx<R>* lds, const LD& ld, const 
     const unsigned nxt = box_next_live(st, a, b + gridDim.x, nb);
     for (int i = 0; i < a.nsub; ++i) {
       if (i == a.pf_at && nxt < nb) box_fetch<R, TM>(v, ld, a, nxt, nb, csp, ct, tsp, tt);
-      box_dispatch<R>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS);
+      box_dispatch<R>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS, sign);
       __syncthreads();
     }
     if (a.pf_at >= a.nsub && nxt < nb) box_fetch<R, TM>(v, ld, a, nxt, nb, csp, ct, tsp, tt);
@@ -671,7 +734,7 @@ struct GridLoad;  // fft.h
 template <typename R, int MAXR, class AC>
 __global__ void __launch_bounds__(BOX_THREADS)
 box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC ac, const BoxArgs* __restrict__ ap,
-                 const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale) {
+                 const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale, int sign) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const BoxArgs& a = *ap;
@@ -734,7 +797,7 @@ box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC a
       if constexpr (PF) {
         if (i == a.pf_at && b2 < nb) box_fetch<R, TM>(v, ld2, a, b2, nb, csp, ct, tsp, tt);
       }
-      box_dispatch<R, MAXR>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS);
+      box_dispatch<R, MAXR>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS, sign);
       __syncthreads();
     }
     if constexpr (PF) {
@@ -773,12 +836,12 @@ box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC a
 template <typename R, class LD, class ST>
 __global__ void __launch_bounds__(BOX_THREADS)
 box_fft_kernel(LD ld, ST st, const BoxArgs* __restrict__ ap, const cpx<R>* __restrict__ tw,
-               const int* __restrict__ tabs, R scale) {
+               const int* __restrict__ tabs, R scale, int sign) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const BoxArgs& a = *ap;
   if (a.tiled) {
-    box_persistent<R>(lds, ld, st, a, tw, tabs, scale, threadIdx.x);
+    box_persistent<R>(lds, ld, st, a, tw, tabs, scale, threadIdx.x, sign);
     return;
   }
   const unsigned lb = box_remap(blockIdx.x, gridDim.x, a.xcd_group);
@@ -789,7 +852,7 @@ box_fft_kernel(LD ld, ST st, const BoxArgs* __restrict__ ap, const cpx<R>* __res
   box_load<R>(lds, ld, a, O, tid, BOX_THREADS, tabs);
   __syncthreads();
   for (int i = 0; i < a.nsub; ++i) {
-    box_dispatch<R>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS);
+    box_dispatch<R>(lds, a, box_sub_at(a, i), tw, tid, BOX_THREADS, sign);
     __syncthreads();
   }
   box_store<R>(lds, st, a, O, scale, tw, tid, BOX_THREADS, tabs);
