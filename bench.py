@@ -51,12 +51,13 @@ HBM_PEAK_GBS = 8000.0
 # (libs/src/fchi.f90:852-892; sw_tail as main.py:165 sets it)
 SW_TAIL = True
 # HBM bytes per FP32 FFT-pass launch (box_fft_kernel and box_accum_kernel) on
-# the 4-orbital 32x32x16 x 1024-Matsubara workload, averaged over all its FP32
-# pass dispatches: rocprofv3 FETCH_SIZE x 2
-# + WRITE_SIZE (KiB x 1024), both calibrated on a device copy of known size
-# (FETCH_SIZE reads half the bytes on this device).  Measured offline: counter
-# runs cannot be part of the timed run.
-FP32_BOX_TRAFFIC_BYTES = {3: 1249746321}
+# the 4-orbital 32x32x16 x 1024-Matsubara workload (tail-corrected chi0, warm
+# start), averaged over the FP32 pass dispatches of the timed mixed solve:
+# rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE (KiB x 1024), both calibrated on a
+# device copy of known size (FETCH_SIZE reads half the bytes on this device),
+# per functor class, weighted by the classes' dispatch counts in the full
+# solve.  Measured offline: counter runs cannot be part of the timed run.
+FP32_BOX_TRAFFIC_BYTES = {3: 1812791481}
 
 
 def log(*a):

@@ -93,6 +93,51 @@ void gmod_get_chis_chic_all(gmod_z* chis, gmod_z* chic, const gmod_z* chi, const
 int gmod_eig_or_tr_chi_noalloc(gmod_z* chiqout, const gmod_z* chi, const int64_t* invk, int64_t Nkall, int64_t Nk,
                                int64_t Nchi, int sw_eig);
 
+/* ---------------- real-frequency susceptibility, Pade continuation ------- */
+/* libs/src/fchi.f90:1-92 (bind name get_qshift_) [libs/flibs/_hamiltonian.py:330-341]:
+ * qshift(Nk) = 1-based index of k + q in klist(3,Nk) (periodic in [0,1)). */
+void get_qshift_(const double* qpoint, const double* klist, int64_t* qshift, const int64_t* Nk);
+/* libs/src/fchi.f90:206-240 (calc_chi fchi.f90:98-157) [libs/flibs/_susceptibility.py:271-293]:
+ * chi(Nchi,Nchi,Nw) = chi0(q, wl + i idelta) by the band sum over k and band pairs. */
+void get_chi_irr(gmod_z* chi, const gmod_z* uni, const double* eig, const double* ffermi, const int64_t* qshift,
+                 const int64_t* ol, const double* wl, const int64_t* Nchi, const int64_t* Norb, const int64_t* Nk,
+                 const int64_t* Nw, const double* idelta, const double* eps, const double* temp);
+/* libs/src/fchi.f90:242-317 [libs/flibs/_susceptibility.py:295-324]: traces of chi_s and chi0
+ * (orbital-diagonal pairs) at w = ecut on the q plane; trchis, trchi (Ny,Nx). */
+void chiq_map(gmod_z* trchis, gmod_z* trchi, const gmod_z* uni, const double* eig, const double* ffermi,
+              const double* klist, const double* Smat, const int64_t* ol, const double* temp, const double* ecut,
+              const double* idelta, const double* eps, const int64_t* Nx, const int64_t* Ny, const int64_t* Nk,
+              const int64_t* Norb, const int64_t* Nchi);
+/* libs/src/fchi.f90:160-204 [libs/flibs/_susceptibility.py:326-344] */
+void get_tr_chi(gmod_z* trchis, gmod_z* trchi0, gmod_z* chis_orb, const gmod_z* chis, const gmod_z* chi0,
+                const int64_t* olist, const int64_t* Nw, const int64_t* Nchi, const int64_t* Norb);
+/* libs/src/fpade.f90:1-42 (bind name get_a_) [libs/flibs/_green.py:177-188]: continued-fraction
+ * coefficients a(Np) of the data inp_data(Np) at the points xn(Np). */
+void get_a_(gmod_z* a, const gmod_z* xn, const gmod_z* inp_data, const int64_t* Np);
+/* libs/src/fpade.f90:44-85 (bind name get_qp_) [libs/flibs/_green.py:190-204]: numerator P(Nw)
+ * and denominator Q(Nw) of the continued fraction at wlist(Nw). */
+void get_qp_(gmod_z* P, gmod_z* Q, const gmod_z* a, const gmod_z* xn, const gmod_z* wlist, const int64_t* Nw,
+             const int64_t* Np);
+/* libs/src/fpade.f90:87-106 (bind name pade_analytic_continuation_arrays_):
+ * arrayout(Nk,Nw) = Pade continuation of every row of arrayin(Nk,Niw). */
+void pade_analytic_continuation_arrays_(const gmod_z* arrayin, gmod_z* arrayout, const gmod_z* iwlist,
+                                        const gmod_z* wlist, const int64_t* Nk, const int64_t* Niw,
+                                        const int64_t* Nw);
+/* libs/src/fpade.f90:108-129 [libs/flibs/_green.py:206-220]: arrayout(Nw,Nk) = sum over the
+ * orbital-diagonal blocks of arrayin(Nk,Niw,Norb,Norb) of their continuations. */
+void pade_with_trace(const gmod_z* arrayin, gmod_z* arrayout, const gmod_z* iwlist, const gmod_z* wlist,
+                     const int64_t* Nk, const int64_t* Niw, const int64_t* Nw, const int64_t* Norb);
+
+/* ---------------- Wannier-R output --------------------------------------- */
+/* libs/plibs/_wannier_io.py:48-99 (_irr_to_full_kgrid, the ifftn of _wannier_all_nonzero),
+ * the transform behind output_self_wannier / output_gap_wannier (:145-216):
+ * out(Nx,Ny,Nz,Ncut,Norb,Norb) = (1/Nkall) sum_k e^{+2 pi i k.R} data(k) for the first Ncut
+ * frequencies of data(Nk,Nw,Norb,Norb).  kmap, invk (3,Nkall) as gen_irr_k_TRS returns them
+ * (0-based grid indices; 1-based irreducible index and time-reversal flag).  full != 0: data
+ * hold every grid point (Nk = Nkall, placed through kmap alone). */
+void gmod_wannier_r(gmod_z* out, const gmod_z* data, const int64_t* kmap, const int64_t* invk, int64_t Nx,
+                    int64_t Ny, int64_t Nz, int64_t Nw, int64_t Nk, int64_t Norb, int64_t Ncut, int full);
+
 /* ---------------- Green functions ---------------------------------------- */
 /* libs/src/fself.f90:1-41 (bind name gen_green0_) [libs/flibs/_green.py:80-97] */
 void gen_green0_(gmod_z* Gk, const double* eig, const gmod_z* uni, const double* mu, const double* temp,

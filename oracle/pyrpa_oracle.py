@@ -1291,6 +1291,155 @@ def remap_gap(delta0, plist, invk, gap_sym):
 
 
 # --------------------------------------------------------------------------- #
+#  real-frequency susceptibility (fchi.f90:1-317), Pade continuation (fpade.f90)
+# --------------------------------------------------------------------------- #
+def get_qshift(klist, qpoint):
+    """1-based index of k + q (periodic in [0,1)) for every k of klist[Nk,3]
+    (fchi.f90:1-92): coordinates compared on a 2^20 grid; a shifted point off
+    the mesh falls back to index 1."""
+    scale = 1 << 20
+    kl = np.asarray(klist, dtype=np.float64)
+
+    def keys(x):
+        i = np.floor(x * scale + 0.5).astype(np.int64) % scale
+        return i[:, 0] + i[:, 1] * scale + i[:, 2] * scale * scale
+    t = kl + np.asarray(qpoint, dtype=np.float64)[None, :]
+    t = np.where(t >= 1.0, t - 1.0, np.where(t < 0.0, t + 1.0, t))
+    where = {int(k): j for j, k in enumerate(keys(kl))}
+    return np.array([where.get(int(k), 0) + 1 for k in keys(t)], dtype=np.int64)
+
+
+def _calc_chi(uni, eig, ffermi, olist, temp, qshift, w, idelta, eps):
+    """chi0(q, w + i delta) as the Fortran matrix chi(Nchi,Nchi) (calc_chi,
+    fchi.f90:98-157): (1/Nk) sum_k sum_{l at k+q, m at k} weight B_i A_j with
+    B_i = conj(U_{o_i1,l}(k+q)) U_{o_i2,m}(k), A_j = U_{o_j1,l}(k+q) conj(U_{o_j2,m}(k)),
+    weight = (f_l(k+q) - f_m(k)) / (w + e_m(k) - e_l(k+q) + i delta), f (1-f)/T for
+    equal energies at w = 0, dropped where |f_l(k+q) - f_m(k)| <= eps."""
+    U = np.asarray(uni)                   # [k, band, orb]
+    e = np.asarray(eig)
+    f = np.asarray(ffermi)
+    q0 = np.asarray(qshift) - 1
+    Uq, eq, fq = U[q0], e[q0], f[q0]
+    dE = e[:, None, :] - eq[:, :, None]   # [k, l, m] = e_m(k) - e_l(k+q)
+    df = fq[:, :, None] - f[:, None, :]
+    tsafe = max(temp, 1e-12)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        wt = np.where(np.abs(df) > eps, df / (w + dE + 1j * idelta), 0.0)
+    if abs(w) < 1e-12:
+        deg = np.abs(dE) < 1e-9
+        wt = np.where(deg, (f * (1 - f) / tsafe)[:, None, :] + 0j, wt)
+    o1 = np.asarray(olist)[:, 0] - 1
+    o2 = np.asarray(olist)[:, 1] - 1
+    B = np.conj(Uq[:, :, None, o1]) * U[:, None, :, o2]   # [k, l, m, i]
+    A = Uq[:, :, None, o1] * np.conj(U[:, None, :, o2])   # [k, l, m, j]
+    return np.einsum('klm,klmi,klmj->ij', wt, B, A) / len(e)
+
+
+def get_chi_irr(uni, eig, ffermi, qshift, olist, wlist, idelta, temp):
+    """[Nw, Nchi, Nchi] (Python view of the Fortran chi(Nchi,Nchi,Nw));
+    fchi.f90:206-240, wrapper _susceptibility.py:271-293 (eps = idelta 1e-3)."""
+    eps = idelta * 1e-3
+    return np.array([_calc_chi(uni, eig, ffermi, olist, temp, qshift, w, idelta, eps).T for w in wlist])
+
+
+def chis_qmap(uni, eig, ffermi, klist, Smat, olist, Nx, Ny, temp, ecut, idelta):
+    """(chis, chi) traces over the orbital-diagonal pairs on the (qx, qy)
+    plane at w = ecut, [Nx, Ny] each (fchi.f90:242-317, wrapper
+    _susceptibility.py:295-324): chi_s = (I - chi0 S)^-1 chi0 per q."""
+    eps = idelta * 1e-3
+    S = np.asarray(Smat).T
+    ol = np.asarray(olist)
+    diag = np.nonzero(ol[:, 0] == ol[:, 1])[0]
+    chis = np.zeros((Nx, Ny), dtype=complex)
+    chi = np.zeros((Nx, Ny), dtype=complex)
+    for i in range(Nx):
+        for j in range(Ny):
+            qs = get_qshift(klist, [i / Nx, j / Ny, 0.0])
+            c = _calc_chi(uni, eig, ffermi, olist, temp, qs, ecut, idelta, eps)
+            cs = np.linalg.solve(np.eye(len(S)) - c @ S, c)
+            chis[i, j] = cs[np.ix_(diag, diag)].sum()
+            chi[i, j] = c[np.ix_(diag, diag)].sum()
+    return chis, chi
+
+
+def get_tr_chi(chis, chi0, olist):
+    """(trchis[Nw], trchi0[Nw], chis_orb[Nw, Norb+2]) (fchi.f90:160-204)."""
+    S = np.asarray(chis).T                 # Fortran chis(k, j, w)
+    C = np.asarray(chi0).T
+    ol = np.asarray(olist)
+    Nchi, Nw = S.shape[0], S.shape[2]
+    Norb = int(ol.max())
+    ts = np.zeros(Nw, dtype=complex)
+    t0 = np.zeros(Nw, dtype=complex)
+    so = np.zeros((Norb + 2, Nw), dtype=complex)
+    for i in range(Nw):
+        for j in range(Nchi):
+            if ol[j, 0] != ol[j, 1]:
+                continue
+            for k in range(Nchi):
+                if ol[k, 0] != ol[k, 1]:
+                    continue
+                ts[i] += S[k, j, i]
+                t0[i] += C[k, j, i]
+                if ol[j, 0] == ol[k, 0]:
+                    so[ol[j, 0] - 1, i] = S[k, j, i]
+                if ol[k, 0] == 2 and ol[j, 0] == 3:
+                    so[Norb, i] = S[k, j, i]
+                if ol[k, 0] == 2 and ol[j, 0] == 4:
+                    so[Norb + 1, i] = S[k, j, i]
+    return ts, t0, np.ascontiguousarray(so.T)
+
+
+def get_a(inp_data, xlist):
+    """Continued-fraction (Thiele) coefficients of the data at the points
+    (fpade.f90:1-42): g_1 = data, g_i(z_j) = (g_{i-1}(z_{i-1}) - g_{i-1}(z_j))
+    / ((z_j - z_{i-1}) g_{i-1}(z_j)) (0 where g_{i-1}(z_j) = 0), a_i = g_i(z_i)."""
+    g0 = np.asarray(inp_data, dtype=complex).copy()
+    x = np.asarray(xlist, dtype=complex)
+    Np = len(g0)
+    a = np.zeros(Np, dtype=complex)
+    a[0] = g0[0]
+    with np.errstate(divide='ignore', invalid='ignore'):
+        for i in range(1, Np):
+            z = g0 == 0
+            den = np.where(z, 1.0, (x - x[i - 1]) * g0)
+            g0 = np.where(z, 0.0, (g0[i - 1] - g0) / den)
+            a[i] = g0[i]
+    return a
+
+
+def get_QP(a, xlist, wlist):
+    """(Q, P): denominator and numerator of the continued fraction at wlist
+    (fpade.f90:44-85; the wrapper returns them in this order)."""
+    a = np.asarray(a, dtype=complex)
+    x = np.asarray(xlist, dtype=complex)
+    z = np.asarray(wlist, dtype=complex)
+    P0, P1 = np.zeros_like(z), np.full_like(z, a[0])
+    Q0, Q1 = np.ones_like(z), np.ones_like(z)
+    P, Q = P1.copy(), Q1.copy()
+    for i in range(1, len(a)):
+        fct = a[i] * (z - x[i - 1])
+        P = P1 + fct * P0
+        Q = Q1 + fct * Q0
+        P0, P1, Q0, Q1 = P1, P, Q1, Q
+    return Q, P
+
+
+def pade_with_trace(A, iwlist, wlist):
+    """[Nk, Nw]: sum over the orbital-diagonal blocks of A[Norb,Norb,Niw,Nk]
+    of their Pade continuations (fpade.f90:87-129)."""
+    A = np.asarray(A)
+    Norb, Nk = A.shape[0], A.shape[3]
+    out = np.zeros((Nk, len(wlist)), dtype=complex)
+    for o in range(Norb):
+        for k in range(Nk):
+            a = get_a(A[o, o, :, k], iwlist)
+            Q, P = get_QP(a, iwlist, wlist)
+            out[k] += P / Q
+    return out
+
+
+# --------------------------------------------------------------------------- #
 #  sigma.bin (fself.f90:742-780): Fortran unformatted sequential records
 # --------------------------------------------------------------------------- #
 def write_sigma_bin(path, mu, mu_old, sigmak, temp, Nw, Norb, Nk):
@@ -1302,3 +1451,351 @@ def write_sigma_bin(path, mu, mu_old, sigmak, temp, Nw, Norb, Nk):
         f.write(rec(struct.pack('<d', mu_old)))
         f.write(rec(np.ascontiguousarray(sigmak, dtype=np.complex128).tobytes()))
         f.write(rec(struct.pack('<4d', temp, float(Nw), float(Norb), float(Nk))))
+
+
+# --------------------------------------------------------------------------- #
+#  Wannier-R output (libs/plibs/_wannier_io.py)
+# --------------------------------------------------------------------------- #
+def wannier_r(data, kmap, invk, Nx, Ny, Nz, N_cut, full=False):
+    """data[No,No,Nw,Nk] -> data_R[No,No,min(N_cut,Nw),Nx,Ny,Nz].
+
+    The irreducible values are placed on the grid through kmap, conjugated
+    where invk flags the time-reversed partner (_wannier_io.py:48-68; full:
+    data already on every grid point in kmap order, :195-199), then numpy's
+    normalised inverse FFT over the three k axes, first N_cut frequencies
+    (:81-83)."""
+    data = np.asarray(data)
+    No, _, Nw, _ = data.shape
+    grid = np.zeros((No, No, Nw, Nx, Ny, Nz), dtype=np.complex128)
+    ix, iy, iz = kmap[:, 0], kmap[:, 1], kmap[:, 2]
+    if full:
+        grid[:, :, :, ix, iy, iz] = data
+    else:
+        v = data[..., invk[:, 0] - 1]
+        tr = invk[:, 1] != 0
+        v[..., tr] = np.conj(v[..., tr])
+        grid[:, :, :, ix, iy, iz] = v
+    return np.fft.ifftn(grid, axes=(-3, -2, -1))[:, :, :min(N_cut, Nw)]
+
+
+def wannier_select(data_R, Nx, Ny, Nz, zero_tol):
+    """Zero-tolerance cut and nonzero-R selection of _wannier_io.py:84-99
+    (on a copy): (data_out[No,No,Ncut,Nr], rvec[Nr,3])."""
+    d = np.array(data_R)
+    tol = np.abs(d).max() * zero_tol
+    d.real[np.abs(d.real) < tol] = 0.0
+    d.imag[np.abs(d.imag) < tol] = 0.0
+    sel = np.abs(d).max(axis=(0, 1, 2)) > 0
+    ix, iy, iz = np.where(sel)
+    rv = np.stack([np.where(ix <= Nx // 2, ix, ix - Nx), np.where(iy <= Ny // 2, iy, iy - Ny),
+                   np.where(iz <= Nz // 2, iz, iz - Nz)], axis=1).astype(np.int64)
+    return d[:, :, :, ix, iy, iz], rv
+
+
+# --------------------------------------------------------------------------- #
+#  spin-orbit coupling (fchi0withsoc.f90, fselfwithsoc.f90, fUJ.f90:109-251,
+#  367-399).  Orbitals carry spin: slist[a] = +-1, invs[a] (1-based) is the
+#  spin-flipped partner of a; olist runs over all Norb^2 pairs.
+# --------------------------------------------------------------------------- #
+def soc_spin_lists(Norb):
+    """Default slist / invs of the reference driver (main.py:1373-1385): the
+    first (Norb+1)//2 orbitals spin up, the rest down, partners in order."""
+    slist = np.ones(Norb, dtype=np.int64)
+    slist[(Norb + 1) // 2:] = -1
+    invs = np.concatenate([np.arange((Norb + 1) // 2, Norb), np.arange((Norb + 1) // 2)]) + 1
+    return slist, invs.astype(np.int64)
+
+
+def get_vmat_soc(olist, slist, site, invs, U, J):
+    """SOC interaction vertex (fUJ.f90:109-180).  Python V[i,j] is Fortran
+    Vmat(j,i), the value the reference assigns for the pair (i, j)."""
+    ol = olist_f(olist)
+    sl = np.asarray(slist)
+    inv = np.asarray(invs) - 1
+    Nchi = len(ol)
+    V = np.zeros((Nchi, Nchi))
+    Up = U - 2 * J
+    for i in range(Nchi):
+        for j in range(Nchi):
+            if site[i] != site[j]:
+                continue
+            a, b = ol[i]
+            c, d = ol[j]
+            if sl[a] == sl[b] and sl[c] == sl[d]:
+                if sl[a] == sl[c]:                       # same-spin density channel
+                    if a == d and c == b:
+                        continue
+                    if a == b and c == d:
+                        V[i, j] = Up - J
+                    elif a == c and b == d:
+                        V[i, j] = J - Up
+                else:                                    # opposite-spin density channel
+                    if a == b and c == d:
+                        V[i, j] = U if a == inv[c] else Up
+                    elif a == inv[c] and b == inv[d]:
+                        V[i, j] = J
+                    elif a == inv[d] and c == inv[b]:
+                        V[i, j] = J
+            elif sl[a] == sl[c] and sl[b] == sl[d]:      # spin-flip channel
+                if a == inv[b] and c == inv[d]:
+                    V[i, j] = -U if a == c else -J
+                elif a == c and b == d:
+                    V[i, j] = -Up
+                elif a == inv[d] and c == inv[b]:
+                    V[i, j] = -J
+    return V
+
+
+def get_chi_map_soc(olist, invs):
+    """Exchange-symmetry partners chi_{1234} = sgn chi*_{-4-3-2-1} with spin
+    flips (fchi0withsoc.f90:216-279): cmap[m1,l1] = (m2,l2), irr[i] = (l1,m1),
+    0-based."""
+    ol = olist_f(olist)
+    inv = np.asarray(invs) - 1
+    Nchi = len(ol)
+    cmap = np.zeros((Nchi, Nchi, 2), dtype=np.int64)
+    cirr = np.zeros((Nchi, Nchi), dtype=np.int64)
+    for l1 in range(Nchi):
+        for m1 in range(Nchi):
+            t1 = (ol[l1, 0], ol[l1, 1], ol[m1, 0], ol[m1, 1])
+            found = False
+            for l2 in range(Nchi):
+                for m2 in range(Nchi):
+                    if t1 == (inv[ol[m2, 1]], inv[ol[m2, 0]], inv[ol[l2, 1]], inv[ol[l2, 0]]):
+                        cmap[m1, l1] = (m2, l2)
+                        if cirr[m2, l2] == 0:
+                            cirr[m1, l1] = 1
+                        found = True
+                        break
+                if found:
+                    break
+    irr = [(l1, m1) for l1 in range(Nchi) for m1 in range(Nchi) if cirr[m1, l1] == 1]
+    return cmap, np.array(irr, dtype=np.int64).reshape(-1, 2)
+
+
+def soc_signs(olist, slist):
+    """sgnsig(i,j) = s_i s_j and sgnsig2(l,m) = s_l1 s_l2 s_m1 s_m2
+    (fchi0withsoc.f90:136-154)."""
+    ol = olist_f(olist)
+    s = np.asarray(slist, dtype=float)
+    sg = np.outer(s, s)
+    p = s[ol[:, 0]] * s[ol[:, 1]]
+    return sg, np.outer(p, p)
+
+
+def get_invschi(olist, invs):
+    """invschi(l) = 1-based index of the spin-flipped pair of l, or 0
+    (fchi0withsoc.f90:200-213)."""
+    ol = olist_f(olist)
+    inv = np.asarray(invs) - 1
+    out = np.zeros(len(ol), dtype=np.int64)
+    for l in range(len(ol)):
+        hit = np.nonzero((ol[:, 0] == inv[ol[l, 0]]) & (ol[:, 1] == inv[ol[l, 1]]))[0]
+        if len(hit):
+            out[l] = hit[0] + 1
+    return out
+
+
+def _full_green_soc(Gf, a, b, sg, inv, kmap, invk, Nx, Ny, Nz):
+    """Full-grid G_ab(k, slot) with the time-reversal rule of a spinful
+    basis (fchi0withsoc.f90:57-75): G_ab(-k, w) = s_a s_b G_{-b,-a}(k, w),
+    -w through G(k,-w) = G(k,w)^dagger."""
+    Nk, Nw = Gf.shape[:2]
+    M = 2 * Nw
+    out = np.zeros((Nx * Ny * Nz, M), dtype=complex)
+    irr = invk[:, 0] - 1
+    fl = invk[:, 1]
+    sp = _spatial(kmap, Nx, Ny, Nz)
+    d = fl == 0
+    r = fl == 1
+    out[sp[d], :Nw] = Gf[irr[d], :, a, b]
+    out[sp[d], Nw:] = np.conj(Gf[irr[d], :, b, a])[:, ::-1]
+    out[sp[r], :Nw] = sg[b, a] * Gf[irr[r], :, inv[b], inv[a]]
+    out[sp[r], Nw:] = (sg[a, b] * np.conj(Gf[irr[r], :, inv[a], inv[b]]))[:, ::-1]
+    return out.reshape(Nx, Ny, Nz, M)
+
+
+def get_chi0_conv_soc(Gk, kmap, invk, invs, olist, slist, temp, Nx, Ny, Nz):
+    """chi0 of a spinful basis by FFT convolution (fchi0withsoc.f90:1-109).
+    Returns chi[Nchi,Nchi,Nw,Nk]."""
+    Gf = np.asarray(Gk).T
+    Nk, Nw = Gf.shape[:2]
+    ol = olist_f(olist)
+    inv = np.asarray(invs) - 1
+    sg, sg2 = soc_signs(olist, slist)
+    cmap, irr = get_chi_map_soc(olist, invs)
+    Nchi = len(ol)
+    weight = temp / len(kmap)
+    chif = np.zeros((Nk, Nw, Nchi, Nchi), dtype=complex)
+    for l, m in irr:
+        g13 = fft4(_full_green_soc(Gf, ol[l, 0], ol[m, 0], sg, inv, kmap, invk, Nx, Ny, Nz))
+        g42 = fft4(_full_green_soc(Gf, ol[m, 1], ol[l, 1], sg, inv, kmap, invk, Nx, Ny, Nz))
+        tmp = ifft4(-g13 * _reverse(g42))
+        chif[:, :, m, l] = _threshold(_gather(tmp, kmap, invk, Nk, Nw) * weight)
+        pm, pl = cmap[m, l]
+        chif[:, :, pm, pl] = sg2[m, l] * np.conj(chif[:, :, m, l])
+    return np.ascontiguousarray(chif.T)
+
+
+def ckchi_soc(chi, Vmat):
+    """max_q max Re eig(-chi0(q,0) V) (fselfwithsoc.f90:191-226)."""
+    chif = np.asarray(chi).T
+    Vf = np.asarray(Vmat).T
+    return max(np.linalg.eigvals(-chif[i, 0] @ Vf).real.max() for i in range(chif.shape[0]))
+
+
+def get_chi0_soc(Vmat, Gk, olist, slist, kmap, invk, invs, temp, Nx, Ny, Nz):
+    """(chi, sgnsig, sgnsig2, invschi) as the reference wrapper returns them
+    (fchi0withsoc.f90:111-214, _susceptibility.py:496-528)."""
+    chi = get_chi0_conv_soc(Gk, kmap, invk, invs, olist, slist, temp, Nx, Ny, Nz)
+    sg, sg2 = soc_signs(olist, slist)
+    return chi, sg, sg2, get_invschi(olist, invs)
+
+
+def get_V_soc_flex(chi, Vmat):
+    """-V + V (I + chi0 V)^-1 chi0 V per (q, i nu) (fUJ.f90:367-399)."""
+    chif = np.asarray(chi).T
+    Vf = np.asarray(Vmat).T
+    AV = chif @ Vf
+    X = np.linalg.solve(np.eye(len(Vf)) + AV, AV)
+    return np.ascontiguousarray((-Vf + Vf @ X).T)
+
+
+def calc_sigma_soc(Gk, Vsigma, Vmat, kmap, invk, invs, olist, slist, temp, Nx, Ny, Nz):
+    """Sigma = (T/Nkall) sum over all Nchi^2 pairs of V (*) G in a spinful
+    basis, the nu -> infinity slot -Vmat (fselfwithsoc.f90:540-667)."""
+    Gf = np.asarray(Gk).T
+    Vf = np.asarray(Vsigma).T
+    Vm = np.asarray(Vmat).T
+    ol = olist_f(olist)
+    inv = np.asarray(invs) - 1
+    sg, _ = soc_signs(olist, slist)
+    Nk, Nw, No, _ = Gf.shape
+    Nchi = len(ol)
+    sig = np.zeros((Nk, Nw, No, No), dtype=complex)
+    gft = {}
+    for l in range(Nchi):
+        for m in range(Nchi):
+            v = fft4(_full_boson(Vf, m, l, -Vm[m, l], kmap, invk, Nx, Ny, Nz))
+            key = (ol[m, 1], ol[l, 1])
+            if key not in gft:
+                gft[key] = fft4(_full_green_soc(Gf, key[0], key[1], sg, inv, kmap, invk, Nx, Ny, Nz))
+            sig[:, :, ol[m, 0], ol[l, 0]] += _gather(ifft4(v * gft[key]), kmap, invk, Nk, Nw)
+    sig *= temp / len(kmap)
+    return np.ascontiguousarray(_threshold(sig).T)
+
+
+def calc_sigma_soc_direct(Gk, Vsigma, Vmat, kmap, invk, invs, olist, slist, temp, Nx, Ny, Nz):
+    """The same Sigma as a direct circular sum over (q, nu) (independent check)."""
+    Gf = np.asarray(Gk).T
+    Vf = np.asarray(Vsigma).T
+    Vm = np.asarray(Vmat).T
+    ol = olist_f(olist)
+    inv = np.asarray(invs) - 1
+    sg, _ = soc_signs(olist, slist)
+    Nk, Nw, No, _ = Gf.shape
+    M = 2 * Nw
+    sig = np.zeros((Nk, Nw, No, No), dtype=complex)
+    d = np.nonzero(invk[:, 1] == 0)[0]
+    for l in range(len(ol)):
+        for m in range(len(ol)):
+            v = _full_boson(Vf, m, l, -Vm[m, l], kmap, invk, Nx, Ny, Nz)
+            g = _full_green_soc(Gf, ol[m, 1], ol[l, 1], sg, inv, kmap, invk, Nx, Ny, Nz)
+            for i in d:
+                kx, ky, kz = kmap[i]
+                gs = g[(kx - np.arange(Nx)) % Nx][:, (ky - np.arange(Ny)) % Ny][:, :, (kz - np.arange(Nz)) % Nz]
+                for n in range(Nw):
+                    sig[invk[i, 0] - 1, n, ol[m, 0], ol[l, 0]] += (v * gs[..., (n - np.arange(M)) % M]).sum()
+    sig *= temp / len(kmap)
+    return np.ascontiguousarray(sig.T)
+
+
+def get_chis_chic_soc(chi, Vmat, olist, slist, invs):
+    """Charge, longitudinal and transverse spin susceptibilities at the first
+    boson frequency (fchi0withsoc.f90:281-326; orb_list of
+    _susceptibility.py:545-556).  Returns (chic, chiszz, chispm)
+    [Nchi/4, Nchi/4, Nk]."""
+    chif = np.asarray(chi).T
+    Vf = np.asarray(Vmat).T
+    ol = olist_f(olist)
+    sl = np.asarray(slist)
+    inv = np.asarray(invs) - 1
+    Nk = chif.shape[0]
+    Nchi = len(ol)
+    up = np.nonzero(sl == 1)[0]
+    nu = len(up)
+    # pair n = i nu + j of the spin-up orbitals is (up[j], up[i])
+    pos = {(up[j], up[i]): i * nu + j for i in range(nu) for j in range(nu)}
+    orb = np.zeros(Nchi, dtype=np.int64)
+    for i in range(Nchi):
+        a = ol[i, 0] if sl[ol[i, 0]] == 1 else inv[ol[i, 0]]
+        b = ol[i, 1] if sl[ol[i, 1]] == 1 else inv[ol[i, 1]]
+        orb[i] = pos[(a, b)]
+    n4 = Nchi // 4
+    chic = np.zeros((Nk, n4, n4), dtype=complex)
+    szz = np.zeros((Nk, n4, n4), dtype=complex)
+    spm = np.zeros((Nk, n4, n4), dtype=complex)
+    s1, s2 = sl[ol[:, 0]], sl[ol[:, 1]]
+    for q in range(Nk):
+        A = chif[q, 0]
+        X = np.linalg.solve(np.eye(Nchi) + A @ Vf, A)
+        for l in range(Nchi):
+            for m in range(Nchi):
+                if s1[l] == s1[m] and s2[l] == s2[m] and s1[l] == -s2[l] and s1[l] == 1:
+                    spm[q, orb[l], orb[m]] += X[m, l]
+                if s1[l] == s2[l] and s1[m] == s2[m]:
+                    chic[q, orb[l], orb[m]] += X[m, l]
+                    szz[q, orb[l], orb[m]] += X[m, l] if s1[l] == s1[m] else -X[m, l]
+    return (np.ascontiguousarray((0.5 * chic).T), np.ascontiguousarray((0.25 * szz).T),
+            np.ascontiguousarray(spm.T))
+
+
+def mkself_soc(Vmat, kmap, invk, invs, olist, slist, hamk, eig, uni, mu, fill, temp, Nw, Nx, Ny, Nz,
+               sw_out, sw_in, sub_sigma=1, scf_loop=300, eps=1.0e-4, pp=0.3, m_diis=5, sw_rescale=False,
+               sigma_scale=1.0, sigma_seed=None, return_info=False):
+    """FLEX loop in a spinful basis (fselfwithsoc.f90:1-539; wrapper
+    _selfenergy.py:129-174): the loop of mkself with the spinful chi0, the
+    Stoner factor of -chi0 V, the vertex -V + V chi_RPA V and no tail."""
+    Hf = np.asarray(hamk).T
+    No, Nk = Hf.shape[0], Hf.shape[2]
+    fl = _Flex(hamk, eig, fill, temp, m_diis, No, Nk, Nw)
+    if sw_in:
+        sig_py, mu, mu_old = sigma_seed
+        sigf = np.asarray(sig_py).T.copy() * sigma_scale
+        sig0f = sigf.copy()
+        Gf = np.asarray(gen_green(np.ascontiguousarray(sigf.T), hamk, mu, temp)).T
+    else:
+        mu_old = mu * 1.2
+        sigf = np.zeros((Nk, Nw, No, No), dtype=complex)
+        sig0f = sigf.copy()
+        Gf = np.asarray(gen_Green0(eig, uni, mu, temp, Nw)).T
+    info = dict(iters=0, esterr=[], mu=[], stoner=[])
+    for it in range(1, scf_loop + 1):
+        Gk_py = np.ascontiguousarray(Gf.T)
+        chi = get_chi0_conv_soc(Gk_py, kmap, invk, invs, olist, slist, temp, Nx, Ny, Nz)
+        stoner = ckchi_soc(chi, Vmat)
+        if sw_rescale and stoner >= 1.0:
+            chi = chi * ((1.0 - 1.0e-4) / stoner)
+        V = get_V_soc_flex(chi, Vmat)
+        sigf = np.asarray(calc_sigma_soc(Gk_py, V, Vmat, kmap, invk, invs, olist, slist, temp,
+                                         Nx, Ny, Nz)).T.copy()
+        if sub_sigma > 0:
+            base = sigf[:, 0] if sub_sigma == 1 else sigf.mean(axis=1)
+            sub = 0.5 * (base + np.conj(np.swapaxes(base, 1, 2)))
+            sigf = sigf - sub[:, None]
+        sigf = fl.compare_sigma(sigf, sig0f, pp)
+        info['iters'] = it
+        info['esterr'].append(fl.esterr)
+        info['stoner'].append(stoner)
+        if fl.esterr < eps:
+            break
+        mu, mu_old = fl.renew_mu(mu, mu_old, sigf)
+        info['mu'].append(mu)
+        Gf = np.asarray(gen_green(np.ascontiguousarray(sigf.T), hamk, mu, temp)).T
+        sig0f = sigf.copy()
+    info['mu_file'], info['mu_old_file'] = mu, mu_old
+    mu, mu_old = fl.renew_mu(mu, mu_old, sigf)
+    out = np.ascontiguousarray(sigf.T)
+    if return_info:
+        return out, mu, info
+    return out, mu

@@ -36,6 +36,16 @@ _SIGS = {
     "get_eig_or_tr_chi": [_z, _z, _i, _pi, _pi, _pi, _pb],
     # addition: chi_s / chi_c at every (q, i nu)
     "gmod_get_chis_chic_all": [c_void_p, c_void_p, _z, _d, _d, _pi, _pi, _pi],
+    # libs/flibs/_susceptibility.py:22-62 (real-frequency maps), _hamiltonian.py:79-85
+    "get_chi_irr": [_z, _z, _d, _d, _i, _i, _d, _pi, _pi, _pi, _pi, _pd, _pd, _pd],
+    "chiq_map": [_z, _z, _z, _d, _d, _d, _d, _i, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, _pi],
+    "get_tr_chi": [_z, _z, _z, _z, _z, _i, _pi, _pi, _pi],
+    "get_qshift_": [_d, _d, _i, _pi],
+    # libs/flibs/_green.py:54-78 (Pade continuation)
+    "get_a_": [_z, _z, _z, _pi],
+    "get_qp_": [_z, _z, _z, _z, _z, _pi, _pi],
+    "pade_with_trace": [_z, _z, _z, _z, _pi, _pi, _pi, _pi],
+    "pade_analytic_continuation_arrays_": [_z, _z, _z, _z, _pi, _pi, _pi],
     # libs/flibs/_green.py:7-27
     "gen_green0_": [_z, _d, _z, _pd, _pd, _pi, _pi, _pi],
     "gen_green_inv_": [_z, _z, _z, _pd, _pd, _pi, _pi, _pi],
@@ -80,6 +90,8 @@ def _L():
         L.gmod_regrid_sigma.argtypes = [_z, _z, _i, _i, c_int64, c_int64, c_int64, c_int64, c_int64, _i, _i, c_int64,
                                         c_int64, c_int64, c_int64, c_int64, c_int64, c_double]
         L.gmod_regrid_sigma.restype = None
+        L.gmod_wannier_r.argtypes = [_z, _z, _i, _i] + [c_int64] * 7 + [c_int]
+        L.gmod_wannier_r.restype = None
         L.gmod_profile.argtypes = [c_int]
         L.gmod_profile_read.argtypes = [_d, c_int]
         L.gmod_profile_read.restype = c_int
@@ -315,6 +327,97 @@ def get_chis_chic_all(chi, Smat, Cmat, spin=True, charge=True):
     return chis, chic
 
 
+def get_qshift(klist, qpoint):
+    """1-based index of k + q for every k of klist[Nk,3]  [_hamiltonian.py:330-341]"""
+    kl = np.ascontiguousarray(klist, dtype=np.float64)
+    Nk = len(kl)
+    qshift = np.zeros(Nk, dtype=np.int64)
+    _call("get_qshift_", np.ascontiguousarray(qpoint, dtype=np.float64), kl, qshift, i64(Nk))
+    return qshift
+
+
+def get_chi_irr(uni, eig, ffermi, qshift, olist, wlist, idelta, temp):
+    """chi0(q, w + i idelta) [Nw, Nchi, Nchi] by the band sum  [_susceptibility.py:271-293]"""
+    Nk, Nw = len(eig), len(wlist)
+    Norb, Nchi = eig.shape[1], len(olist)
+    chi = np.zeros((Nw, Nchi, Nchi), dtype=np.complex128)
+    eps = idelta * 1e-3
+    _call("get_chi_irr", chi, np.ascontiguousarray(uni), np.ascontiguousarray(eig), np.ascontiguousarray(ffermi),
+          np.ascontiguousarray(qshift, dtype=np.int64), olist, np.ascontiguousarray(wlist, dtype=np.float64),
+          i64(Nchi), i64(Norb), i64(Nk), i64(Nw), dbl(idelta), dbl(eps), dbl(temp))
+    return chi
+
+
+def chis_qmap(uni, eig, ffermi, klist, Smat, olist, Nx, Ny, temp, ecut, idelta):
+    """(chis, chi) [Nx, Ny]: traces over the orbital-diagonal pairs of chi_s
+    and chi0 at w = ecut on the q plane  [_susceptibility.py:295-324]"""
+    Nk = len(eig)
+    Norb, Nchi = eig.shape[1], len(olist)
+    chi = np.zeros((Nx, Ny), dtype=np.complex128)
+    chis = np.zeros((Nx, Ny), dtype=np.complex128)
+    eps = idelta * 1e-3
+    _call("chiq_map", chis, chi, np.ascontiguousarray(uni), np.ascontiguousarray(eig), np.ascontiguousarray(ffermi),
+          np.ascontiguousarray(klist, dtype=np.float64), np.ascontiguousarray(Smat), olist, dbl(temp), dbl(ecut),
+          dbl(idelta), dbl(eps), i64(Nx), i64(Ny), i64(Nk), i64(Norb), i64(Nchi))
+    return chis, chi
+
+
+def get_tr_chi(chis, chi0, olist):
+    """(trchis[Nw], trchi0[Nw], chis_orb[Nw, Norb+2])  [_susceptibility.py:326-344]"""
+    Nchi, Nw, Norb = len(olist), len(chi0), int(np.asarray(olist).max())
+    trchis = np.zeros(Nw, dtype=np.complex128)
+    trchi0 = np.zeros(Nw, dtype=np.complex128)
+    chis_orb = np.zeros((Nw, Norb + 2), dtype=np.complex128)
+    _call("get_tr_chi", trchis, trchi0, chis_orb, np.ascontiguousarray(chis), np.ascontiguousarray(chi0), olist,
+          i64(Nw), i64(Nchi), i64(Norb))
+    return trchis, trchi0, chis_orb
+
+
+def get_a(inp_data, xlist):
+    """Pade continued-fraction coefficients [Np]  [_green.py:177-188]"""
+    Np = len(inp_data)
+    a = np.zeros(Np, dtype=np.complex128)
+    _call("get_a_", a, np.ascontiguousarray(xlist, dtype=np.complex128),
+          np.ascontiguousarray(inp_data, dtype=np.complex128), i64(Np))
+    return a
+
+
+def get_QP(a, xlist, wlist):
+    """(Q, P) of the continued fraction at wlist, in the reference wrapper's
+    return order  [_green.py:190-204]"""
+    Nw, Np = len(wlist), len(a)
+    Q = np.zeros(Nw, dtype=np.complex128)
+    P = np.zeros(Nw, dtype=np.complex128)
+    _call("get_qp_", P, Q, np.ascontiguousarray(a, dtype=np.complex128), np.ascontiguousarray(xlist, dtype=np.complex128),
+          np.ascontiguousarray(wlist, dtype=np.complex128), i64(Nw), i64(Np))
+    return Q, P
+
+
+def pade_with_trace(A, iwlist, wlist):
+    """[Nk, Nw]: Pade continuation of the orbital-diagonal blocks of
+    A[Norb,Norb,Niw,Nk], summed  [_green.py:206-220]"""
+    Nk, Nw, Niw = len(A.T), len(wlist), len(iwlist)
+    Norb = A.shape[0]
+    B = np.zeros((Nk, Nw), dtype=np.complex128)
+    _call("pade_with_trace", np.ascontiguousarray(A, dtype=np.complex128), B,
+          np.ascontiguousarray(iwlist, dtype=np.complex128), np.ascontiguousarray(wlist, dtype=np.complex128),
+          i64(Nk), i64(Niw), i64(Nw), i64(Norb))
+    return B
+
+
+def pade_arrays(arrayin, iwlist, wlist):
+    """[Nw, Nk] view of the Fortran arrayout(Nk,Nw): the continuation of every
+    row of arrayin (Python [Niw, Nk], Fortran arrayin(Nk,Niw))
+    (pade_analytic_continuation_arrays_, fpade.f90:87-106; no reference wrapper)"""
+    Niw, Nk = np.shape(arrayin)
+    Nw = len(wlist)
+    out = np.zeros((Nw, Nk), dtype=np.complex128)
+    _call("pade_analytic_continuation_arrays_", np.ascontiguousarray(arrayin, dtype=np.complex128), out,
+          np.ascontiguousarray(iwlist, dtype=np.complex128), np.ascontiguousarray(wlist, dtype=np.complex128),
+          i64(Nk), i64(Niw), i64(Nw))
+    return out
+
+
 def get_eig_or_tr_chi(chi, invk, sw_eig):
     """[_susceptibility.py:615-629]"""
     Nkall, Nk = len(invk), chi.shape[2]
@@ -536,3 +639,19 @@ def regrid_sigma(sigmak_c, kmap_c, invk_c, mesh_c, kmap_f, invk_f, mesh_f, temp)
                            Nkf, Nxf, Nyf, Nzf, Nwf, No, float(temp))
     check()
     return out
+
+
+def wannier_r(data, kmap, invk, Nx, Ny, Nz, N_cut, full=False):
+    """data[Norb, Norb, Nw, Nk] on the irreducible k points (full: on every
+    grid point, placed through kmap) -> data_R[Norb, Norb, min(N_cut, Nw), Nx,
+    Ny, Nz], the normalised inverse transform over the k grid of the first
+    N_cut frequencies (numpy's ifftn), computed by gmod_wannier_r on the
+    device (the transform of _wannier_io.py:48-99)."""
+    d = np.ascontiguousarray(data, dtype=np.complex128)
+    No, _, Nw, Nk = d.shape
+    Ncut = min(int(N_cut), Nw)
+    out = np.zeros((No, No, Ncut, Nz, Ny, Nx), dtype=np.complex128)
+    _L().gmod_wannier_r(out, d, np.ascontiguousarray(kmap, dtype=np.int64), np.ascontiguousarray(invk, dtype=np.int64),
+                        Nx, Ny, Nz, Nw, Nk, No, Ncut, 1 if full else 0)
+    check()
+    return out.transpose(0, 1, 2, 5, 4, 3)
