@@ -173,6 +173,40 @@ void remap_delta_(gmod_z* delta, const gmod_z* delta0, const double* prt, const 
                   const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Norb,
                   const int64_t* gap_sym);
 
+/* ---------------- spinful basis (spin-orbit coupling) -------------------- */
+/* Orbitals carry spin: slist(Norb) = +-1, invs(Norb) the 1-based spin-flipped
+ * partner, olist over all Norb^2 pairs, Nchi = Norb^2. */
+/* libs/src/fselfwithsoc.f90:1-539 [libs/flibs/_selfenergy.py:129-174]; precision 0 FP64,
+ * 1 and 2 FP32 transforms of the full fields */
+void mkself_soc(gmod_z* sigmak, double* mu, const double* Vmat, const int64_t* kmap, const int64_t* invk,
+                const int64_t* invs, const int64_t* olist, const int64_t* slist, const gmod_z* hamk, const double* eig,
+                const gmod_z* uni, const double* mu_init, const double* rfill, const double* temp,
+                const int64_t* scf_loop, const double* pp, const double* eps, const int64_t* Nkall, const int64_t* Nk,
+                const int64_t* Nw, const int64_t* Norb, const int64_t* Nchi, const int64_t* Nx, const int64_t* Ny,
+                const int64_t* Nz, const int64_t* sub_sigma, const bool* sw_out, const bool* sw_in,
+                const int64_t* m_diis, const bool* sw_rescale, const double* sigma_scale);
+/* libs/src/fchi0withsoc.f90:111-214 [libs/flibs/_susceptibility.py:496-528] */
+void get_chi0_soc(gmod_z* chi, double* sgnsig, double* sgnsig2, int64_t* invschi, const double* Vmat,
+                  const gmod_z* Gk, const int64_t* kmap, const int64_t* invk, const int64_t* invs,
+                  const int64_t* olist, const int64_t* slist, const double* temp, const int64_t* Nx,
+                  const int64_t* Ny, const int64_t* Nz, const int64_t* Nw, const int64_t* Nk, const int64_t* Nkall,
+                  const int64_t* Nchi, const int64_t* Norb);
+/* libs/src/fchi0withsoc.f90:281-326 [libs/flibs/_susceptibility.py:530-566]: chic, chiszz,
+ * chispm (Nk, Nchi/4, Nchi/4), added to the arrays passed in and then scaled, as the reference does */
+void get_chis_chic_soc(gmod_z* chic, gmod_z* chiszz, gmod_z* chispm, const gmod_z* chi, const double* Vmat,
+                       const int64_t* orb_list, const int64_t* olist, const int64_t* slist, const int64_t* invs,
+                       const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi, const int64_t* Norb);
+/* libs/src/fUJ.f90:367-399 (get_V_soc_flex, internal to mkself_soc): in place on
+ * chi(Nk,Nw,Nchi,Nchi), -V + V (I + chi0 V)^-1 chi0 V */
+void gmod_v_soc_flex(gmod_z* chi, const double* Vmat, const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi);
+/* libs/src/fUJ.f90:109-180 [libs/flibs/_interaction.py:83-100] */
+void get_vmat_soc(double* Vmat, const int64_t* ol, const int64_t* sl, const int64_t* site, const int64_t* invs,
+                  const double* Uval, const double* Jval, const int64_t* Nchi, const int64_t* Norb);
+/* libs/src/fUJ.f90:182-251 [libs/flibs/_interaction.py:102-119] */
+void get_vmat_soc_orb(double* Vmat, const int64_t* ol, const int64_t* sl, const int64_t* site,
+                      const int64_t* invs, const double* Umat, const double* Jmat, const int64_t* Nchi,
+                      const int64_t* Norb);
+
 /* ---------------- interaction vertices and setup ------------------------- */
 /* libs/src/fUJ.f90:1-57 [libs/flibs/_interaction.py:47-62] */
 void get_scmat(double* Smat, double* Cmat, const int64_t* ol, const int64_t* site, const double* Uval,

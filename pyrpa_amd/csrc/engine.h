@@ -14,7 +14,15 @@ struct OrbPairs {
   std::vector<std::array<int, 2>> ol;       // 0-based (o1, o2) per chi index (Fortran olist(:,1), olist(:,2))
   std::vector<std::array<int, 2>> irr;      // irreducible (l, m) pairs  (fchi.f90:470-480)
   std::vector<std::array<int, 2>> partner;  // partner[(m + Nchi*l)] = (m2, l2)  (fchi.f90:456-457)
+  // spinful basis (fchi0withsoc.f90): spin-flipped orbital, spin sign per
+  // orbital and the sign of each partner copy (sgnsig2; +1 without spin)
+  bool soc = false;
+  std::vector<int> inv;
+  std::vector<double> spin;
+  std::vector<double> psign;
   void from_olist(const int64_t* olist, int Nchi, int No);
+  // partners chi_{1234} = s chi*_{-4-3-2-1} with spin flips (fchi0withsoc.f90:216-279)
+  void from_olist_soc(const int64_t* olist, int Nchi, int No, const int64_t* invs, const int64_t* slist);
   void from_maps(const int32_t* irr_chi, const int32_t* chi_map);  // caller-provided tables
 };
 
@@ -40,6 +48,9 @@ class ConvEngine {
   // sigma(Nk,Nw,No,No) <- (T/Nkall) sum_pairs V (*) G, using Fg; thresholds at 1e-9
   void sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s,
              bool apply_threshold = true);
+  // the same with the nu -> infinity value of each pair given directly,
+  // stat_h[m + Nchi*l] (-Vmat(m,l) in a spinful basis, fselfwithsoc.f90:592)
+  void sigma(zd* sig, const zd* V, const double* stat_h, double temp, hipStream_t s, bool apply_threshold = true);
 
   // tail constants J (per chi pair, [l + Nchi*m]) at mu and the sawtooth table (built once)
   void tail_constants(double temp, const double* eig_h, const zd* uni_h, const int64_t* invk, double mu,

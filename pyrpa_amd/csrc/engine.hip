@@ -30,6 +30,7 @@ void OrbPairs::from_olist(const int64_t* olist, int Nchi_, int No_) {
   }
   // exchange-symmetry partners chi_{1234} = chi*_{4321} (fchi.f90:421-481)
   partner.assign(Nchi * Nchi, {-1, -1});
+  psign.assign(Nchi * Nchi, 1.0);
   std::vector<char> cirr(Nchi * Nchi, 0);
   for (int l1 = 0; l1 < Nchi; ++l1)
     for (int m1 = 0; m1 < Nchi; ++m1) {
@@ -53,11 +54,59 @@ void OrbPairs::from_olist(const int64_t* olist, int Nchi_, int No_) {
   if (irr.size() > P) irr.resize(P);
 }
 
+void OrbPairs::from_olist_soc(const int64_t* olist, int Nchi_, int No_, const int64_t* invs, const int64_t* slist) {
+  Nchi = Nchi_;
+  No = No_;
+  soc = true;
+  ol.resize(Nchi);
+  inv.resize(No);
+  spin.resize(No);
+  for (int a = 0; a < No; ++a) {
+    inv[a] = (int)invs[a] - 1;
+    spin[a] = (double)slist[a];
+    if (inv[a] < 0 || inv[a] >= No) throw Error("invs entry outside [1,Norb]");
+    if (slist[a] != 1 && slist[a] != -1) throw Error("slist entries must be +1 or -1");
+  }
+  for (int l = 0; l < Nchi; ++l) {
+    ol[l] = {(int)olist[l] - 1, (int)olist[l + Nchi] - 1};
+    if (ol[l][0] < 0 || ol[l][0] >= No || ol[l][1] < 0 || ol[l][1] >= No)
+      throw Error("olist entry outside [1,Norb]");
+  }
+  // the pair (l, m) and its spin-flipped exchange partner (m2, l2):
+  // (l1, l2, m1, m2) = (-m2_2, -m2_1, -l2_2, -l2_1); the first of the two
+  // in (l, m) order is the one computed
+  partner.assign(Nchi * Nchi, {-1, -1});
+  psign.assign(Nchi * Nchi, 1.0);
+  std::vector<char> cirr(Nchi * Nchi, 0);
+  for (int l1 = 0; l1 < Nchi; ++l1)
+    for (int m1 = 0; m1 < Nchi; ++m1) {
+      bool found = false;
+      for (int l2 = 0; l2 < Nchi && !found; ++l2)
+        for (int m2 = 0; m2 < Nchi; ++m2) {
+          if (ol[l1][0] == inv[ol[m2][1]] && ol[l1][1] == inv[ol[m2][0]] && ol[m1][0] == inv[ol[l2][1]] &&
+              ol[m1][1] == inv[ol[l2][0]]) {
+            partner[m1 + Nchi * l1] = {m2, l2};
+            if (!cirr[m2 + Nchi * l2]) cirr[m1 + Nchi * l1] = 1;
+            found = true;
+            break;
+          }
+        }
+      psign[m1 + Nchi * l1] = spin[ol[l1][0]] * spin[ol[l1][1]] * spin[ol[m1][0]] * spin[ol[m1][1]];
+    }
+  irr.clear();
+  for (int l1 = 0; l1 < Nchi; ++l1)
+    for (int m1 = 0; m1 < Nchi; ++m1)
+      if (cirr[m1 + Nchi * l1]) irr.push_back({l1, m1});
+  const size_t P = (size_t)Nchi * (Nchi + 1) / 2;
+  if (irr.size() > P) irr.resize(P);
+}
+
 void OrbPairs::from_maps(const int32_t* irr_chi, const int32_t* chi_map) {
   const int P = Nchi * (Nchi + 1) / 2;
   irr.resize(P);
   for (int i = 0; i < P; ++i) irr[i] = {irr_chi[i] - 1, irr_chi[i + P] - 1};
   partner.assign(Nchi * Nchi, {-1, -1});
+  if (psign.size() != (size_t)Nchi * Nchi) psign.assign(Nchi * Nchi, 1.0);
   for (int l = 0; l < Nchi; ++l)
     for (int m = 0; m < Nchi; ++m)
       partner[m + Nchi * l] = {chi_map[m + Nchi * l] - 1, chi_map[m + Nchi * l + Nchi * Nchi] - 1};
@@ -88,7 +137,12 @@ void ConvEngine<R>::transform_green(const zd* G, bool g0, hipStream_t s) {
   for (int b = 0; b < No; ++b)
     for (int a = 0; a < No; ++a) {
       cpx<R>* g = grid(dst, a, b);
-      p4->exec_fused(GreenLoad<R>{G, a, b, No, md}, GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
+      if (op.soc) {
+        const SocGreenLoad<R> ld{G, a, b, op.inv[a], op.inv[b], No, op.spin[a] * op.spin[b], md};
+        p4->exec_fused(ld, GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
+      } else {
+        p4->exec_fused(GreenLoad<R>{G, a, b, No, md}, GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
+      }
     }
 }
 
@@ -261,7 +315,8 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
                      true);
       continue;
     }
-    if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
+    // the spinful partner carries the sign s_l1 s_l2 s_m1 s_m2 (fchi0withsoc.f90:107)
+    if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, op.psign[m + Nchi * l], s);
   }
   if (apply_threshold) threshold(chi, NkNw * Nchi * Nchi, 1e-9, s);
 }
@@ -360,6 +415,16 @@ void ConvEngine<R>::chi0_split_tail(zd* chi, double temp, const double* eig_d, c
 template <typename R>
 void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s,
                           bool apply_threshold) {
+  // the nu -> infinity slot of V_sigma is 1.5 S - 0.5 C (fself.f90:208)
+  const int Nchi = op.Nchi;
+  std::vector<double> stat((size_t)Nchi * Nchi);
+  for (size_t e = 0; e < stat.size(); ++e) stat[e] = 1.5 * S_h[e] - 0.5 * C_h[e];
+  sigma(sig, V, stat.data(), temp, s, apply_threshold);
+}
+
+template <typename R>
+void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* stat_h, double temp, hipStream_t s,
+                          bool apply_threshold) {
   const int No = op.No, Nchi = op.Nchi;
   const long NkNw = mesh.Nk * (long)mesh.Nw;
   if (acc.n != (size_t)N) acc.alloc(N);
@@ -367,7 +432,7 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double*
   const R invN = (R)(1.0 / (double)p4->transformed_points());
   const long NC = NkNw;
   // the accumulating last pass sums the pairs of one output in registers
-  const bool fused = p4->accum_ok(-1) && !getenv("GMOD_SIGMA_UNFUSED");
+  const bool fused = p4->accum_ok(-1);
   std::vector<std::array<int, 2>> pairs;
   if (fused) {
     accum_tab_h.clear();
@@ -396,7 +461,7 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double*
       for (int l = 0; l < Nchi; ++l)
         for (int m = 0; m < Nchi; ++m) {
           if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
-          zd stat(1.5 * S_h[m + Nchi * l] - 0.5 * C_h[m + Nchi * l], 0);
+          const zd stat(stat_h[m + Nchi * l], 0);
           BosonLoad<R> bl{V, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md};
           if (fused) {
             p4->exec_head(bl, srcs.p + (long)nsrc * N, tmp.p, -1, s);
@@ -657,7 +722,7 @@ void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, d
     if (dvbuf.n != (size_t)nC) dvbuf.alloc(nC);
     diff_to_f32(dvbuf.p, V, Vprev, nC, s);
   }
-  const bool fused = p4->accum_ok(-1) && !getenv("GMOD_SIGMA_UNFUSED");
+  const bool fused = p4->accum_ok(-1);
   if (fused) {
     accum_tab_h.clear();
     int nmax = 0;

@@ -48,8 +48,9 @@ struct Upload {
 
 // Stoner factor: max_q max Re eig(chi0(q,0) S) (fchi.f90:352-399); thr > 0:
 // of the thresholded chi0 (the first frequency slice is copied with it)
+// (spinful basis: max Re eig(-chi0(q,0) V), fchi0withsoc.f90:156-198, alpha = -1)
 static double stoner_factor(const zd* chi, long Nk, long Nw, int Nchi, const double* S_d, hipStream_t s,
-                            double thr = 0.0) {
+                            double thr = 0.0, double alpha = 1.0) {
   DBuf<double> out(Nk);
   DBuf<int> err(1);
   err.zero(s);
@@ -57,9 +58,9 @@ static double stoner_factor(const zd* chi, long Nk, long Nw, int Nchi, const dou
   if (thr > 0.0) {
     slice.alloc((size_t)Nk * Nchi * Nchi);
     threshold_slice(slice.p, chi, Nk, Nk * Nw, (long)Nchi * Nchi, thr, s);
-    max_re_eig(out.p, slice.p, BatchLayout{1, Nk}, Nk, Nchi, S_d, 1.0, err.p, s);
+    max_re_eig(out.p, slice.p, BatchLayout{1, Nk}, Nk, Nchi, S_d, alpha, err.p, s);
   } else {
-    max_re_eig(out.p, chi, BatchLayout{1, Nk * Nw}, Nk, Nchi, S_d, 1.0, err.p, s);
+    max_re_eig(out.p, chi, BatchLayout{1, Nk * Nw}, Nk, Nchi, S_d, alpha, err.p, s);
   }
   check_err(err, s, "ckchi");
   std::vector<double> h(Nk);
@@ -74,7 +75,8 @@ static double stoner_factor(const zd* chi, long Nk, long Nw, int Nchi, const dou
 template <typename R>
 static void chi0_driver(zd* chi_h, const zd* Gk, const int64_t* kmap, const int64_t* invk, OrbPairs& op,
                         double temp, long Nx, long Ny, long Nz, long Nw, long Nk, long Nkall, int No, bool tail,
-                        const double* eig, const zd* uni, double mu, const double* S, double* stoner) {
+                        const double* eig, const zd* uni, double mu, const double* S, double* stoner,
+                        double alpha = 1.0) {
   hipStream_t s = ctx().stream;
   Mesh mesh;
   mesh.build(kmap, invk, Nkall, Nk, (int)Nx, (int)Ny, (int)Nz, (int)Nw, s);
@@ -90,7 +92,7 @@ static void chi0_driver(zd* chi_h, const zd* Gk, const int64_t* kmap, const int6
   eng.chi0(chi.p, temp, tail, eig_d.p, uni_d.p, eig, uni, invk, mu, s);
   if (stoner) {
     up(S_d, S, (size_t)Nchi * Nchi);
-    *stoner = stoner_factor(chi.p, Nk, Nw, Nchi, S_d.p, s);
+    *stoner = stoner_factor(chi.p, Nk, Nw, Nchi, S_d.p, s, 0.0, alpha);
   }
   chi.download(chi_h, NkNw * Nchi * Nchi, s);
   GMOD_HIP(hipStreamSynchronize(s));
@@ -137,6 +139,10 @@ struct FlexIn {
   long scf_loop, Nkall, Nk, Nw, Nx, Ny, Nz, sub_sigma, m_diis;
   int No, Nchi;
   bool sw_out, sw_in, sw_rescale, sw_tail;
+  // spinful basis (fselfwithsoc.f90): the vertex Vmat replaces S and C
+  bool soc = false;
+  const double* Vmat = nullptr;
+  const int64_t *invs = nullptr, *slist = nullptr;
 };
 
 // Precision modes: R = double, mixed = false: FP64 throughout (the reference's
@@ -155,15 +161,26 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
   const size_t nS = (size_t)NkNw * No * No, nC = (size_t)NkNw * Nchi * Nchi;
   const double T = A.temp;
   if (A.m_diis < 1) throw Error("m_diis must be >= 1");
+  if (A.soc && (mixed || A.sw_tail)) throw Error("mkself_soc: the spinful loop runs without the tail and increments");
   Mesh mesh;
   mesh.build(A.kmap, A.invk, A.Nkall, Nk, (int)A.Nx, (int)A.Ny, (int)A.Nz, (int)Nw, s);
   OrbPairs op;
-  op.from_olist(A.olist, Nchi, No);
+  if (A.soc) op.from_olist_soc(A.olist, Nchi, No, A.invs, A.slist);
+  else op.from_olist(A.olist, Nchi, No);
   Upload up{s};
   DBuf<double> S_d, C_d, eig_d;
   DBuf<zd> ham_d, uni_d;
-  up(S_d, A.Smat, (size_t)Nchi * Nchi);
-  up(C_d, A.Cmat, (size_t)Nchi * Nchi);
+  // spinful: S_d holds Vmat, and the nu -> infinity value of pair (m, l) is
+  // -Vmat(m, l) (fselfwithsoc.f90:592)
+  std::vector<double> soc_stat;
+  if (A.soc) {
+    up(S_d, A.Vmat, (size_t)Nchi * Nchi);
+    soc_stat.resize((size_t)Nchi * Nchi);
+    for (size_t e = 0; e < soc_stat.size(); ++e) soc_stat[e] = -A.Vmat[e];
+  } else {
+    up(S_d, A.Smat, (size_t)Nchi * Nchi);
+    up(C_d, A.Cmat, (size_t)Nchi * Nchi);
+  }
   up(eig_d, A.eig, (size_t)No * Nk);
   up(ham_d, A.hamk, (size_t)No * No * Nk);
   up(uni_d, A.uni, (size_t)No * No * Nk);
@@ -356,7 +373,7 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
       e32->chi0_increment(chi_acc.p, G.p, Gprev.p, G0.p, G0prev.p, T, A.sw_tail, eig_d.p, uni_d.p, mu, mu_prev, dJ,
                           sawc_d.p, s);
     }
-    stoner = stoner_factor(chi_src, Nk, Nw, Nchi, S_d.p, s, 1e-9);
+    stoner = stoner_factor(chi_src, Nk, Nw, Nchi, S_d.p, s, 1e-9, A.soc ? -1.0 : 1.0);
     t_chi += secs(t0);
     if (verbose) printf("SDW max eig(chi0 S) = %12.8f\n", stoner);
     const bool rescale = A.sw_rescale && stoner >= 1.0;
@@ -372,19 +389,23 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
     const bool dv_fused = inc && !rescale;
     {
       ProfScope ps(PROF_VERTEX, (double)nC * 2 * sizeof(zd), s);
-      if (rescale)
+      if (A.soc)  // chi already thresholded (and rescaled) when rescale
+        soc_vertex(rescale ? chi.p : chi_src, chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, rescale ? 0.0 : 1e-9,
+                   err.p, s);
+      else if (rescale)
         rpa_vertex(chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 0, err.p, s);
       else
         rpa_vertex_fused(chi_src, chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 1e-9,
                          dv_fused ? Vprev.p : nullptr, dv_fused ? e32->dv_buffer(nC) : nullptr, err.p, s);
     }
-    check_err(err, s, "get_vsigma_flex_nosoc");
+    check_err(err, s, A.soc ? "get_V_soc_flex" : "get_vsigma_flex_nosoc");
     t_vert += secs(t0);
     if (verbose)
       printf("Re V_sigma: max:%12.4E min:%12.4E\n", max_real(chi.p, nC, s), min_real(chi.p, nC, s));
     t0 = clk::now();
     if (!inc) {
-      eng->sigma(mixed ? sig_acc.p : sig.p, chi.p, A.Smat, A.Cmat, T, s, !mixed);
+      if (A.soc) eng->sigma(sig.p, chi.p, soc_stat.data(), T, s, true);
+      else eng->sigma(mixed ? sig_acc.p : sig.p, chi.p, A.Smat, A.Cmat, T, s, !mixed);
       if (mixed) {
         threshold_copy(sig.p, sig_acc.p, nS, 1e-9, s);
         copy(Vprev.p, chi.p, nC, s);
@@ -815,6 +836,29 @@ extern "C" void get_vsigma_flex_nosoc_(gmod_z* chi, const double* Smat, const do
   GMOD_CATCH
 }
 
+// the spinful vertex of fUJ.f90:367-399 in place on chi(Nk,Nw,Nchi,Nchi)
+// (internal to mkself_soc in the reference, exported here for parity tests)
+extern "C" void gmod_v_soc_flex(gmod_z* chi, const double* Vmat, const int64_t* Nk, const int64_t* Nw,
+                                const int64_t* Nchi) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const long NkNw = *Nk * *Nw;
+  const int n = (int)*Nchi;
+  Upload up{s};
+  DBuf<zd> c;
+  DBuf<double> V;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(c, (const zd*)chi, (size_t)NkNw * n * n);
+  up(V, Vmat, (size_t)n * n);
+  soc_vertex(c.p, c.p, NkNw, BatchLayout{1, NkNw}, n, V.p, 0.0, err.p, s);
+  check_err(err, s, "get_V_soc_flex");
+  c.download((zd*)chi, (size_t)NkNw * n * n, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
 extern "C" void get_chis(gmod_z* chis, const gmod_z* chi0, const double* Smat, const int64_t* Nchi,
                          const int64_t* Nw) {
   GMOD_TRY
@@ -1086,6 +1130,128 @@ extern "C" void mkself(gmod_z* sigmak, double* mu, const double* Smat, const dou
   A.sw_out = *sw_out; A.sw_in = *sw_in; A.sw_rescale = *sw_rescale; A.sw_tail = *sw_tail;
   if (ctx().precision == 1) mkself_driver<double>(A, (zd*)sigmak, mu, true);
   else if (ctx().precision == 2) mkself_driver<float>(A, (zd*)sigmak, mu, false);
+  else mkself_driver<double>(A, (zd*)sigmak, mu, false);
+  GMOD_CATCH
+}
+
+// ====================================================================== spinful basis (SOC)
+// fchi0withsoc.f90:111-214: chi0 of a spinful basis with the sign tables
+// sgnsig(Norb,Norb), sgnsig2(Nchi,Nchi) and invschi(Nchi) it returns
+extern "C" void get_chi0_soc(gmod_z* chi, double* sgnsig, double* sgnsig2, int64_t* invschi, const double* Vmat,
+                             const gmod_z* Gk, const int64_t* kmap, const int64_t* invk, const int64_t* invs,
+                             const int64_t* olist, const int64_t* slist, const double* temp, const int64_t* Nx,
+                             const int64_t* Ny, const int64_t* Nz, const int64_t* Nw, const int64_t* Nk,
+                             const int64_t* Nkall, const int64_t* Nchi, const int64_t* Norb) {
+  GMOD_TRY
+  auto g = api_lock();
+  const int No = (int)*Norb, n = (int)*Nchi;
+  OrbPairs op;
+  op.from_olist_soc(olist, n, No, invs, slist);
+  for (int j = 0; j < No; ++j)
+    for (int i = 0; i < No; ++i) sgnsig[i + No * j] = op.spin[i] * op.spin[j];
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) sgnsig2[i + n * j] = op.psign[i + n * j];
+  for (int l = 0; l < n; ++l) {
+    invschi[l] = 0;
+    for (int m = 0; m < n; ++m)
+      if (op.ol[m][0] == op.inv[op.ol[l][0]] && op.ol[m][1] == op.inv[op.ol[l][1]]) {
+        invschi[l] = m + 1;
+        break;
+      }
+  }
+  double stoner = 0.0;
+  double* st = ctx().verbose ? &stoner : nullptr;
+  if (ctx().precision >= 1)
+    chi0_driver<float>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, No, false,
+                       nullptr, nullptr, 0.0, Vmat, st, -1.0);
+  else
+    chi0_driver<double>((zd*)chi, (const zd*)Gk, kmap, invk, op, *temp, *Nx, *Ny, *Nz, *Nw, *Nk, *Nkall, No, false,
+                        nullptr, nullptr, 0.0, Vmat, st, -1.0);
+  if (st) printf("SDW/CDW max eig(-chi0 V) = %12.8f\n", stoner);
+  GMOD_CATCH
+}
+
+// fchi0withsoc.f90:281-326: charge, longitudinal and transverse spin
+// susceptibilities (Nk, Nchi/4, Nchi/4) at the first boson frequency from
+// X = (I + chi0 V)^-1 chi0, summed over the pairs orb_list maps together.
+// Like the reference, the sums are added to the arrays passed in, which are
+// then scaled (chic 1/2, chiszz 1/4).
+extern "C" void get_chis_chic_soc(gmod_z* chic, gmod_z* chiszz, gmod_z* chispm, const gmod_z* chi,
+                                  const double* Vmat, const int64_t* orb_list, const int64_t* olist,
+                                  const int64_t* slist, const int64_t* invs, const int64_t* Nk, const int64_t* Nw,
+                                  const int64_t* Nchi, const int64_t* Norb) {
+  GMOD_TRY
+  auto g = api_lock();
+  (void)invs;
+  (void)Norb;
+  hipStream_t s = ctx().stream;
+  const int n = (int)*Nchi, n4 = n / 4;
+  const long nk = *Nk, NkNw = nk * *Nw;
+  const size_t tot = (size_t)nk * n * n;
+  for (int l = 0; l < n; ++l)
+    if (orb_list[l] < 1 || orb_list[l] > n4) throw Error("get_chis_chic_soc: orb_list entry outside [1, Nchi/4]");
+  Upload up{s};
+  DBuf<zd> a, o(tot);
+  DBuf<double> V;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(a, (const zd*)chi, (size_t)NkNw * n * n);
+  up(V, Vmat, (size_t)n * n);
+  rpa_solve(o.p, BatchLayout{1, nk}, a.p, BatchLayout{1, NkNw}, nk, n, V.p, 1.0, err.p, s);
+  check_err(err, s, "get_chis_chic_soc");
+  std::vector<zd> X(tot);
+  o.download(X.data(), tot, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  auto sp = [&](int orb) { return slist[orb - 1]; };
+  zd* cc = (zd*)chic;
+  zd* zz = (zd*)chiszz;
+  zd* pm = (zd*)chispm;
+  for (long q = 0; q < nk; ++q)
+    for (int l = 0; l < n; ++l) {
+      const int64_t l1 = olist[l], l2 = olist[l + n];
+      for (int m = 0; m < n; ++m) {
+        const int64_t m1 = olist[m], m2 = olist[m + n];
+        const zd x = X[q + nk * (m + (long)n * l)];
+        const long o = q + nk * ((orb_list[l] - 1) + (long)n4 * (orb_list[m] - 1));
+        if (sp(l1) == sp(m1) && sp(l2) == sp(m2) && sp(l1) == -sp(l2) && sp(l1) == 1) pm[o] += x;
+        if (sp(l1) == sp(l2) && sp(m1) == sp(m2)) {
+          cc[o] += x;
+          if (sp(l1) == sp(m1)) zz[o] += x;
+          else zz[o] -= x;
+        }
+      }
+    }
+  for (long e = 0; e < nk * n4 * n4; ++e) {
+    cc[e] = cc[e] * 0.5;
+    zz[e] = zz[e] * 0.25;
+  }
+  GMOD_CATCH
+}
+
+// fselfwithsoc.f90:1-539: the FLEX loop in a spinful basis.  Precision 0 runs
+// FP64 throughout; 1 and 2 run the transforms in FP32 (full fields: the
+// rolling-increment loop covers the spinless basis only).
+extern "C" void mkself_soc(gmod_z* sigmak, double* mu, const double* Vmat, const int64_t* kmap, const int64_t* invk,
+                           const int64_t* invs, const int64_t* olist, const int64_t* slist, const gmod_z* hamk,
+                           const double* eig, const gmod_z* uni, const double* mu_init, const double* rfill,
+                           const double* temp, const int64_t* scf_loop, const double* pp, const double* eps,
+                           const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Norb,
+                           const int64_t* Nchi, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+                           const int64_t* sub_sigma, const bool* sw_out, const bool* sw_in, const int64_t* m_diis,
+                           const bool* sw_rescale, const double* sigma_scale) {
+  GMOD_TRY
+  auto g = api_lock();
+  FlexIn A;
+  A.Smat = nullptr; A.Cmat = nullptr; A.eig = eig;
+  A.kmap = kmap; A.invk = invk; A.olist = olist;
+  A.hamk = (const zd*)hamk; A.uni = (const zd*)uni;
+  A.mu_init = *mu_init; A.rfill = *rfill; A.temp = *temp; A.pp = *pp; A.eps = *eps; A.sigma_scale = *sigma_scale;
+  A.scf_loop = *scf_loop; A.Nkall = *Nkall; A.Nk = *Nk; A.Nw = *Nw; A.Nx = *Nx; A.Ny = *Ny; A.Nz = *Nz;
+  A.sub_sigma = *sub_sigma; A.m_diis = *m_diis;
+  A.No = (int)*Norb; A.Nchi = (int)*Nchi;
+  A.sw_out = *sw_out; A.sw_in = *sw_in; A.sw_rescale = *sw_rescale; A.sw_tail = false;
+  A.soc = true; A.Vmat = Vmat; A.invs = invs; A.slist = slist;
+  if (ctx().precision >= 1) mkself_driver<float>(A, (zd*)sigmak, mu, false);
   else mkself_driver<double>(A, (zd*)sigmak, mu, false);
   GMOD_CATCH
 }

@@ -77,6 +77,37 @@ struct GreenLoad {
   __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(zd); }
 };
 
+// G_ab of a spinful basis on the full grid (fchi0withsoc.f90:57-75,
+// fselfwithsoc.f90:627-635): a point reached through -k reads
+// s_a s_b G_{-b,-a}(k, w) at positive w and s_a s_b conj G_{-a,-b}(k, w) at
+// negative w (time reversal; -a the spin-flipped orbital ai, -b = bi); a
+// point of the irreducible set reads G_ab(k, w) and conj G_ba(k, w)
+template <typename R>
+struct SocGreenLoad {
+  const zd* G;
+  int a, b, ai, bi, No;
+  double sg;
+  MeshDev m;
+  using Key = int;  // packed 2 k + flag (Mesh::kf_s)
+  __device__ Key key(unsigned sp) const { return m.kf_s[sp]; }
+  using Raw = zd;
+  __device__ Raw raw(Key kf, unsigned, unsigned tu) const {
+    const int t = (int)tu;
+    const bool lo = t < m.Nw;
+    const int j = lo ? t : m.M - 1 - t;
+    const bool rev = (kf & 1) != 0;
+    const int p = rev ? (lo ? bi : ai) : (lo ? a : b);
+    const int r = rev ? (lo ? ai : bi) : (lo ? b : a);
+    const long NkNw = m.Nk * (long)m.Nw;
+    const zd v = G[(kf >> 1) + m.Nk * (long)j + NkNw * (p + (long)No * r)];
+    return rev ? v * sg : v;
+  }
+  __device__ cpx<R> finish(const Raw& v, unsigned tu) const { return cpx<R>((int)tu < m.Nw ? v : conj(v)); }
+  __device__ cpx<R> value(Key kf, unsigned sp, unsigned t) const { return finish(raw(kf, sp, t), t); }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+  __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(zd); }
+};
+
 // bosonic V_(m,l) (fself.f90:203-224, feliash.f90:543-564): slot 0 = nu_0,
 // slot Nw = the nu -> infinity value `stat`, slots t and M-t = +-nu_t
 template <typename R, typename S = double>
@@ -451,6 +482,7 @@ void mul_acc(cpx<R>* acc, const cpx<R>* A, const cpx<R>* B, bool first, long n, 
 
 // ---- elementwise
 void conj_copy(zd* dst, const zd* src, long n, hipStream_t s);
+void conj_copy(zd* dst, const zd* src, long n, double sign, hipStream_t s);  // dst = sign conj(src), sign = +-1
 void threshold(zd* a, long n, double eps, hipStream_t s);
 // dst(r, t) = src(-r, -t) on an (Nx, Ny, Nz, M) grid (the operand a
 // (-r, -tau) product reads, laid out to be read in place)

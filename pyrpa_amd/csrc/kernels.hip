@@ -101,6 +101,20 @@ void conj_copy(zd* dst, const zd* src, long n, hipStream_t s) {
   hipLaunchKernelGGL(k_conj_copy, dim3(nblocks(n)), dim3(TPB), 0, s, dst, src, n);
   GMOD_CHECK_LAUNCH();
 }
+__global__ void k_conj_copy_neg(zd* dst, const zd* src, long n) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const zd v = src[e];
+    dst[e] = zd(-v.re, v.im);
+  }
+}
+void conj_copy(zd* dst, const zd* src, long n, double sign, hipStream_t s) {
+  if (sign > 0) {
+    conj_copy(dst, src, n, s);
+    return;
+  }
+  hipLaunchKernelGGL(k_conj_copy_neg, dim3(nblocks(n)), dim3(TPB), 0, s, dst, src, n);
+  GMOD_CHECK_LAUNCH();
+}
 
 // |Re| < eps -> Re = 0, then |Im| < eps -> Im = 0 (fchi.f90:499-500)
 __global__ void k_threshold(zd* a, long n, double eps) {

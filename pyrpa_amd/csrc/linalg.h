@@ -24,6 +24,10 @@ void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, cons
 // transforms)
 void rpa_vertex_fused(const zd* chi_in, zd* chi_out, long batch, BatchLayout L, int n, const double* S,
                       const double* C, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s);
+// spinful FLEX vertex in a batch (fUJ.f90:367-399): out = -V + V (I + chi0 V)^-1 chi0 V,
+// chi0 thresholded at thr (> 0) as it is read; out may equal chi_in
+void soc_vertex(const zd* chi_in, zd* out, long batch, BatchLayout L, int n, const double* V, double thr, int* err,
+                hipStream_t s);
 // X = (I + alpha A W)^-1 A  (fchi.f90:319-350, fself.f90:783-816); out may alias nothing
 void rpa_solve(zd* out, BatchLayout Lo, const zd* A, BatchLayout La, long batch, int n, const double* W,
                double alpha, int* err, hipStream_t s);

@@ -691,6 +691,74 @@ void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, cons
   GMOD_CHECK_LAUNCH();
 }
 
+// ---------------------------------------------------------------- spinful vertex
+// V_sigma = -V + V X with X = (I + chi0 V)^-1 chi0 V per (q, i nu) (fUJ.f90:367-399),
+// chi0 read with the 1e-9 threshold of the loop applied (thr > 0).  One
+// wavefront per matrix; chi0, I + chi0 V and the right-hand side live in LDS,
+// or, when three n x n complex matrices exceed it, in a per-work-group slice
+// of the global workspace gws.  out may alias chi_in.
+__global__ void __launch_bounds__(64)
+k_soc_vertex(const zd* chi_in, zd* out, BatchLayout L, long batch, int n, const double* __restrict__ V, double thr,
+             zd* gws, int* err) {
+  extern __shared__ unsigned char sm[];
+  __shared__ int shi[2];
+  const int nn = n * n;
+  zd* A = gws ? gws + (size_t)blockIdx.x * 3 * nn : reinterpret_cast<zd*>(sm);
+  zd* M = A + nn;
+  zd* X = M + nn;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+    for (int e = tid; e < nn; e += nt) {
+      zd a = chi_in[b * L.bs + e * L.es];
+      if (thr > 0.0) {
+        if (fabs(a.re) < thr) a.re = 0.0;
+        if (fabs(a.im) < thr) a.im = 0.0;
+      }
+      A[e] = a;
+    }
+    __syncthreads();
+    for (int e = tid; e < nn; e += nt) {
+      const int p = e % n, q = e / n;
+      zd av(0, 0);
+      for (int r = 0; r < n; ++r) av += A[p + n * r] * V[r + n * q];
+      M[e] = zd(p == q ? 1.0 : 0.0, 0) + av;
+      X[e] = av;
+    }
+    __syncthreads();
+    if (!lds_lu_solve(M, X, n, n, shi)) {
+      if (tid == 0) atomicOr(err, 1);
+      __syncthreads();
+      continue;
+    }
+    for (int e = tid; e < nn; e += nt) {
+      const int p = e % n, q = e / n;
+      zd v(-V[e], 0);
+      for (int r = 0; r < n; ++r) v += X[r + n * q] * V[p + n * r];
+      out[b * L.bs + e * L.es] = v;
+    }
+    __syncthreads();
+  }
+}
+
+void soc_vertex(const zd* chi_in, zd* out, long batch, BatchLayout L, int n, const double* V, double thr, int* err,
+                hipStream_t s) {
+  if (batch <= 0) return;
+  const size_t shm = 3 * (size_t)n * n * sizeof(zd);
+  if (shm <= 160 * 1024) {
+    const unsigned grid = (unsigned)std::min<long>(batch, 256L * 64);
+    hipLaunchKernelGGL(k_soc_vertex, dim3(grid), dim3(64), shm, s, chi_in, out, L, batch, n, V, thr, (zd*)nullptr,
+                       err);
+    GMOD_CHECK_LAUNCH();
+    return;
+  }
+  const long grid = std::min<long>(batch, 2048L);
+  DBuf<zd> ws((size_t)grid * 3 * n * n);
+  hipLaunchKernelGGL(k_soc_vertex, dim3((unsigned)grid), dim3(64), 0, s, chi_in, out, L, batch, n, V, thr, ws.p,
+                     err);
+  GMOD_CHECK_LAUNCH();
+  GMOD_HIP(hipStreamSynchronize(s));  // ws is released at return
+}
+
 __global__ void __launch_bounds__(64)
 k_rpa_solve(zd* out, BatchLayout Lo, const zd* Ain, BatchLayout La, long batch, int n, const double* W,
             double alpha, int* err) {

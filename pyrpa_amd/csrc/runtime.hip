@@ -489,6 +489,64 @@ extern "C" void get_scmat(double* S, double* C, const int64_t* ol, const int64_t
     }
 }
 
+// Spinful on-site vertex (fUJ.f90:109-251).  ol, site (Nchi), sl, invs (Norb)
+// as the reference passes them; u(p, q) and jv(p, q) give the intra-orbital U
+// (p == q), the inter-orbital U' and the Hund / pair-hopping J between
+// spinful orbitals p and q (1-based).  Vmat(j, i) holds the value of the pair
+// (i, j): density channels of equal spins (C - S)/2, of opposite spins
+// (C + S)/2, the spin-flip channel -S.
+template <class UF, class JF>
+static void vmat_soc(double* V, const int64_t* ol, const int64_t* sl, const int64_t* site, const int64_t* invs,
+                     long n, UF u, JF jv) {
+  for (long e = 0; e < n * n; ++e) V[e] = 0;
+  auto s = [&](long p) { return sl[p - 1]; };
+  auto fl = [&](long p) { return invs[p - 1]; };
+  for (long i = 0; i < n; ++i)
+    for (long j = 0; j < n; ++j) {
+      if (site[i] != site[j]) continue;
+      const long a = ol[i], b = ol[i + n], c = ol[j], d = ol[j + n];
+      double v = 0;
+      if (s(a) == s(b) && s(c) == s(d)) {
+        if (s(a) == s(c)) {
+          if (a == d && c == b) v = 0;  // intra-orbital and exchange terms vanish
+          else if (a == b && c == d) v = u(a, c) - jv(a, c);
+          else if (a == c && b == d) v = jv(a, b) - u(a, b);
+        } else {
+          if (a == b && c == d) v = a == fl(c) ? u(a, a) : u(a, c);
+          else if (a == fl(c) && b == fl(d)) v = jv(a, b);
+          else if (a == fl(d) && c == fl(b)) v = jv(a, b);
+        }
+      } else if (s(a) == s(c) && s(b) == s(d)) {
+        if (a == fl(b) && c == fl(d)) v = a == c ? -u(a, a) : -jv(a, c);
+        else if (a == c && b == d) v = -u(a, b);
+        else if (a == fl(d) && c == fl(b)) v = -jv(a, b);
+      }
+      V[j + n * i] = v;
+    }
+}
+
+// fUJ.f90:109-180: uniform U, U' = U - 2J, J
+extern "C" void get_vmat_soc(double* Vmat, const int64_t* ol, const int64_t* sl, const int64_t* site,
+                             const int64_t* invs, const double* Uval, const double* Jval, const int64_t* Nchi,
+                             const int64_t* Norb) {
+  (void)Norb;
+  const double U = *Uval, J = *Jval, Up = U - 2 * J;
+  // u(p, q): U for the same orbital of the other spin (p == -q), U' otherwise
+  auto u = [&](long p, long q) { return p == q || p == invs[q - 1] ? U : Up; };
+  auto jv = [&](long, long) { return J; };
+  vmat_soc(Vmat, ol, sl, site, invs, *Nchi, u, jv);
+}
+
+// fUJ.f90:182-251: orbital-dependent Umat, Jmat (Norb x Norb, spinful indices)
+extern "C" void get_vmat_soc_orb(double* Vmat, const int64_t* ol, const int64_t* sl, const int64_t* site,
+                                 const int64_t* invs, const double* Umat, const double* Jmat, const int64_t* Nchi,
+                                 const int64_t* Norb) {
+  const long No = *Norb;
+  auto u = [&](long p, long q) { return Umat[(p - 1) + No * (q - 1)]; };
+  auto jv = [&](long p, long q) { return Jmat[(p - 1) + No * (q - 1)]; };
+  vmat_soc(Vmat, ol, sl, site, invs, *Nchi, u, jv);
+}
+
 // fUJ.f90:59-107
 extern "C" void get_scmat_orb(double* S, double* C, const int64_t* ol, const int64_t* site, const double* Um,
                               const double* Jm, const int64_t* Nchi_, const int64_t* Norb_) {

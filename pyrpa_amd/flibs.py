@@ -57,9 +57,17 @@ _SIGS = {
     "get_initial_delta_": [_z, _d, _z, _i, _i, _pi, _pi, _pi, _pi, _pi],
     "conv_delta_orb_to_band": [_z, _z, _z, _d, _i, _pi, _pi, _pi, _pi, _pi],
     "remap_delta_": [_z, _z, _d, _i, _pi, _pi, _pi, _pi, _pi],
-    # libs/flibs/_interaction.py:7-24
+    # libs/flibs/_interaction.py:7-45
     "get_scmat": [_d, _d, _i, _i, _pd, _pd, _pi],
     "get_scmat_orb": [_d, _d, _i, _i, _d, _d, _pi, _pi],
+    "get_vmat_soc": [_d, _i, _i, _i, _i, _pd, _pd, _pi, _pi],
+    "get_vmat_soc_orb": [_d, _i, _i, _i, _i, _d, _d, _pi, _pi],
+    # spinful basis: libs/flibs/_selfenergy.py:38-60, _susceptibility.py:141-173
+    "mkself_soc": [_z, _pd, _d, _i, _i, _i, _i, _i, _z, _d, _z, _pd, _pd, _pd, _pi, _pd, _pd, _pi, _pi, _pi, _pi,
+                   _pi, _pi, _pi, _pi, _pi, _pb, _pb, _pi, _pb, _pd],
+    "get_chi0_soc": [_z, _d, _d, _i, _d, _z, _i, _i, _i, _i, _i, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi],
+    "get_chis_chic_soc": [_z, _z, _z, _z, _d, _i, _i, _i, _i, _pi, _pi, _pi, _pi],
+    "gmod_v_soc_flex": [_z, _d, _pi, _pi, _pi],
     # libs/flibs/_hamiltonian.py:7-106
     "openmp_params": [_pi, _pb],
     "gen_ham": [_z, _d, _z, _d, _pi, _pi, _pi],
@@ -449,6 +457,97 @@ def mkself(Smat, Cmat, kmap, invk, olist, hamk, eig, uni, mu, fill, temp, Nw, Nx
           dbl(pp), dbl(eps), i64(Nkall), i64(Nk), i64(Nw), i64(Norb), i64(Nchi), i64(Nx), i64(Ny), i64(Nz),
           i64(sub_sigma), byref(c_bool(sw_out)), byref(c_bool(sw_in)), i64(m_diis), byref(c_bool(sw_rescale)),
           byref(c_bool(sw_tail)), dbl(sigma_scale))
+    return sigmak, mu_self.value
+
+
+# =========================================================================== spinful basis (SOC)
+def gen_Vmatrix(olist, slist, site, invs, U, J):
+    """SOC interaction vertex [Nchi, Nchi]  [_interaction.py:83-100]"""
+    Nchi, Norb = len(olist), len(slist)
+    V = np.zeros((Nchi, Nchi), dtype=np.float64)
+    _call("get_vmat_soc", V, olist, np.asarray(slist, dtype=np.int64), np.asarray(site, dtype=np.int64),
+          np.asarray(invs, dtype=np.int64), dbl(U), dbl(J), i64(Nchi), i64(Norb))
+    return V
+
+
+def gen_Vmatrix_orb(olist, slist, site, invs, Umat, Jmat):
+    """[_interaction.py:102-119]"""
+    Nchi, Norb = len(olist), len(Umat)
+    V = np.zeros((Nchi, Nchi), dtype=np.float64)
+    _call("get_vmat_soc_orb", V, olist, np.asarray(slist, dtype=np.int64), np.asarray(site, dtype=np.int64),
+          np.asarray(invs, dtype=np.int64), np.asarray(Umat, dtype=np.float64), np.asarray(Jmat, dtype=np.float64),
+          i64(Nchi), i64(Norb))
+    return V
+
+
+def get_chi0_soc(Vmat, Gk, olist, slist, kmap, invk, invs, temp, Nx, Ny, Nz):
+    """(chi[Nchi,Nchi,Nw,Nk], sgnsig[Norb,Norb], sgnsig2[Nchi,Nchi], invschi[Nchi])
+    [_susceptibility.py:496-528]"""
+    Norb, Nchi = len(slist), len(olist)
+    Nkall, Nk, Nw = len(kmap), Gk.shape[3], Gk.shape[2]
+    chi = np.zeros((Nchi, Nchi, Nw, Nk), dtype=np.complex128)
+    sgnsig = np.zeros((Norb, Norb), dtype=np.float64)
+    sgnsig2 = np.zeros((Nchi, Nchi), dtype=np.float64)
+    invschi = np.zeros(Nchi, dtype=np.int64)
+    _call("get_chi0_soc", chi, sgnsig, sgnsig2, invschi, np.asarray(Vmat, dtype=np.float64), np.ascontiguousarray(Gk),
+          kmap, invk, np.asarray(invs, dtype=np.int64), olist, np.asarray(slist, dtype=np.int64), dbl(temp), i64(Nx),
+          i64(Ny), i64(Nz), i64(Nw), i64(Nk), i64(Nkall), i64(Nchi), i64(Norb))
+    return chi, sgnsig, sgnsig2, invschi
+
+
+def _soc_orb_list(olist, slist, invs):
+    """1-based index of each chi pair among the spin-up orbital pairs
+    (get_orb_list of _susceptibility.py:545-556): pair n = i nu + j of the
+    spin-up orbitals up[] is (up[j], up[i]); a spin-down orbital counts as its
+    spin-flipped partner."""
+    ol = np.asarray(olist)
+    sl = np.asarray(slist)
+    inv = np.asarray(invs)
+    up = np.nonzero(sl == 1)[0] + 1
+    pos = {(int(up[j]), int(up[i])): i * len(up) + j + 1 for i in range(len(up)) for j in range(len(up))}
+    out = np.zeros(len(ol), dtype=np.int64)
+    for i, (a, b) in enumerate(ol):
+        a = a if sl[a - 1] == 1 else inv[a - 1]
+        b = b if sl[b - 1] == 1 else inv[b - 1]
+        out[i] = pos.get((int(a), int(b)), 0)
+    return out
+
+
+def get_chis_chic_soc(chi, Vmat, olist, slist, invs):
+    """(chic, chiszz, chispm) [Nchi/4, Nchi/4, Nk]  [_susceptibility.py:530-566]"""
+    Norb, Nchi = len(slist), len(olist)
+    Nk, Nw = chi.shape[3], chi.shape[2]
+    n4 = Nchi // 4
+    chic = np.zeros((n4, n4, Nk), dtype=np.complex128)
+    chiszz = np.zeros((n4, n4, Nk), dtype=np.complex128)
+    chispm = np.zeros((n4, n4, Nk), dtype=np.complex128)
+    _call("get_chis_chic_soc", chic, chiszz, chispm, np.ascontiguousarray(chi), np.asarray(Vmat, dtype=np.float64),
+          _soc_orb_list(olist, slist, invs), olist, np.asarray(slist, dtype=np.int64), np.asarray(invs, dtype=np.int64),
+          i64(Nk), i64(Nw), i64(Nchi), i64(Norb))
+    return chic, chiszz, chispm
+
+
+def get_V_soc_flex(chi, Vmat):
+    """-V + V (I + chi0 V)^-1 chi0 V per (q, i nu) (fUJ.f90:367-399; internal to
+    mkself_soc in the reference): a new array [Nchi,Nchi,Nw,Nk]"""
+    out = np.array(chi, dtype=np.complex128, order='C')
+    Nchi, _, Nw, Nk = out.shape
+    _call("gmod_v_soc_flex", out, np.asarray(Vmat, dtype=np.float64), i64(Nk), i64(Nw), i64(Nchi))
+    return out
+
+
+def mkself_soc(Vmat, kmap, invk, invs, olist, slist, hamk, eig, uni, mu, fill, temp, Nw, Nx, Ny, Nz, sw_out, sw_in,
+               sub_sigma=1, scf_loop=300, eps=1.0e-4, pp=0.3, m_diis=5, sw_rescale=False, sigma_scale=1.0):
+    """FLEX loop in a spinful basis  [_selfenergy.py:129-174]"""
+    Nkall, Nk, Nchi = len(kmap), len(hamk), len(Vmat)
+    Norb = hamk.shape[1]
+    mu_self = c_double()
+    sigmak = np.zeros((Norb, Norb, Nw, Nk), dtype=np.complex128)
+    _call("mkself_soc", sigmak, byref(mu_self), np.asarray(Vmat, dtype=np.float64), kmap, invk,
+          np.asarray(invs, dtype=np.int64), olist, np.asarray(slist, dtype=np.int64), np.ascontiguousarray(hamk),
+          np.ascontiguousarray(eig), np.ascontiguousarray(uni), dbl(mu), dbl(fill), dbl(temp), i64(scf_loop), dbl(pp),
+          dbl(eps), i64(Nkall), i64(Nk), i64(Nw), i64(Norb), i64(Nchi), i64(Nx), i64(Ny), i64(Nz), i64(sub_sigma),
+          byref(c_bool(sw_out)), byref(c_bool(sw_in)), i64(m_diis), byref(c_bool(sw_rescale)), dbl(sigma_scale))
     return sigmak, mu_self.value
 
 

@@ -128,6 +128,59 @@ def bubble_exact(st, mlist):
     return out
 
 
+PAULI = {'x': np.array([[0, 1], [1, 0]], dtype=complex), 'y': np.array([[0, -1j], [1j, 0]]),
+         'z': np.array([[1, 0], [0, -1]], dtype=complex)}
+
+
+def spinful(st, lam=0.0, alpha=0.0, U=0.0, J=0.0):
+    """The spinful version of a model on RVEC_NN (index s No + o, spin up
+    first, as the reference driver's default slist / invs): H(k) for each
+    spin, plus for two orbitals an atomic lam/2 L_z s_z term and a Rashba-like
+    hopping i alpha (s_y along x, -s_x along y).  Every added term keeps time
+    reversal, H(-k) = s_y H(k)^* s_y, the symmetry the spinful k / -k
+    unfolding assumes.  Adds slist, invs and the SOC vertex Vmat(U, J)."""
+    No = st['Norb']
+    Ns = 2 * No
+    hs = np.array([np.kron(np.eye(2), h) for h in st['ham_r']])
+    if lam:
+        assert No == 2
+        Lz = np.array([[0, -1j], [1j, 0]])
+        hs[0] += 0.5 * lam * np.kron(PAULI['z'], Lz)
+    if alpha:
+        for i, R in enumerate(st['rvec']):
+            R = tuple(int(v) for v in R)
+            if R == (1, 0, 0):
+                hs[i] += np.kron(1j * alpha * PAULI['y'], np.eye(No))
+            elif R == (-1, 0, 0):
+                hs[i] += np.kron(1j * alpha * PAULI['y'], np.eye(No)).conj().T
+            elif R == (0, 1, 0):
+                hs[i] += np.kron(-1j * alpha * PAULI['x'], np.eye(No))
+            elif R == (0, -1, 0):
+                hs[i] += np.kron(-1j * alpha * PAULI['x'], np.eye(No)).conj().T
+    out = _bundle(hs, st['rvec'], Ns, st['Nx'], st['Ny'], st['Nz'], st['Nw'], st['temp'], st['mu'], U, J)
+    out['slist'], out['invs'] = O.soc_spin_lists(Ns)
+    out['Vmat'] = O.get_vmat_soc(out['olist'], out['slist'], out['site'], out['invs'], U, J)
+    return out
+
+
+def full_green(st, Nw=None, mu=None):
+    """G0_ab(k, slot) on every grid point (computed there, no unfolding),
+    slots 0..Nw-1 = w_j, slot 2Nw-1-j = -w_j through G(k,-w) = G(k,w)^dagger:
+    array [a, b, x, y, z, slot], (a, b) in the Fortran order of the library's
+    Gk(k, w, a, b) (the Python array Gk[b, a, w, k])."""
+    Nw = st['Nw'] if Nw is None else Nw
+    mu = st['mu'] if mu is None else mu
+    Nx, Ny, Nz = st['Nx'], st['Ny'], st['Nz']
+    ix, iy, iz = np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing='ij')
+    kf = np.stack([ix.ravel() / Nx, iy.ravel() / Ny, iz.ravel() / Nz], axis=1)
+    e, u = O.get_eig(O.gen_ham(kf, st['ham_r'], st['rvec']))
+    G = np.swapaxes(O.gen_Green0(e, u, mu, st['temp'], Nw), 0, 1)   # Fortran (a, b): [a, b, w, k]
+    Gm = np.conj(np.swapaxes(G, 0, 1))[:, :, ::-1]                  # G(k,-w) at slots Nw.. (reversed)
+    full = np.concatenate([G, Gm], axis=2)                  # [a, b, slot, k]
+    No = full.shape[0]
+    return np.moveaxis(full, 2, 3).reshape(No, No, Nx, Ny, Nz, 2 * Nw)
+
+
 def G0_at(st, Nw, mu=None):
     mu = st['mu'] if mu is None else mu
     return O.gen_Green0(st['eig'], st['uni'], mu, st['temp'], Nw)

@@ -34,7 +34,8 @@ def test_header_declares_the_hot_path_entry_points():
                  "lin_eliash", "mkfk_trs_nsoc_", "get_initial_delta_", "conv_delta_orb_to_band", "remap_delta_",
                  "get_scmat", "get_scmat_orb", "generate_irr_kpoint_inv", "gen_ham", "get_eig", "get_ffermi",
                  "openmp_params", "gmod_last_error", "gmod_get_chis_chic_all", "gmod_nn_create",
-                 "gmod_nn_train_epoch"]:
+                 "gmod_nn_train_epoch", "mkself_soc", "get_chi0_soc", "get_chis_chic_soc", "get_vmat_soc",
+                 "get_vmat_soc_orb", "get_chi_irr", "chiq_map", "pade_with_trace", "gmod_wannier_r"]:
         assert name in syms
 
 
@@ -93,6 +94,18 @@ def test_setup_host_entry_points_match_oracle(run):
     assert np.abs(F.get_ffermi(e, 0.1, 0.05) - O.get_ffermi(e, 0.1, 0.05)).max() < 1e-15
     w = F.gen_kpoint_weight(st['invk'], len(st['klist']))
     assert np.array_equal(w, O.kweight(st['invk'], len(st['klist'])))
+    # the spinful vertex (its spin-degenerate channels are pinned in test_soc_oracle.py);
+    # the orbital-dependent form with uniform matrices is the uniform one
+    for Ns in (2, 4, 6):
+        slist, invs = O.soc_spin_lists(Ns)
+        ol, site = O.get_chi_orb_list(Ns, [Ns])
+        V = F.gen_Vmatrix(ol, slist, site, invs, 1.1, 0.25)
+        assert np.array_equal(V, O.get_vmat_soc(ol, slist, site, invs, 1.1, 0.25))
+        Um = np.full((Ns, Ns), 1.1 - 2 * 0.25)
+        for a in range(Ns):
+            Um[a, a] = Um[a, invs[a] - 1] = 1.1
+        Vo = F.gen_Vmatrix_orb(ol, slist, site, invs, Um, np.full((Ns, Ns), 0.25))
+        assert np.array_equal(Vo, V)
 
 
 @RUN

@@ -131,7 +131,7 @@ def test_writers_match_oracle(tmp_path):
     assert np.array_equal(z["rvec"], rv) and len(rv) == 5
     assert np.abs(z["sigma"] - ref).max() < 1e-13 * np.abs(ref).max()
     assert np.allclose(z["iw"], (2 * np.arange(8) + 1) * np.pi * temp) and z["mu"] == 0.3 and z["temp"] == temp
-    assert open(f + ".dat").readline().startswith('# Self-energy: Norb=2 N_iw=8 Nr=6 mu=0.30000000')
+    assert open(f + ".dat").readline().startswith('# Self-energy: Norb=2 N_iw=8 Nr=5 mu=0.30000000')
     # gap on every grid point (the spin-orbit layout)
     kfull = kmap / np.array([Nx, Ny, Nz], dtype=float)
     c, gap = _closed_form(Nx, Ny, Nz, Nw, No, False, 5, kfull)
