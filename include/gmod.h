@@ -199,6 +199,25 @@ void get_chis_chic_soc(gmod_z* chic, gmod_z* chiszz, gmod_z* chispm, const gmod_
 /* libs/src/fUJ.f90:367-399 (get_V_soc_flex, internal to mkself_soc): in place on
  * chi(Nk,Nw,Nchi,Nchi), -V + V (I + chi0 V)^-1 chi0 V */
 void gmod_v_soc_flex(gmod_z* chi, const double* Vmat, const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi);
+/* libs/src/feliashwithsoc.f90:1-424 [libs/flibs/_eliashberg.py:183-223]: delta(Nkall,Nw,Norb,Norb)
+ * on the full zone; chi leaves holding V_Delta; the eigenvalue through gmod_last_lambda */
+void lin_eliash_soc(gmod_z* delta, gmod_z* chi, const gmod_z* Gk, const gmod_z* uni, const double* init_delta,
+                    const double* Vmat, const double* sgnsig, const double* sgnsig2, const double* prt,
+                    const int64_t* olist, const int64_t* slist, const int64_t* kmap, const int64_t* invk,
+                    const int64_t* invs, const int64_t* invschi, const double* temp, const double* eps,
+                    const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi,
+                    const int64_t* Norb, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+                    const int64_t* itemax, const int64_t* gap_sym, const int64_t* arnoldi_m);
+/* No counterpart in the reference (which prints it): the eigenvalue of the last lin_eliash_soc call */
+double gmod_last_lambda(void);
+/* libs/src/feliashwithsoc.f90:426-518 (bind name mkfk_trs_soc_) [libs/flibs/_eliashberg.py:357-376] */
+void mkfk_trs_soc_(gmod_z* fk, const gmod_z* Gk, const gmod_z* delta, const double* sgnsig, const int64_t* slist,
+                   const int64_t* invk, const int64_t* invs, const int64_t* Nkall, const int64_t* Nk,
+                   const int64_t* Nw, const int64_t* Norb, const int64_t* gap_sym);
+/* libs/src/feliashwithsoc.f90:750-794 [libs/flibs/_eliashberg.py:325-343]: deltab(Nkall,Norb,Norb) */
+void conv_delta_orb_to_band_soc(gmod_z* deltab, const gmod_z* delta, const gmod_z* uni, const int64_t* invk,
+                                const int64_t* invs, const int64_t* slist, const int64_t* Norb,
+                                const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw);
 /* libs/src/fUJ.f90:109-180 [libs/flibs/_interaction.py:83-100] */
 void get_vmat_soc(double* Vmat, const int64_t* ol, const int64_t* sl, const int64_t* site, const int64_t* invs,
                   const double* Uval, const double* Jval, const int64_t* Nchi, const int64_t* Norb);

@@ -1118,6 +1118,58 @@ def get_initial_delta(init_delta, uni, kmap, invk, Nw, gap_sym):
     return np.ascontiguousarray(D.T)
 
 
+def _arnoldi_two_pass(apply, norm, inner, init, m_dim, info):
+    """Two-pass Arnoldi of solve_arnoldi (feliash.f90, feliashwithsoc.f90:190-387):
+    pass 1 from the initial gap takes the most negative Ritz value lambda_-;
+    unless a Ritz value >= 0.1 appeared, pass 2 on (A + shift) with
+    shift = -lambda_-, projected off the pass-1 Ritz vector, takes the largest.
+    Returns (unit eigenvector, lambda)."""
+    def arnoldi(v0, shift, defl):
+        V = [v0]
+        H = np.zeros((m_dim + 1, m_dim), dtype=complex)
+        m_act = m_dim
+        for j in range(m_dim):
+            nd = apply(V[j])
+            if defl is not None:
+                nd = nd + shift * V[j]
+                h = inner(defl, nd)
+                nd = nd - h * defl
+            for ii in range(j + 1):
+                h = inner(V[ii], nd)
+                H[ii, j] = h
+                nd = nd - h * V[ii]
+            beta = norm(nd)
+            H[j + 1, j] = beta
+            if beta < 1e-14:
+                m_act = j + 1
+                break
+            if j < m_dim - 1:
+                V.append(nd * (1.0 / beta))
+        ev, vr = _geev_right(H[:m_act, :m_act])
+        return V, ev, vr, m_act
+
+    V, ev, vr, m_act = arnoldi(init(), 0.0, None)
+    idx = int(np.argmin(ev.real))
+    lam1 = ev[idx].real
+    info['pass1'] = ev
+    if ev.real.max() >= 0.1:
+        idx = int(np.argmax(ev.real))
+        D = sum(vr[ii, idx] * V[ii] for ii in range(m_act))
+        return D * (1.0 / norm(D)), ev[idx].real
+    d1 = sum(vr[ii, idx] * V[ii] for ii in range(m_act))
+    d1 = d1 * (1.0 / norm(d1))
+    shift = -lam1
+    v0 = init()
+    h = inner(d1, v0)
+    v0 = v0 - h * d1
+    v0 = v0 * (1.0 / norm(v0))
+    V, ev, vr, m_act = arnoldi(v0, shift, d1)
+    info['pass2'] = ev - shift
+    idx = int(np.argmax(ev.real))
+    D = sum(vr[ii, idx] * V[ii] for ii in range(m_act))
+    return D * (1.0 / norm(D)), ev[idx].real - shift
+
+
 def linearized_eliashberg(chi, Gk, uni, init_delta, Smat, Cmat, olist, plist, kmap, invk,
                           Nx, Ny, Nz, temp, gap_sym, eps=1.0e-5, itemax=300, arnoldi_m=10,
                           return_info=False):
@@ -1148,55 +1200,7 @@ def linearized_eliashberg(chi, Gk, uni, init_delta, Smat, Cmat, olist, plist, km
     info = {}
     lam_out = 0.0
     if arnoldi_m > 0:
-        m_dim = int(arnoldi_m)
-
-        def arnoldi(v0, shift, defl):
-            V = [v0]
-            H = np.zeros((m_dim + 1, m_dim), dtype=complex)
-            m_act = m_dim
-            for j in range(m_dim):
-                nd = apply(V[j])
-                if defl is not None:
-                    nd = nd + shift * V[j]
-                    h = inner(defl, nd)
-                    nd = nd - h * defl
-                for ii in range(j + 1):
-                    h = inner(V[ii], nd)
-                    H[ii, j] = h
-                    nd = nd - h * V[ii]
-                beta = norm(nd)
-                H[j + 1, j] = beta
-                if beta < 1e-14:
-                    m_act = j + 1
-                    break
-                if j < m_dim - 1:
-                    V.append(nd * (1.0 / beta))
-            ev, vr = _geev_right(H[:m_act, :m_act])
-            return V, ev, vr, m_act
-
-        V, ev, vr, m_act = arnoldi(init(), 0.0, None)
-        idx = int(np.argmin(ev.real))
-        lam1 = ev[idx].real
-        info['pass1'] = ev
-        if ev.real.max() >= 0.1:
-            idx = int(np.argmax(ev.real))
-            lam_out = ev[idx].real
-            D = sum(vr[ii, idx] * V[ii] for ii in range(m_act))
-            D = D * (1.0 / norm(D))
-        else:
-            d1 = sum(vr[ii, idx] * V[ii] for ii in range(m_act))
-            d1 = d1 * (1.0 / norm(d1))
-            shift = -lam1
-            v0 = init()
-            h = inner(d1, v0)
-            v0 = v0 - h * d1
-            v0 = v0 * (1.0 / norm(v0))
-            V, ev, vr, m_act = arnoldi(v0, shift, d1)
-            info['pass2'] = ev - shift
-            idx = int(np.argmax(ev.real))
-            lam_out = ev[idx].real - shift
-            D = sum(vr[ii, idx] * V[ii] for ii in range(m_act))
-            D = D * (1.0 / norm(D))
+        D, lam_out = _arnoldi_two_pass(apply, norm, inner, init, int(arnoldi_m), info)
     else:
         lam1 = 0.0
         delta1 = None
@@ -1799,3 +1803,238 @@ def mkself_soc(Vmat, kmap, invk, invs, olist, slist, hamk, eig, uni, mu, fill, t
     if return_info:
         return out, mu, info
     return out, mu
+
+
+# --------------------------------------------------------------------------- #
+#  linearized Eliashberg in a spinful basis (feliashwithsoc.f90).  The gap
+#  lives on the full zone: Python [No, No, Nw, Nkall], Fortran view [i, j, a, b].
+# --------------------------------------------------------------------------- #
+def get_initial_delta_soc(init_delta, uni, kmap, invk, slist, invs, Nw, gap_sym):
+    """feliashwithsoc.f90:665-748: flag-0 points -U diag(d) (s_b conj U(-b,:))^T,
+    flag-1 points (s_a conj U(-a,:)) diag(d) U^T, the same at every frequency,
+    same-spin blocks zero for gap_sym > 0, normalised to sqrt(2 sum |.|^2) = 1."""
+    Uf = np.asarray(uni).T                    # uni(a, n, k)
+    df0 = np.asarray(init_delta).T            # init(k, n)
+    sl = np.asarray(slist, dtype=float)
+    inv = np.asarray(invs) - 1
+    Nkall = len(invk)
+    No = Uf.shape[0]
+    D = np.zeros((Nkall, No, No), dtype=complex)
+    for i in range(Nkall):
+        k, f = invk[i, 0] - 1, invk[i, 1]
+        U = Uf[:, :, k]
+        d = df0[k]
+        if f == 0:
+            D[i] = -(U * d[None, :]) @ (sl[:, None] * np.conj(U[inv, :])).T
+        elif f == 1:
+            D[i] = (sl[:, None] * np.conj(U[inv, :]) * d[None, :]) @ U.T
+    if gap_sym > 0:
+        D[:, np.outer(sl, sl) >= 0] = 0.0
+    full = np.repeat(D[:, None], Nw, axis=1)  # [i, j, a, b]
+    full = full / np.sqrt(2.0 * np.sum(np.abs(full) ** 2))
+    return np.ascontiguousarray(full.T)
+
+
+def mkfk_trs_soc(Gk, delta, sgnsig, slist, invk, invs, gap_sym):
+    """F = -G(k) Delta(k) G(-k)^dagger on the full zone with its spin structure
+    (feliashwithsoc.f90:426-518).  Returns Python [No, No, Nw, Nkall]."""
+    Gf = np.asarray(Gk).T                     # [k, j, a, b]
+    Df = np.asarray(delta).T                  # [i, j, a, b]
+    sg = np.asarray(sgnsig).T                 # sgnsig(a, b)
+    sl = np.asarray(slist)
+    inv = np.asarray(invs) - 1
+    Nkall, Nw, No, _ = Df.shape
+    F = np.zeros_like(Df)
+    for i in range(Nkall):
+        k = invk[i, 0] - 1
+        G = Gf[k]                              # [j, a, b]
+        if invk[i, 1] == 0:
+            g1 = G
+            g2 = sg.T[None] * G[:, inv][:, :, inv]          # g2(l, n) = sg(n, l) G(-l, -n)
+        else:
+            g1 = sg.T[None] * np.swapaxes(G[:, inv][:, :, inv], 1, 2)   # g1(l, n) = sg(n, l) G(-n, -l)
+            g2 = np.swapaxes(G, 1, 2)                        # g2(l, n) = G(n, l)
+        F[i] = -(g1 @ Df[i]) @ np.conj(np.swapaxes(g2, 1, 2))
+    for l in range(No):
+        if sl[l] != 1:
+            continue
+        for m in range(No):
+            if sl[m] == 1:
+                if gap_sym < 0:
+                    F[:, :, inv[l], inv[m]] = -np.conj(F[:, :, m, l])
+                else:
+                    F[:, :, m, l] = 0.0
+                    F[:, :, inv[l], inv[m]] = 0.0
+            elif inv[m] == l:
+                F[:, :, inv[l], inv[m]] = F[:, :, m, l].real
+    for l in range(No):
+        if sl[l] != 1:
+            continue
+        for m in range(No):
+            if sl[m] != -1 or gap_sym == -10:
+                continue
+            F[:, :, inv[m], inv[l]] = F[:, :, m, l] if gap_sym < 0 else -F[:, :, m, l]
+    return np.ascontiguousarray(F.T)
+
+
+def _full_boson_soc(Vf, m, l, li, mi, sg2, static, kmap, invk, Nx, Ny, Nz):
+    """V_(m,l)(q, slot) of a spinful basis (feliashwithsoc.f90:545-573): -k
+    points read sg2 V(q)_(l',m') at +nu and sg2 conj V(q)_(m',l') at -nu."""
+    Nk, Nw = Vf.shape[:2]
+    M = 2 * Nw
+    out = np.zeros((Nx * Ny * Nz, M), dtype=complex)
+    irr = invk[:, 0] - 1
+    fl = invk[:, 1]
+    sp = _spatial(kmap, Nx, Ny, Nz)
+    d, r = fl == 0, fl == 1
+    out[sp[d], :Nw] = Vf[irr[d], :, m, l]
+    out[sp[r], :Nw] = sg2 * Vf[irr[r], :, li, mi]
+    out[:, Nw] = static
+    out[sp[d], Nw + 1:] = np.conj(Vf[irr[d], 1:, l, m])[:, ::-1]
+    out[sp[r], Nw + 1:] = (sg2 * np.conj(Vf[irr[r], 1:, mi, li]))[:, ::-1]
+    return out.reshape(Nx, Ny, Nz, M)
+
+
+def _soc_gap_pairs(ol, sl, inv, gap_sym):
+    """(l, m) pairs mkdelta_soc convolves (feliashwithsoc.f90:538-540)."""
+    out = []
+    for l in range(len(ol)):
+        for m in range(len(ol)):
+            l1, m2 = ol[l, 0], ol[m, 1]
+            ss = sl[l1] * sl[m2]
+            if not (gap_sym < 0 or (gap_sym > 0 and ss < 0)):
+                continue
+            if not ((gap_sym == -10 and ss < 0) or sl[l1] == 1):
+                continue
+            if not (sl[m2] == 1 or l1 >= inv[m2]):
+                continue
+            out.append((l, m))
+    return out
+
+
+def mkdelta_soc(fk, Vdelta, Vmat, kmap, invk, invs, invschi, olist, slist, Nx, Ny, Nz, gap_sym):
+    """newdelta = sum over the pairs of V_Delta (*) F on the full zone, then its
+    spin structure (feliashwithsoc.f90:520-663); un-weighted.  Python in / out
+    [No, No, Nw, Nkall]."""
+    Ff = np.asarray(fk).T                     # [i, j, a, b]
+    Vf = np.asarray(Vdelta).T
+    Vm = np.asarray(Vmat).T
+    ol = olist_f(olist)
+    sl = np.asarray(slist)
+    inv = np.asarray(invs) - 1
+    isc = np.asarray(invschi) - 1
+    _, sg2 = soc_signs(olist, slist)
+    Nkall, Nw, No, _ = Ff.shape
+    M = 2 * Nw
+    sp = _spatial(kmap, Nx, Ny, Nz)
+    minus = invk[:, 2] - 1
+    nd = np.zeros_like(Ff)
+    for l, m in _soc_gap_pairs(ol, sl, inv, gap_sym):
+        v = fft4(_full_boson_soc(Vf, m, l, isc[l], isc[m], sg2[m, l], -Vm[m, l], kmap, invk, Nx, Ny, Nz))
+        g = np.zeros((Nx * Ny * Nz, M), dtype=complex)
+        g[sp, :Nw] = Ff[:, :, ol[l, 1], ol[m, 0]]
+        g[sp, Nw:] = -Ff[minus, :, ol[m, 0], ol[l, 1]][:, ::-1]
+        tmp = ifft4(v * fft4(g.reshape(Nx, Ny, Nz, M))).reshape(-1, M)
+        nd[:, :, ol[l, 0], ol[m, 1]] += tmp[sp, :Nw]
+    for l in range(No):
+        if sl[l] != 1:
+            continue
+        for m in range(No):
+            if sl[m] == 1:
+                nd[:, :, inv[l], inv[m]] = -np.conj(nd[:, :, m, l])
+            elif inv[m] == l:
+                nd[:, :, l, m] = nd[:, :, l, m].real
+            elif l > inv[m]:
+                nd[:, :, inv[m], inv[l]] = np.conj(nd[:, :, l, m])
+    for l in range(No):
+        if sl[l] != 1:
+            continue
+        for m in range(No):
+            if sl[m] != -1 or gap_sym == -10:
+                continue
+            nd[:, :, inv[l], inv[m]] = -nd[:, :, l, m] if gap_sym > 0 else nd[:, :, l, m]
+    return np.ascontiguousarray(nd.T)
+
+
+def linearized_eliashberg_soc(chi, Gk, uni, init_delta, Vmat, sgnsig, sgnsig2, plist, slist, olist, kmap, invk,
+                              invs, invschi, Nx, Ny, Nz, temp, gap_sym, eps=1.0e-4, itemax=300, arnoldi_m=10,
+                              return_info=False):
+    """lin_eliash_soc (feliashwithsoc.f90:1-424; wrapper _eliashberg.py:183-223).
+    Returns (delta[No,No,Nw,Nkall], lambda); the reference wrapper returns
+    delta and prints lambda."""
+    weight = temp / len(kmap)
+    Vd = get_V_soc_flex(chi, Vmat)
+    Nw = np.asarray(Gk).shape[2]
+
+    def apply(v):
+        fk = mkfk_trs_soc(Gk, np.ascontiguousarray(v.T), sgnsig, slist, invk, invs, gap_sym)
+        return -np.asarray(mkdelta_soc(fk, Vd, Vmat, kmap, invk, invs, invschi, olist, slist, Nx, Ny, Nz,
+                                       gap_sym)).T * weight
+
+    def norm(f):
+        return np.sqrt(2.0 * np.sum(np.abs(f) ** 2))
+
+    def inner(u, v):
+        return 2.0 * np.sum(np.conj(u) * v)
+
+    def init():
+        return np.asarray(get_initial_delta_soc(init_delta, uni, kmap, invk, slist, invs, Nw, gap_sym)).T.copy()
+
+    info = {}
+    if arnoldi_m > 0:
+        D, lam = _arnoldi_two_pass(apply, norm, inner, init, int(arnoldi_m), info)
+    else:
+        norm2 = 0.0
+        for i_eig in (1, 2):
+            D = init()
+            count = 0
+            for it in range(1, itemax + 1):
+                nd = apply(D) + D * norm2
+                nrm = norm(nd)
+                inorm = 1.0 / nrm
+                lam_rq = inner(D, nd).real
+                vp = np.sum(np.abs(nd * inorm - D) ** 2)
+                vn = np.sum(np.abs(nd * inorm + D) ** 2)
+                verr = np.sqrt(2.0 * min(vp, vn))
+                if verr < eps:
+                    if lam_rq - norm2 > 0.1 or lam_rq - norm2 < 0.0:
+                        break
+                    if verr < eps * 1e-2:
+                        count += 1
+                    if count > 30:
+                        break
+                D = nd * inorm
+            lam = lam_rq - norm2
+            if i_eig == 1 and lam > 0.0:
+                break
+            norm2 = nrm
+        D = nd * (1.0 / norm(nd))
+    out = np.ascontiguousarray(D.T)
+    if return_info:
+        return out, lam, info
+    return out, lam
+
+
+def conv_delta_orb_to_band_soc(delta, uni, invk, invs, slist):
+    """Band-basis gap at the first frequency (feliashwithsoc.f90:750-794):
+    flag 0: U^dagger Delta (s_n U(-n,:)), flag 1: (-s_n U(-n,:))^T-column form
+    Delta conj(U).  Returns Python [No, No, Nkall]."""
+    Df = np.asarray(delta).T                  # [i, j, a, b]
+    Uf = np.asarray(uni).T                    # uni(a, n, k)
+    sl = np.asarray(slist, dtype=float)
+    inv = np.asarray(invs) - 1
+    Nkall, _, No, _ = Df.shape
+    out = np.zeros((Nkall, No, No), dtype=complex)
+    for i in range(Nkall):
+        k, f = invk[i, 0] - 1, invk[i, 1]
+        U = Uf[:, :, k]
+        if f == 0:
+            L = np.conj(U).T
+            R = sl[:, None] * U[inv, :]
+        elif f == 1:
+            L = (-sl[None, :] * U[inv, :].T)
+            R = np.conj(U)
+        else:
+            continue
+        out[i] = L @ Df[i, 0] @ R
+    return np.ascontiguousarray(out.T)

@@ -84,6 +84,12 @@ class ConvEngine {
   void prepare_gap(const zd* Vd, const double* S_h, const double* C_h, bool singlet, hipStream_t s);
   // newdelta (un-weighted, Hermitised) <- sum V_Delta (*) F  (feliash.f90:486-635)
   void gap(zd* nd, const zd* F, const double* prt, bool singlet, hipStream_t s);
+  // Eliashberg in a spinful basis (feliashwithsoc.f90:520-612): the pairs
+  // mkdelta_soc convolves for gap_sym and FFT[V_Delta] of each (cached when
+  // memory allows); gap_soc: nd(Nkall,Nw,No,No) <- the sum over them of
+  // V (*) F, F on the full zone, before the spin symmetrisation
+  void prepare_gap_soc(const zd* Vd, const double* Vmat_h, const int64_t* invschi, long gap_sym, hipStream_t s);
+  void gap_soc(zd* nd, const zd* F, hipStream_t s);
 
   const Mesh& mesh;
   const OrbPairs& op;
@@ -107,6 +113,7 @@ class ConvEngine {
   // the gather of a spatial-first inverse transform
   GatherStore<R> gather_sf(zd* dst, double w, zd J, const zd* sawc, int acc, hipStream_t s);
   std::vector<std::array<int, 2>> gap_pairs;
+  std::vector<SocBosonLoad<R>> soc_vload;  // per gap pair of a spinful basis
   bool vft_cached = false;
   const zd* Vd_dev = nullptr;
   std::vector<double> S_cache, C_cache;

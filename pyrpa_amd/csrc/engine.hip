@@ -557,6 +557,84 @@ void ConvEngine<R>::gap(zd* nd, const zd* F, const double* prt, bool singlet, hi
   hermitize_lower(nd, mesh.Nk, mesh.Nw, No, s);
 }
 
+template <typename R>
+void ConvEngine<R>::prepare_gap_soc(const zd* Vd, const double* Vmat_h, const int64_t* invschi, long gap_sym,
+                                    hipStream_t s) {
+  if (!op.soc) throw Error("prepare_gap_soc: the orbital pairs carry no spin");
+  if (!mesh.nfull_ok) throw Error("lin_eliash_soc: invk(3,:) must give the -k point of every k");
+  const int Nchi = op.Nchi;
+  const long NC = mesh.Nk * (long)mesh.Nw;
+  const MeshDev md = mesh_dev(mesh);
+  // the pairs of feliashwithsoc.f90:538-540: every spin for a triplet gap,
+  // opposite spins of (l1, m2) for a singlet; of those the up-l1 ones (or,
+  // for the parity-mixed gap_sym = -10, every opposite-spin one), and of the
+  // opposite-spin ones the upper triangle l1 >= -m2
+  gap_pairs.clear();
+  soc_vload.clear();
+  for (int l = 0; l < Nchi; ++l)
+    for (int m = 0; m < Nchi; ++m) {
+      const int l1 = op.ol[l][0], m2 = op.ol[m][1];
+      const double ss = op.spin[l1] * op.spin[m2];
+      if (!(gap_sym < 0 || (gap_sym > 0 && ss < 0))) continue;
+      if (!((gap_sym == -10 && ss < 0) || op.spin[l1] > 0)) continue;
+      if (!(op.spin[m2] > 0 || l1 >= op.inv[m2])) continue;
+      const long li = invschi[l] - 1, mi = invschi[m] - 1;
+      if (li < 0 || li >= Nchi || mi < 0 || mi >= Nchi) throw Error("lin_eliash_soc: invschi entry outside [1,Nchi]");
+      gap_pairs.push_back({l, m});
+      soc_vload.push_back(SocBosonLoad<R>{Vd, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m),
+                                          NC * (li + (long)Nchi * mi), NC * (mi + (long)Nchi * li),
+                                          op.psign[m + Nchi * l], zd(-Vmat_h[m + Nchi * l], 0), md});
+    }
+  size_t free_b = 0, total_b = 0;
+  GMOD_HIP(hipMemGetInfo(&free_b, &total_b));
+  const double need = (double)gap_pairs.size() * N * sizeof(cpx<R>);
+  vft_cached = need < 0.5 * (double)free_b;
+  if (!vft_cached) return;
+  Vft.alloc(gap_pairs.size() * (size_t)N);
+  for (size_t q = 0; q < gap_pairs.size(); ++q) {
+    cpx<R>* g = Vft.p + q * N;
+    p4->exec_fused(soc_vload[q], GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
+  }
+}
+
+template <typename R>
+void ConvEngine<R>::gap_soc(zd* nd, const zd* F, hipStream_t s) {
+  const int No = op.No;
+  const long NW = mesh.Nkall * (long)mesh.Nw;
+  if (Ff.n != (size_t)(No * No) * N) Ff.alloc((size_t)No * No * N);
+  if (acc.n != (size_t)N) acc.alloc(N);
+  const MeshDev md = mesh_dev(mesh);
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
+  std::vector<char> need((size_t)No * No, 0);
+  for (auto& lm : gap_pairs) need[op.ol[lm[0]][1] + No * op.ol[lm[1]][0]] = 1;
+  for (int b = 0; b < No; ++b)
+    for (int a = 0; a < No; ++a) {
+      if (!need[a + No * b]) continue;
+      cpx<R>* g = grid(Ff, a, b);
+      p4->exec_fused(FullAnomLoad<R>{F, NW * (a + (long)No * b), NW * (b + (long)No * a), md},
+                     GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
+    }
+  GMOD_HIP(hipMemsetAsync(nd, 0, NW * No * No * sizeof(zd), s));
+  for (int m2 = 0; m2 < No; ++m2)
+    for (int l1 = 0; l1 < No; ++l1) {
+      bool first = true;
+      for (size_t q = 0; q < gap_pairs.size(); ++q) {
+        const int l = gap_pairs[q][0], m = gap_pairs[q][1];
+        if (op.ol[l][0] != l1 || op.ol[m][1] != m2) continue;
+        const cpx<R>* fg = grid(Ff, op.ol[l][1], op.ol[m][0]);
+        if (vft_cached)
+          mul_acc<R>(acc.p, Vft.p + q * N, fg, first, N, s);
+        else
+          p4->exec_fused(soc_vload[q], MulAccStore<R>{acc.p, fg, first ? 1 : 0, (unsigned)mesh.Nkall}, work.p,
+                         tmp.p, -1, (R)1, s);
+        first = false;
+      }
+      if (first) continue;
+      p4->exec_fused(GridLoad<R>{acc.p, (unsigned)mesh.Nkall}, FullGatherStore<R>{nd + NW * (l1 + (long)No * m2), 0, md},
+                     acc.p, tmp.p, +1, invN, s);
+    }
+}
+
 // ====================================================================== increments
 // Rolling increments for the mixed-precision FLEX loop.  Between iterations
 // n-1 and n the bubble and the self-energy change by

@@ -573,6 +573,93 @@ struct EliashIn {
   int No, Nchi;
 };
 
+// Two-pass Arnoldi of the linearized gap equation (feliash.f90 and
+// feliashwithsoc.f90:190-387, solve_arnoldi): pass 1 builds an md-dimensional
+// Krylov basis from the initial gap (modified Gram-Schmidt) and takes the
+// most negative Ritz value lambda_-; unless a Ritz value >= 0.1 already
+// appeared, pass 2 runs on (A + shift) with shift = -lambda_-, projected off
+// the pass-1 Ritz vector, and takes its largest Ritz value.  apply(w, v):
+// w = A v; norm, inner: the gap's norm and inner product; init(v): the unit
+// initial gap.  D receives the unit eigenvector; returns lambda.
+template <class Apply, class Norm, class Inner, class Init>
+static double arnoldi_two_pass(int md, size_t n, zd* D, Apply apply, Norm norm, Inner inner, Init init,
+                               hipStream_t s, int verbose) {
+  std::vector<DBuf<zd>> V(md);
+  for (auto& v : V) v.alloc(n);
+  DBuf<zd> d1(n), nd(n);
+  auto arnoldi = [&](double shift, bool defl, std::vector<zd>& Hm, int& m_act) {
+    Hm.assign((size_t)(md + 1) * md, zd(0, 0));
+    m_act = md;
+    for (int j = 0; j < md; ++j) {
+      apply(nd.p, V[j].p);
+      if (defl) {
+        axpy(nd.p, V[j].p, zd(shift, 0), n, s);
+        zd h = inner(d1.p, nd.p);
+        axpy(nd.p, d1.p, zd(0, 0) - h, n, s);
+      }
+      for (int ii = 0; ii <= j; ++ii) {
+        zd h = inner(V[ii].p, nd.p);
+        Hm[ii + (md + 1) * j] = h;
+        axpy(nd.p, V[ii].p, zd(0, 0) - h, n, s);
+      }
+      double beta = norm(nd.p);
+      Hm[j + 1 + (md + 1) * j] = zd(beta, 0);
+      if (beta < 1.0e-14) { m_act = j + 1; break; }
+      if (j < md - 1) {
+        copy(V[j + 1].p, nd.p, n, s);
+        scale(V[j + 1].p, n, 1.0 / beta, s);
+      }
+    }
+  };
+  auto ritz = [&](const std::vector<zd>& Hm, int m_act, std::vector<zd>& ev) {
+    std::vector<zd> Am((size_t)m_act * m_act);
+    for (int q = 0; q < m_act; ++q)
+      for (int p = 0; p < m_act; ++p) Am[p + m_act * q] = Hm[p + (md + 1) * q];
+    if (host_eigvals(Am, m_act, ev)) throw Error("ZGEEV (Hessenberg QR) failed in Arnoldi");
+    return Am;
+  };
+  auto build = [&](zd* out, const std::vector<zd>& Am, int m_act, zd lam) {
+    std::vector<zd> y = host_eigvec(Am, m_act, lam);
+    GMOD_HIP(hipMemsetAsync(out, 0, n * sizeof(zd), s));
+    for (int ii = 0; ii < m_act; ++ii) axpy(out, V[ii].p, y[ii], n, s);
+    scale(out, n, 1.0 / norm(out), s);
+  };
+  init(V[0].p);
+  std::vector<zd> Hm, ev;
+  int m_act;
+  arnoldi(0.0, false, Hm, m_act);
+  auto Am = ritz(Hm, m_act, ev);
+  int idx = 0;
+  for (int i = 1; i < m_act; ++i)
+    if (ev[i].re < ev[idx].re) idx = i;
+  const double lam1 = ev[idx].re;
+  double maxre = ev[0].re;
+  int imax = 0;
+  for (int i = 1; i < m_act; ++i)
+    if (ev[i].re > maxre) { maxre = ev[i].re; imax = i; }
+  if (verbose) {
+    printf("Pass 1 Ritz values (real):\n");
+    for (int i = 0; i < m_act; ++i) printf("%4d%14.6f\n", i + 1, ev[i].re);
+  }
+  if (maxre >= 0.1) {
+    build(D, Am, m_act, ev[imax]);
+    return ev[imax].re;
+  }
+  build(d1.p, Am, m_act, ev[idx]);
+  const double shift = -lam1;
+  init(V[0].p);
+  zd h = inner(d1.p, V[0].p);
+  axpy(V[0].p, d1.p, zd(0, 0) - h, n, s);
+  scale(V[0].p, n, 1.0 / norm(V[0].p), s);
+  arnoldi(shift, true, Hm, m_act);
+  Am = ritz(Hm, m_act, ev);
+  imax = 0;
+  for (int i = 1; i < m_act; ++i)
+    if (ev[i].re > ev[imax].re) imax = i;
+  build(D, Am, m_act, ev[imax]);
+  return ev[imax].re - shift;
+}
+
 template <typename R>
 static double eliash_driver(const EliashIn& A, zd* delta_out) {
   hipStream_t s = ctx().stream;
@@ -611,82 +698,8 @@ static double eliash_driver(const EliashIn& A, zd* delta_out) {
   DBuf<zd> D(nS);
   double lam_out = 0.0;
   if (A.arnoldi_m > 0) {
-    const int md = (int)A.arnoldi_m;
-    std::vector<DBuf<zd>> V(md);
-    for (auto& v : V) v.alloc(nS);
-    DBuf<zd> d1(nS);
-    auto arnoldi = [&](double shift, bool defl, std::vector<zd>& Hm, int& m_act) {
-      Hm.assign((size_t)(md + 1) * md, zd(0, 0));
-      m_act = md;
-      for (int j = 0; j < md; ++j) {
-        apply(nd.p, V[j].p);
-        if (defl) {
-          axpy(nd.p, V[j].p, zd(shift, 0), nS, s);
-          zd h = inner(d1.p, nd.p);
-          axpy(nd.p, d1.p, zd(0, 0) - h, nS, s);
-        }
-        for (int ii = 0; ii <= j; ++ii) {
-          zd h = inner(V[ii].p, nd.p);
-          Hm[ii + (md + 1) * j] = h;
-          axpy(nd.p, V[ii].p, zd(0, 0) - h, nS, s);
-        }
-        double beta = norm(nd.p);
-        Hm[j + 1 + (md + 1) * j] = zd(beta, 0);
-        if (beta < 1.0e-14) { m_act = j + 1; break; }
-        if (j < md - 1) {
-          copy(V[j + 1].p, nd.p, nS, s);
-          scale(V[j + 1].p, nS, 1.0 / beta, s);
-        }
-      }
-    };
-    auto ritz = [&](const std::vector<zd>& Hm, int m_act, std::vector<zd>& ev) {
-      std::vector<zd> Am((size_t)m_act * m_act);
-      for (int q = 0; q < m_act; ++q)
-        for (int p = 0; p < m_act; ++p) Am[p + m_act * q] = Hm[p + (md + 1) * q];
-      if (host_eigvals(Am, m_act, ev)) throw Error("ZGEEV (Hessenberg QR) failed in Arnoldi");
-      return Am;
-    };
-    auto build = [&](zd* out, const std::vector<zd>& Am, int m_act, zd lam) {
-      std::vector<zd> y = host_eigvec(Am, m_act, lam);
-      GMOD_HIP(hipMemsetAsync(out, 0, nS * sizeof(zd), s));
-      for (int ii = 0; ii < m_act; ++ii) axpy(out, V[ii].p, y[ii], nS, s);
-      scale(out, nS, 1.0 / norm(out), s);
-    };
-    initial_delta(V[0].p, A.init_delta, A.uni, mesh, No, A.gap_sym, s);
-    std::vector<zd> Hm, ev;
-    int m_act;
-    arnoldi(0.0, false, Hm, m_act);
-    auto Am = ritz(Hm, m_act, ev);
-    int idx = 0;
-    for (int i = 1; i < m_act; ++i)
-      if (ev[i].re < ev[idx].re) idx = i;
-    const double lam1 = ev[idx].re;
-    double maxre = ev[0].re;
-    int imax = 0;
-    for (int i = 1; i < m_act; ++i)
-      if (ev[i].re > maxre) { maxre = ev[i].re; imax = i; }
-    if (verbose) {
-      printf("Pass 1 Ritz values (real):\n");
-      for (int i = 0; i < m_act; ++i) printf("%4d%14.6f\n", i + 1, ev[i].re);
-    }
-    if (maxre >= 0.1) {
-      lam_out = ev[imax].re;
-      build(D.p, Am, m_act, ev[imax]);
-    } else {
-      build(d1.p, Am, m_act, ev[idx]);
-      const double shift = -lam1;
-      initial_delta(V[0].p, A.init_delta, A.uni, mesh, No, A.gap_sym, s);
-      zd h = inner(d1.p, V[0].p);
-      axpy(V[0].p, d1.p, zd(0, 0) - h, nS, s);
-      scale(V[0].p, nS, 1.0 / norm(V[0].p), s);
-      arnoldi(shift, true, Hm, m_act);
-      Am = ritz(Hm, m_act, ev);
-      imax = 0;
-      for (int i = 1; i < m_act; ++i)
-        if (ev[i].re > ev[imax].re) imax = i;
-      lam_out = ev[imax].re - shift;
-      build(D.p, Am, m_act, ev[imax]);
-    }
+    auto init = [&](zd* v) { initial_delta(v, A.init_delta, A.uni, mesh, No, A.gap_sym, s); };
+    lam_out = arnoldi_two_pass((int)A.arnoldi_m, nS, D.p, apply, norm, inner, init, s, verbose);
   } else {
     // power method with shift + deflation (feliash.f90:67-140)
     double lam1 = 0.0, lam_phys = 0.0, lam_rq = 0.0, inorm = 1.0;
@@ -742,6 +755,120 @@ static double eliash_driver(const EliashIn& A, zd* delta_out) {
   D.download(delta_out, nS, s);
   GMOD_HIP(hipStreamSynchronize(s));
   return lam_out;
+}
+
+// ---------------------------------------------------------------- spinful Eliashberg
+struct EliashSocIn {
+  zd* chi;
+  const zd *Gk, *uni;
+  const double *init_delta, *Vmat, *sgnsig;
+  const int64_t *olist, *slist, *kmap, *invk, *invs, *invschi;
+  double temp, eps;
+  long Nkall, Nk, Nw, Nx, Ny, Nz, itemax, gap_sym, arnoldi_m;
+  int No, Nchi;
+};
+static double g_lambda_soc = 0.0;
+
+// lin_eliash_soc (feliashwithsoc.f90:1-424): the gap lives on the full zone,
+// norms and inner products are 2 x the plain sums over it
+template <typename R>
+static double eliash_soc_driver(const EliashSocIn& A, zd* delta_out) {
+  hipStream_t s = ctx().stream;
+  const int verbose = ctx().verbose;
+  const int No = A.No, Nchi = A.Nchi;
+  const long Nk = A.Nk, Nw = A.Nw, NkNw = Nk * Nw, Nkall = A.Nkall, NW = Nkall * Nw;
+  const size_t nF = (size_t)NW * No * No, nC = (size_t)NkNw * Nchi * Nchi, nG = (size_t)NkNw * No * No;
+  const double weight = A.temp / (double)Nkall;
+  Mesh mesh;
+  mesh.build(A.kmap, A.invk, Nkall, Nk, (int)A.Nx, (int)A.Ny, (int)A.Nz, (int)Nw, s);
+  OrbPairs op;
+  op.from_olist_soc(A.olist, Nchi, No, A.invs, A.slist);
+  Upload up{s};
+  DBuf<zd> chi, G, uni_d, Fk(nF), nd(nF), D(nF);
+  DBuf<double> V_d, sg_d, init_d;
+  DBuf<int64_t> invk_d;
+  DBuf<int> sl_d, inv_d, err(1);
+  err.zero(s);
+  std::vector<int> slh(No), invh(No);
+  for (int a = 0; a < No; ++a) { slh[a] = (int)A.slist[a]; invh[a] = op.inv[a]; }
+  up(chi, A.chi, nC);
+  up(G, A.Gk, nG);
+  up(uni_d, A.uni, (size_t)No * No * Nk);
+  up(V_d, A.Vmat, (size_t)Nchi * Nchi);
+  up(sg_d, A.sgnsig, (size_t)No * No);
+  up(init_d, A.init_delta, (size_t)Nk * No);
+  up(invk_d, A.invk, (size_t)3 * Nkall);
+  up(sl_d, slh.data(), (size_t)No);
+  up(inv_d, invh.data(), (size_t)No);
+  soc_vertex(chi.p, chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, V_d.p, 0.0, err.p, s);
+  check_err(err, s, "get_V_soc_flex");
+  chi.download(A.chi, nC, s);  // chi is inout: it leaves holding V_Delta (feliashwithsoc.f90:55)
+  if (verbose)
+    printf("V_delta max is %16.8E\nV_delta min is %16.8E\n", max_real(chi.p, nC, s), min_real(chi.p, nC, s));
+  ConvEngine<R> eng(mesh, op);
+  eng.prepare_gap_soc(chi.p, A.Vmat, A.invschi, A.gap_sym, s);
+  auto apply = [&](zd* w_out, const zd* v_in) {
+    anomalous_f_soc(Fk.p, G.p, v_in, invk_d.p, sg_d.p, sl_d.p, inv_d.p, Nkall, Nk, (int)Nw, No, A.gap_sym, s);
+    eng.gap_soc(w_out, Fk.p, s);
+    delta_sym_soc(w_out, sl_d.p, inv_d.p, NW, No, A.gap_sym, s);
+    scale(w_out, nF, -weight, s);
+  };
+  const long rest = No * No;
+  auto dot = [&](const zd* u, const zd* v) { return dot_w(u, v, nullptr, NW, rest, s); };
+  auto norm = [&](const zd* f) { return std::sqrt(2.0 * dot(f, f).re); };
+  auto inner = [&](const zd* u, const zd* v) { return dot(u, v) * 2.0; };
+  auto init = [&](zd* v) {
+    initial_delta_soc(v, init_d.p, uni_d.p, invk_d.p, sl_d.p, inv_d.p, Nkall, Nk, (int)Nw, No, A.gap_sym, s);
+    const double n2 = dot(v, v).re;
+    if (n2 <= 0.0) throw Error("get_initial_delta_soc: norm <= 0");
+    scale(v, nF, 1.0 / std::sqrt(2.0 * n2), s);
+  };
+  double lam = 0.0;
+  if (A.arnoldi_m > 0) {
+    lam = arnoldi_two_pass((int)A.arnoldi_m, nF, D.p, apply, norm, inner, init, s, verbose);
+  } else {
+    // shifted power method (feliashwithsoc.f90:59-110): newdelta = A delta +
+    // norm2 delta, norm2 = 0 for the first eigenvalue, then the norm of the
+    // first run's last iterate when that eigenvalue is not positive
+    double norm2 = 0.0, lam_rq = 0.0, nrm = 0.0;
+    for (int i_eig = 1; i_eig <= 2; ++i_eig) {
+      init(D.p);
+      int count = 0;
+      for (long it = 1; it <= A.itemax; ++it) {
+        apply(nd.p, D.p);
+        if (norm2 != 0.0) axpy(nd.p, D.p, zd(norm2, 0), nF, s);
+        nrm = norm(nd.p);
+        if (nrm <= 0.0) throw Error("lin_eliash_soc: norm <= 0 in iter_loop");
+        const double inorm = 1.0 / nrm;
+        lam_rq = inner(D.p, nd.p).re;
+        // vec_err = sqrt(2 min(sum|nd/|nd| - D|^2, sum|nd/|nd| + D|^2))
+        const double nn = dot(nd.p, nd.p).re, dd = dot(D.p, D.p).re, xr = dot(nd.p, D.p).re;
+        const double tpos = nn * inorm * inorm + dd - 2.0 * inorm * xr;
+        const double tneg = nn * inorm * inorm + dd + 2.0 * inorm * xr;
+        const double verr = std::sqrt(2.0 * std::max(0.0, std::min(tpos, tneg)));
+        if (verbose) printf("%3ld lambda_elsh=%16.8E  vec_err=%12.4E\n", it, lam_rq - norm2, verr);
+        if (verr < A.eps) {
+          if (lam_rq - norm2 > 1.0e-1) break;
+          if (lam_rq - norm2 < 0.0) break;
+          if (verr < A.eps * 1.0e-2) ++count;
+          if (count > 30) break;
+        }
+        copy(D.p, nd.p, nF, s);
+        scale(D.p, nF, inorm, s);
+      }
+      lam = lam_rq - norm2;
+      if (verbose) printf("eliash=%16.8E\n", lam);
+      if (i_eig == 1 && lam > 0.0) break;
+      norm2 = nrm;
+    }
+    nrm = norm(nd.p);
+    if (nrm <= 0.0) throw Error("lin_eliash_soc: final norm <= 0");
+    copy(D.p, nd.p, nF, s);
+    scale(D.p, nF, 1.0 / nrm, s);
+  }
+  D.download(delta_out, nF, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  return lam;
 }
 
 }  // namespace gmod
@@ -1253,6 +1380,122 @@ extern "C" void mkself_soc(gmod_z* sigmak, double* mu, const double* Vmat, const
   A.soc = true; A.Vmat = Vmat; A.invs = invs; A.slist = slist;
   if (ctx().precision >= 1) mkself_driver<float>(A, (zd*)sigmak, mu, false);
   else mkself_driver<double>(A, (zd*)sigmak, mu, false);
+  GMOD_CATCH
+}
+
+// feliashwithsoc.f90:1-424: the linearized gap equation in a spinful basis.
+// delta(Nkall,Nw,Norb,Norb) on the full zone; chi leaves holding V_Delta.
+// The eigenvalue (printed by the reference) is returned by gmod_last_lambda.
+extern "C" void lin_eliash_soc(gmod_z* delta, gmod_z* chi, const gmod_z* Gk, const gmod_z* uni,
+                               const double* init_delta, const double* Vmat, const double* sgnsig,
+                               const double* sgnsig2, const double* prt, const int64_t* olist, const int64_t* slist,
+                               const int64_t* kmap, const int64_t* invk, const int64_t* invs, const int64_t* invschi,
+                               const double* temp, const double* eps, const int64_t* Nkall, const int64_t* Nk,
+                               const int64_t* Nw, const int64_t* Nchi, const int64_t* Norb, const int64_t* Nx,
+                               const int64_t* Ny, const int64_t* Nz, const int64_t* itemax, const int64_t* gap_sym,
+                               const int64_t* arnoldi_m) {
+  GMOD_TRY
+  auto g = api_lock();
+  (void)sgnsig2;  // the partner signs follow from slist
+  (void)prt;
+  EliashSocIn A;
+  A.chi = (zd*)chi; A.Gk = (const zd*)Gk; A.uni = (const zd*)uni;
+  A.init_delta = init_delta; A.Vmat = Vmat; A.sgnsig = sgnsig;
+  A.olist = olist; A.slist = slist; A.kmap = kmap; A.invk = invk; A.invs = invs; A.invschi = invschi;
+  A.temp = *temp; A.eps = *eps;
+  A.Nkall = *Nkall; A.Nk = *Nk; A.Nw = *Nw; A.Nx = *Nx; A.Ny = *Ny; A.Nz = *Nz;
+  A.itemax = *itemax; A.gap_sym = *gap_sym; A.arnoldi_m = *arnoldi_m;
+  A.No = (int)*Norb; A.Nchi = (int)*Nchi;
+  g_lambda_soc = ctx().precision >= 1 ? eliash_soc_driver<float>(A, (zd*)delta)
+                                      : eliash_soc_driver<double>(A, (zd*)delta);
+  GMOD_CATCH
+}
+
+extern "C" double gmod_last_lambda(void) { return g_lambda_soc; }
+
+// feliashwithsoc.f90:426-518 (bind name mkfk_trs_soc_): F(Nkall,Nw,Norb,Norb)
+extern "C" void mkfk_trs_soc_(gmod_z* fk, const gmod_z* Gk, const gmod_z* delta, const double* sgnsig,
+                              const int64_t* slist, const int64_t* invk, const int64_t* invs, const int64_t* Nkall,
+                              const int64_t* Nk, const int64_t* Nw, const int64_t* Norb, const int64_t* gap_sym) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int No = (int)*Norb;
+  const long nkall = *Nkall, nk = *Nk, nw = *Nw;
+  const size_t nF = (size_t)nkall * nw * No * No;
+  std::vector<int> slh(No), invh(No);
+  for (int a = 0; a < No; ++a) {
+    slh[a] = (int)slist[a];
+    invh[a] = (int)invs[a] - 1;
+    if (invh[a] < 0 || invh[a] >= No) throw Error("mkfk_trs_soc: invs entry outside [1,Norb]");
+  }
+  for (long i = 0; i < nkall; ++i)
+    if (invk[3 * i] < 1 || invk[3 * i] > nk) throw Error("mkfk_trs_soc: invk(1,:) outside [1,Nk]");
+  Upload up{s};
+  DBuf<zd> G, D, F(nF);
+  DBuf<double> sg;
+  DBuf<int64_t> iv;
+  DBuf<int> sl, inv;
+  up(G, (const zd*)Gk, (size_t)nk * nw * No * No);
+  up(D, (const zd*)delta, nF);
+  up(sg, sgnsig, (size_t)No * No);
+  up(iv, invk, (size_t)3 * nkall);
+  up(sl, slh.data(), (size_t)No);
+  up(inv, invh.data(), (size_t)No);
+  anomalous_f_soc(F.p, G.p, D.p, iv.p, sg.p, sl.p, inv.p, nkall, nk, (int)nw, No, *gap_sym, s);
+  F.download((zd*)fk, nF, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+// feliashwithsoc.f90:750-794: band-basis gap at the first frequency,
+// deltab(Nkall,Norb,Norb) (host; output formatting)
+extern "C" void conv_delta_orb_to_band_soc(gmod_z* deltab, const gmod_z* delta, const gmod_z* uni,
+                                           const int64_t* invk, const int64_t* invs, const int64_t* slist,
+                                           const int64_t* Norb, const int64_t* Nkall, const int64_t* Nk,
+                                           const int64_t* Nw) {
+  GMOD_TRY
+  const int No = (int)*Norb;
+  const long nkall = *Nkall;
+  (void)Nw;
+  const zd* D = (const zd*)delta;
+  const zd* U = (const zd*)uni;
+  zd* out = (zd*)deltab;
+  std::vector<zd> L((size_t)No * No), Rm((size_t)No * No), T((size_t)No * No);
+  for (long i = 0; i < nkall; ++i) {
+    const long k = invk[3 * i] - 1, f = invk[3 * i + 1];
+    if (f != 0 && f != 1) {
+      for (int q = 0; q < No; ++q)
+        for (int p = 0; p < No; ++p) out[i + nkall * (p + (long)No * q)] = zd(0, 0);
+      continue;
+    }
+    if (k < 0 || k >= *Nk) throw Error("conv_delta_orb_to_band_soc: invk(1,:) outside [1,Nk]");
+    auto u = [&](long a, long n) { return U[a + No * (n + (long)No * k)]; };
+    for (int r = 0; r < No; ++r)
+      for (int c = 0; c < No; ++c) {
+        const long inv_c = invs[c] - 1, inv_r = invs[r] - 1;
+        if (f == 0) {
+          L[r + No * c] = conj(u(c, r));                  // U^dagger
+          Rm[r + No * c] = u(inv_r, c) * (double)slist[r];  // s_n U(-n, :)
+        } else {
+          L[r + No * c] = u(inv_c, r) * (-(double)slist[c]);  // column n: -s_n U(-n, :)
+          Rm[r + No * c] = conj(u(r, c));
+        }
+      }
+    // deltab = L delta(i, w_1) Rm; delta(i, 1, a, q) at i + Nkall Nw (a + No q)
+    for (int q = 0; q < No; ++q)
+      for (int p = 0; p < No; ++p) {
+        zd t(0, 0);
+        for (int a = 0; a < No; ++a) t += L[p + No * a] * D[i + nkall * *Nw * (a + (long)No * q)];
+        T[p + No * q] = t;
+      }
+    for (int q = 0; q < No; ++q)
+      for (int p = 0; p < No; ++p) {
+        zd t(0, 0);
+        for (int b = 0; b < No; ++b) t += T[p + No * b] * Rm[b + No * q];
+        out[i + nkall * (p + (long)No * q)] = t;
+      }
+  }
   GMOD_CATCH
 }
 

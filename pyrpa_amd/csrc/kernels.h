@@ -26,6 +26,9 @@ struct Mesh {
   DBuf<int> kgat_s;   // per spatial index: irreducible k if this is its flag-0 point, else -1
   std::vector<int> kgat_host;  // the same on the host
   DBuf<int> kf_s;     // per spatial index: 2 irr_s + (flag_s != 0), one load for both
+  DBuf<int> full_s;   // per spatial index: full-list index i of the point (data on the full zone)
+  DBuf<int> nfull_s;  // per spatial index: full-list index of -k, invk(3,i)-1
+  bool nfull_ok = true;  // every invk(3,i) lies in [1, Nkall]
   DBuf<double> kw;    // full-BZ multiplicity of irreducible k
   std::vector<double> kw_host;
   bool complete = true;  // every irreducible k has a direct (flag 0) full-BZ point
@@ -40,6 +43,7 @@ struct MeshDev {
   const int *irr_s, *flag_s, *kgat_s, *rev, *kf_s;  // per spatial index
   long Nkall, Nk;
   int Nw, M;
+  const int *full_s = nullptr, *nfull_s = nullptr;
   __host__ double irr_elems() const { return (double)Nk * Nw; }  // elements of one irreducible (k, w) slice
 };
 
@@ -137,6 +141,73 @@ struct BosonLoad {
   __device__ cpx<R> value(Key kf, unsigned sp, unsigned t) const { return finish(raw(kf, sp, t), t); }
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
   __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(cpx<S>); }  // V_ml, V_lm slices
+};
+
+// bosonic V_(m,l) of a spinful basis (feliashwithsoc.f90:545-573): a point
+// reached through -k reads s V(q)_(l',m') at positive nu and
+// s conj V(q)_(m',l') at negative nu, l' / m' the spin-flipped pairs and
+// s = s_l1 s_l2 s_m1 s_m2; slot Nw holds `stat`
+template <typename R>
+struct SocBosonLoad {
+  const zd* V;
+  long off_ml, off_lm, off_rp, off_rn;
+  double sg;
+  zd stat;
+  MeshDev m;
+  using Key = int;
+  __device__ Key key(unsigned sp) const { return m.kf_s[sp]; }
+  __device__ cpx<R> value(Key kf, unsigned, unsigned tu) const {
+    const int t = (int)tu;
+    if (t == m.Nw) return cpx<R>(stat);
+    const bool hi = t > m.Nw;
+    const int j = hi ? m.M - t : t;
+    const bool rev = (kf & 1) != 0;
+    const long off = rev ? (hi ? off_rn : off_rp) : (hi ? off_lm : off_ml);
+    zd v = V[(kf >> 1) + m.Nk * (long)j + off];
+    if (hi) v = conj(v);
+    return cpx<R>(rev ? v * sg : v);
+  }
+  __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+  __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(zd); }
+};
+
+// anomalous F_(a,b) held on the full zone (Nkall, Nw, No, No) of a spinful
+// basis (feliashwithsoc.f90:575-580): slot t < Nw reads F_ab(k, w_t), slot
+// M-1-j reads -F_ba(-k, w_j)
+template <typename R>
+struct FullAnomLoad {
+  const zd* F;
+  long oab, oba;
+  MeshDev m;
+  __device__ cpx<R> operator()(unsigned sp, unsigned tu) const {
+    const int t = (int)tu;
+    const long NkW = m.Nkall;
+    if (t < m.Nw) return cpx<R>(F[m.full_s[sp] + NkW * (long)t + oab]);
+    const int j = m.M - 1 - t;
+    const zd v = F[m.nfull_s[sp] + NkW * (long)j + oba];
+    return cpx<R>(zd(-v.re, -v.im));
+  }
+};
+
+// scatter slots t < Nw of every spatial point to the full-zone array
+// dst(Nkall, Nw) (feliashwithsoc.f90:606-612), added when acc
+template <typename R>
+struct FullGatherStore {
+  zd* dst;
+  int acc;
+  MeshDev m;
+  struct Pre { long o; };
+  __device__ Pre prep(unsigned sp, unsigned t) const {
+    return Pre{(int)t < m.Nw ? m.full_s[sp] + m.Nkall * (long)t : -1};
+  }
+  __device__ void commit(unsigned, unsigned, const Pre& q, cpx<R> v) const {
+    if (q.o < 0) return;
+    dst[q.o] = acc ? dst[q.o] + zd(v) : zd(v);
+  }
+  __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
+  __device__ bool skip(unsigned t) const { return (int)t >= m.Nw; }
+  __device__ bool skip_box(unsigned) const { return false; }
+  __host__ double read_fraction(bool t_box) const { return t_box ? 1.0 : (double)m.Nw / m.M; }
 };
 
 // anomalous F_(a,b) with parity factors (feliash.f90:566-577)

@@ -70,11 +70,26 @@ void Mesh::build(const int64_t* kmap, const int64_t* invk, long Nkall_, long Nk_
   rev.upload(hrev.data(), Nkall, s);
   irr_sp.upload(hirrsp.data(), Nk, s);
   kw.upload(kw_host.data(), Nk, s);
+  std::vector<int> hfull(Nkall), hnfull(Nkall);
+  nfull_ok = true;
+  for (long i = 0; i < Nkall; ++i) {
+    hfull[hsp[i]] = (int)i;
+    const long nf = invk[3 * i + 2] - 1;
+    if (nf < 0 || nf >= Nkall) nfull_ok = false;
+    hnfull[hsp[i]] = (nf < 0 || nf >= Nkall) ? 0 : (int)nf;
+  }
+  full_s.alloc(Nkall);
+  nfull_s.alloc(Nkall);
+  full_s.upload(hfull.data(), Nkall, s);
+  nfull_s.upload(hnfull.data(), Nkall, s);
   GMOD_HIP(hipStreamSynchronize(s));
 }
 
 MeshDev mesh_dev(const Mesh& m) {
-  return {m.irr_s.p, m.flag_s.p, m.kgat_s.p, m.rev.p, m.kf_s.p, m.Nkall, m.Nk, m.Nw, m.M};
+  MeshDev d{m.irr_s.p, m.flag_s.p, m.kgat_s.p, m.rev.p, m.kf_s.p, m.Nkall, m.Nk, m.Nw, m.M};
+  d.full_s = m.full_s.p;
+  d.nfull_s = m.nfull_s.p;
+  return d;
 }
 
 template <typename R>

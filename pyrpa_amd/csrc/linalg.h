@@ -63,6 +63,16 @@ void green_inv(zd* G, const zd* sigma, const zd* hamk, double mu, double temp, l
 void anomalous_f(zd* F, const zd* G, const zd* D, long Nk, int Nw, int No, hipStream_t s);
 // Hermitise: upper = conj(lower), diagonal real  (feliash.f90:618-634)
 void hermitize_lower(zd* D, long Nk, int Nw, int No, hipStream_t s);
+// spinful Eliashberg on the full zone, X(Nkall, Nw, No, No); sl = slist,
+// inv = 0-based spin-flipped orbital, sg = s_a s_b (device arrays):
+// the initial gap before normalisation (feliashwithsoc.f90:665-730)
+void initial_delta_soc(zd* D, const double* init, const zd* U, const int64_t* invk, const int* sl, const int* inv,
+                       long Nkall, long Nk, int Nw, int No, long gap_sym, hipStream_t s);
+// F = -G Delta G(-k)^dagger with its spin structure (feliashwithsoc.f90:426-518)
+void anomalous_f_soc(zd* F, const zd* G, const zd* D, const int64_t* invk, const double* sg, const int* sl,
+                     const int* inv, long Nkall, long Nk, int Nw, int No, long gap_sym, hipStream_t s);
+// spin structure of the new gap (feliashwithsoc.f90:614-662), NW = Nkall Nw
+void delta_sym_soc(zd* D, const int* sl, const int* inv, long NW, int No, long gap_sym, hipStream_t s);
 // sub_sigma = 1 / 2 static subtraction (fself.f90:384-417)
 void subtract_static(zd* sigma, long Nk, int Nw, int No, int mode, hipStream_t s);
 

@@ -1483,4 +1483,144 @@ bool host_dgesv(std::vector<double>& A, int n, std::vector<double>& b) {
   return true;
 }
 
+// ====================================================================== spinful Eliashberg
+// Arrays on the full zone: element (i, j, a, b) of X(Nkall, Nw, No, No) at
+// i + Nkall (j + Nw (a + No b)); G(Nk, Nw, No, No) on the irreducible set.
+// sg[a + No b] = s_a s_b, inv[a] the 0-based spin-flipped orbital.
+
+// initial gap (feliashwithsoc.f90:665-748) before its normalisation: one
+// thread per (i, a, b), the same value at every frequency
+__global__ void k_initial_delta_soc(zd* D, const double* init, const zd* U, const int64_t* invk, const int* sl,
+                                    const int* inv, long Nkall, long Nk, int Nw, int No, long gap_sym) {
+  const long tot = Nkall * No * No;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    const long i = e % Nkall;
+    const int a = (int)((e / Nkall) % No), b = (int)(e / (Nkall * No));
+    const long k = invk[3 * i] - 1;
+    const long f = invk[3 * i + 1];
+    zd v(0, 0);
+    if (f == 0 || f == 1) {
+      for (int n = 0; n < No; ++n) {
+        const double d = init[k + Nk * n];
+        if (f == 0)  // -U diag(init) (s_b conj U(-b, :))^T
+          v -= U[a + No * (n + (long)No * k)] * conj(U[inv[b] + No * (n + (long)No * k)]) * (d * sl[b]);
+        else  // (s_a conj U(-a, :)) diag(init) U^T
+          v += conj(U[inv[a] + No * (n + (long)No * k)]) * U[b + No * (n + (long)No * k)] * (d * sl[a]);
+      }
+    }
+    if (gap_sym > 0 && sl[a] * sl[b] >= 0) v = zd(0, 0);
+    for (int j = 0; j < Nw; ++j) D[i + Nkall * (j + (long)Nw * (a + (long)No * b))] = v;
+  }
+}
+
+// F = -G(k) Delta(k) G(-k)^dagger with time reversal (feliashwithsoc.f90:445-470):
+// one thread per (i, j, m, l)
+__global__ void k_anom_f_soc(zd* F, const zd* G, const zd* D, const int64_t* invk, const double* sg, const int* inv,
+                             long Nkall, long Nk, int Nw, int No) {
+  const long NW = Nkall * Nw, NkNw = Nk * (long)Nw;
+  const long tot = NW * No * No;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    const long ij = e % NW;
+    const long i = ij % Nkall, j = ij / Nkall;
+    const int m = (int)((e / NW) % No), l = (int)(e / (NW * No));
+    const long g0 = (invk[3 * i] - 1) + Nk * j;
+    const bool dir = invk[3 * i + 1] == 0;
+    auto Gat = [&](int p, int q) { return G[g0 + NkNw * (p + (long)No * q)]; };
+    zd acc(0, 0);
+    for (int b = 0; b < No; ++b) {
+      zd t(0, 0);  // (G1 Delta)(m, b)
+      for (int a = 0; a < No; ++a) {
+        const zd g1 = dir ? Gat(m, a) : Gat(inv[a], inv[m]) * sg[a + No * m];
+        t += g1 * D[ij + NW * (a + (long)No * b)];
+      }
+      const zd g2 = dir ? Gat(inv[l], inv[b]) * sg[b + No * l] : Gat(b, l);
+      acc += t * conj(g2);
+    }
+    F[e] = zd(-acc.re, -acc.im);
+  }
+}
+
+// spin structure of F (feliashwithsoc.f90:472-517), in the reference's order,
+// one thread per (i, j)
+__global__ void k_fk_sym_soc(zd* F, const int* sl, const int* inv, long NW, int No, long gap_sym) {
+  for (long ij = blockIdx.x * (long)blockDim.x + threadIdx.x; ij < NW; ij += (long)gridDim.x * blockDim.x) {
+    auto at = [&](int p, int q) -> zd& { return F[ij + NW * (p + (long)No * q)]; };
+    for (int l = 0; l < No; ++l) {
+      if (sl[l] != 1) continue;
+      for (int m = 0; m < No; ++m) {
+        if (sl[m] == 1) {
+          if (gap_sym < 0) {
+            const zd v = at(m, l);
+            at(inv[l], inv[m]) = zd(-v.re, v.im);
+          } else {
+            at(m, l) = zd(0, 0);
+            at(inv[l], inv[m]) = zd(0, 0);
+          }
+        } else if (inv[m] == l) {
+          at(inv[l], inv[m]) = zd(at(m, l).re, 0);
+        }
+      }
+    }
+    for (int l = 0; l < No; ++l) {
+      if (sl[l] != 1) continue;
+      for (int m = 0; m < No; ++m) {
+        if (sl[m] != -1 || gap_sym == -10) continue;
+        const zd v = at(m, l);
+        at(inv[m], inv[l]) = gap_sym < 0 ? v : zd(-v.re, -v.im);
+      }
+    }
+  }
+}
+
+// spin structure of the new gap (feliashwithsoc.f90:614-662), one thread per (i, j)
+__global__ void k_delta_sym_soc(zd* D, const int* sl, const int* inv, long NW, int No, long gap_sym) {
+  for (long ij = blockIdx.x * (long)blockDim.x + threadIdx.x; ij < NW; ij += (long)gridDim.x * blockDim.x) {
+    auto at = [&](int p, int q) -> zd& { return D[ij + NW * (p + (long)No * q)]; };
+    for (int l = 0; l < No; ++l) {
+      if (sl[l] != 1) continue;
+      for (int m = 0; m < No; ++m) {
+        if (sl[m] == 1) {
+          const zd v = at(m, l);
+          at(inv[l], inv[m]) = zd(-v.re, v.im);
+        } else if (inv[m] == l) {
+          at(l, m) = zd(at(l, m).re, 0);
+        } else if (l > inv[m]) {
+          at(inv[m], inv[l]) = conj(at(l, m));
+        }
+      }
+    }
+    for (int l = 0; l < No; ++l) {
+      if (sl[l] != 1) continue;
+      for (int m = 0; m < No; ++m) {
+        if (sl[m] != -1 || gap_sym == -10) continue;
+        const zd v = at(l, m);
+        at(inv[l], inv[m]) = gap_sym > 0 ? zd(-v.re, -v.im) : v;
+      }
+    }
+  }
+}
+
+void initial_delta_soc(zd* D, const double* init, const zd* U, const int64_t* invk, const int* sl, const int* inv,
+                       long Nkall, long Nk, int Nw, int No, long gap_sym, hipStream_t s) {
+  const long tot = Nkall * No * No;
+  hipLaunchKernelGGL(k_initial_delta_soc, dim3(nblk(tot, 256)), dim3(256), 0, s, D, init, U, invk, sl, inv, Nkall, Nk,
+                     Nw, No, gap_sym);
+  GMOD_CHECK_LAUNCH();
+}
+
+void anomalous_f_soc(zd* F, const zd* G, const zd* D, const int64_t* invk, const double* sg, const int* sl,
+                     const int* inv, long Nkall, long Nk, int Nw, int No, long gap_sym, hipStream_t s) {
+  const long NW = Nkall * Nw;
+  hipLaunchKernelGGL(k_anom_f_soc, dim3(nblk(NW * No * No, 256)), dim3(256), 0, s, F, G, D, invk, sg, inv, Nkall, Nk,
+                     Nw, No);
+  GMOD_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_fk_sym_soc, dim3(nblk(NW, 256)), dim3(256), 0, s, F, sl, inv, NW, No, gap_sym);
+  GMOD_CHECK_LAUNCH();
+}
+
+void delta_sym_soc(zd* D, const int* sl, const int* inv, long NW, int No, long gap_sym, hipStream_t s) {
+  hipLaunchKernelGGL(k_delta_sym_soc, dim3(nblk(NW, 256)), dim3(256), 0, s, D, sl, inv, NW, No, gap_sym);
+  GMOD_CHECK_LAUNCH();
+}
+
 }  // namespace gmod

@@ -68,6 +68,11 @@ _SIGS = {
     "get_chi0_soc": [_z, _d, _d, _i, _d, _z, _i, _i, _i, _i, _i, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi],
     "get_chis_chic_soc": [_z, _z, _z, _z, _d, _i, _i, _i, _i, _pi, _pi, _pi, _pi],
     "gmod_v_soc_flex": [_z, _d, _pi, _pi, _pi],
+    # libs/flibs/_eliashberg.py:29-52, 96-115
+    "lin_eliash_soc": [_z, _z, _z, _z, _d, _d, _d, _d, _d, _i, _i, _i, _i, _i, _i, _pd, _pd, _pi, _pi, _pi, _pi,
+                       _pi, _pi, _pi, _pi, _pi, _pi, _pi],
+    "mkfk_trs_soc_": [_z, _z, _z, _d, _i, _i, _i, _pi, _pi, _pi, _pi, _pi],
+    "conv_delta_orb_to_band_soc": [_z, _z, _z, _i, _i, _i, _pi, _pi, _pi, _pi],
     # libs/flibs/_hamiltonian.py:7-106
     "openmp_params": [_pi, _pb],
     "gen_ham": [_z, _d, _z, _d, _pi, _pi, _pi],
@@ -100,6 +105,8 @@ def _L():
         L.gmod_regrid_sigma.restype = None
         L.gmod_wannier_r.argtypes = [_z, _z, _i, _i] + [c_int64] * 7 + [c_int]
         L.gmod_wannier_r.restype = None
+        L.gmod_last_lambda.argtypes = []
+        L.gmod_last_lambda.restype = c_double
         L.gmod_profile.argtypes = [c_int]
         L.gmod_profile_read.argtypes = [_d, c_int]
         L.gmod_profile_read.restype = c_int
@@ -534,6 +541,50 @@ def get_V_soc_flex(chi, Vmat):
     Nchi, _, Nw, Nk = out.shape
     _call("gmod_v_soc_flex", out, np.asarray(Vmat, dtype=np.float64), i64(Nk), i64(Nw), i64(Nchi))
     return out
+
+
+def linearized_eliashberg_soc(chi, Gk, uni, init_delta, Vmat, sgnsig, sgnsig2, plist, slist, olist, kmap, invk,
+                              invs, invschi, Nx, Ny, Nz, temp, gap_sym, eps=1.0e-4, itemax=300, arnoldi_m=10):
+    """delta[No,No,Nw,Nkall] on the full zone  [_eliashberg.py:183-223]; chi is
+    overwritten with V_Delta; the eigenvalue (printed by the reference) is
+    last_lambda()"""
+    Norb, Nchi = len(slist), len(Vmat)
+    Nkall, Nk, Nw = len(kmap), Gk.shape[3], Gk.shape[2]
+    delta = np.zeros((Norb, Norb, Nw, Nkall), dtype=np.complex128)
+    _call("lin_eliash_soc", delta, chi, np.ascontiguousarray(Gk), np.ascontiguousarray(uni),
+          np.ascontiguousarray(init_delta, dtype=np.float64), np.asarray(Vmat, dtype=np.float64),
+          np.asarray(sgnsig, dtype=np.float64), np.asarray(sgnsig2, dtype=np.float64),
+          np.asarray(plist, dtype=np.float64), olist, np.asarray(slist, dtype=np.int64), kmap, invk,
+          np.asarray(invs, dtype=np.int64), np.asarray(invschi, dtype=np.int64), dbl(temp), dbl(eps), i64(Nkall),
+          i64(Nk), i64(Nw), i64(Nchi), i64(Norb), i64(Nx), i64(Ny), i64(Nz), i64(itemax), i64(gap_sym),
+          i64(arnoldi_m))
+    return delta
+
+
+def last_lambda():
+    """Eigenvalue of the last linearized_eliashberg_soc call (gmod_last_lambda)"""
+    return float(_L().gmod_last_lambda())
+
+
+def gen_Fk_soc(Gk, delta, invk, invs, slist, gap_sym):
+    """F[No,No,Nw,Nkall]  [_eliashberg.py:357-376]"""
+    Nkall, Nk, Nw, Norb = len(invk), Gk.shape[3], delta.shape[2], len(delta)
+    Fk = np.zeros((Norb, Norb, Nw, Nkall), dtype=np.complex128)
+    sl = np.asarray(slist, dtype=np.int64)
+    sgnsig = np.outer(sl, sl).astype(np.float64)
+    _call("mkfk_trs_soc_", Fk, np.ascontiguousarray(Gk), np.ascontiguousarray(delta), sgnsig, sl, invk,
+          np.asarray(invs, dtype=np.int64), i64(Nkall), i64(Nk), i64(Nw), i64(Norb), i64(gap_sym))
+    return Fk
+
+
+def conv_delta_orb_to_band_soc(delta, uni, invk, invs, slist):
+    """deltab[No,No,Nkall]  [_eliashberg.py:325-343]"""
+    Nkall, Nk, Nw, Norb = len(invk), len(uni), delta.shape[2], len(delta)
+    deltab = np.zeros((Norb, Norb, Nkall), dtype=np.complex128)
+    _call("conv_delta_orb_to_band_soc", deltab, np.ascontiguousarray(delta), np.ascontiguousarray(uni), invk,
+          np.asarray(invs, dtype=np.int64), np.asarray(slist, dtype=np.int64), i64(Norb), i64(Nkall), i64(Nk),
+          i64(Nw))
+    return deltab
 
 
 def mkself_soc(Vmat, kmap, invk, invs, olist, slist, hamk, eig, uni, mu, fill, temp, Nw, Nx, Ny, Nz, sw_out, sw_in,
