@@ -52,6 +52,11 @@ class ConvEngine {
   // stat_h[m + Nchi*l] (-Vmat(m,l) in a spinful basis, fselfwithsoc.f90:592)
   void sigma(zd* sig, const zd* V, const double* stat_h, double temp, hipStream_t s, bool apply_threshold = true);
 
+  // the spatial transforms of the tau reference kernels TA, TB of pair (a, b)
+  // at mu, or (diff) of their change from mu2 to mu (fchi.f90:703-812)
+  void tau_grids(cpx<R>* ta, cpx<R>* tb, int a, int b, const double* eig_d, const zd* uni_d, double mu, double mu2,
+                 bool diff, double temp, hipStream_t s);
+
   // tail constants J (per chi pair, [l + Nchi*m]) at mu and the sawtooth table (built once)
   void tail_constants(double temp, const double* eig_h, const zd* uni_h, const int64_t* invk, double mu,
                       std::vector<zd>& J, DBuf<zd>& sawc_d, hipStream_t s) const;
@@ -118,6 +123,12 @@ class ConvEngine {
   const zd* Vd_dev = nullptr;
   std::vector<double> S_cache, C_cache;
   cpx<R>* grid(DBuf<cpx<R>>& b, int a, int c) { return b.p + (long)(a + op.No * c) * N; }
+  // an inverse transform whose first pass forms a product of 1-3 terms (PL:
+  // ProductLoad or IncProductLoad), the term count fixed at compile time so
+  // the operand pointers stay in scalar registers (a run-time term index puts
+  // the functor in scratch memory)
+  template <class PL, class ST>
+  void product_pass(const PL& pl, const ST& st, hipStream_t s);
   void fft(cpx<R>* g, int sign, bool spatial_only, hipStream_t s);
 };
 

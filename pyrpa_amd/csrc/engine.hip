@@ -146,6 +146,47 @@ void ConvEngine<R>::transform_green(const zd* G, bool g0, hipStream_t s) {
     }
 }
 
+template <typename R>
+template <class PL, class ST>
+void ConvEngine<R>::product_pass(const PL& pl, const ST& st, hipStream_t s) {
+  const R invN = (R)(1.0 / (double)p4->transformed_points());
+  switch (pl.nterm) {
+    case 1: p4->exec_fused(pl.template fixed<1>(), st, work.p, tmp.p, +1, invN, s, true); break;
+    case 2: p4->exec_fused(pl.template fixed<2>(), st, work.p, tmp.p, +1, invN, s, true); break;
+    case 3: p4->exec_fused(pl.template fixed<3>(), st, work.p, tmp.p, +1, invN, s, true); break;
+    default: p4->exec_fused(pl, st, work.p, tmp.p, +1, invN, s, true); break;
+  }
+}
+
+// TA, TB written out by an elementwise kernel (the exponentials advance by a
+// factor per slot, at full occupancy) and then transformed; above 8 orbitals
+// the functions are evaluated inside the transform's first pass instead
+template <typename R>
+void ConvEngine<R>::tau_grids(cpx<R>* ta, cpx<R>* tb, int a, int b, const double* eig_d, const zd* uni_d, double mu,
+                              double mu2, bool diff, double temp, hipStream_t s) {
+  const int No = op.No;
+  const double beta = 1.0 / temp, dtau = beta / (double)mesh.M;
+  const MeshDev md = mesh_dev(mesh);
+  const unsigned S = (unsigned)mesh.Nkall;
+  if (No <= 8) {
+    tau_ref_grids<R>(ta, tb, eig_d, uni_d, a, b, No, mu, mu2, diff, beta, dtau, md, s);
+    p3->exec(ta, tmp.p, -1, (R)1, s);
+    p3->exec(tb, tmp.p, -1, (R)1, s);
+    return;
+  }
+  for (int which = 0; which < 2; ++which) {
+    cpx<R>* t = which == 0 ? ta : tb;
+    if (diff) {
+      const TauRefLoad<double> cur{eig_d, uni_d, a, b, No, which, mu, beta, dtau, md};
+      const TauRefLoad<double> prev{eig_d, uni_d, a, b, No, which, mu2, beta, dtau, md};
+      p3->exec_fused(TauRefDiffLoad<R>{cur, prev}, GridStore<R>{t, S}, t, tmp.p, -1, (R)1, s);
+    } else {
+      p3->exec_fused(TauRefLoad<R>{eig_d, uni_d, a, b, No, which, mu, beta, dtau, md}, GridStore<R>{t, S}, t, tmp.p, -1,
+                     (R)1, s);
+    }
+  }
+}
+
 // Tail constants of chi0 at chemical potential mu (fchi.f90:731-772):
 // J[l + Nchi*m] = delta_{a13 b13} rhobar_{a42 b42} - delta_{a42 b42} rhobar_{a13 b13}
 // with rhobar the BZ-averaged free density matrix, and the sawtooth correction
@@ -248,16 +289,8 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
     green0(G0.p, eig_d, uni_d, mu, temp, Nk, (int)Nw, No, s);
     transform_green(G0.p, true, s);
     if (TA.n != (size_t)(No * No) * N) { TA.alloc((size_t)No * No * N); TB.alloc((size_t)No * No * N); }
-    const MeshDev md = mesh_dev(mesh);
-    const double beta_ = 1.0 / temp, dtau_ = beta_ / (double)M;
     for (int b = 0; b < No; ++b)
-      for (int a = 0; a < No; ++a) {
-        for (int which = 0; which < 2; ++which) {
-          cpx<R>* g = grid(which == 0 ? TA : TB, a, b);
-          p3->exec_fused(TauRefLoad<R>{eig_d, uni_d, a, b, No, which, mu, beta_, dtau_, md}, GridStore<R>{g, (unsigned)mesh.Nkall}, g,
-                         tmp.p, -1, (R)1, s);
-        }
-      }
+      for (int a = 0; a < No; ++a) tau_grids(grid(TA, a, b), grid(TB, a, b), a, b, eig_d, uni_d, mu, mu, false, temp, s);
     std::vector<zd> Jall;
     tail_constants(temp, eig_h, uni_h, invk, mu, Jall, sawc_d, s);
     rb = Jall;
@@ -303,8 +336,7 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
     const zd J = tail ? rb[l + Nchi * m] : zd(0, 0);
     zd* dst = chi + NkNw * (m + (long)Nchi * l);
     const GatherStore<R> gst = gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, s);
-    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s, true);
-    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s, true);
+    product_pass(pl, gst, s);
     if (self_tail) {
       conj_copy(dst, dst, NkNw, s);
       ProductLoad<R> p0{};
@@ -328,14 +360,8 @@ void ConvEngine<R>::tau_reference(zd* chi, double temp, const double* eig_d, con
   const long NkNw = mesh.Nk * (long)mesh.Nw;
   if (TA.n != (size_t)(No * No) * N) { TA.alloc((size_t)No * No * N); TB.alloc((size_t)No * No * N); }
   const MeshDev md = mesh_dev(mesh);
-  const double beta = 1.0 / temp, dtau = beta / (double)mesh.M;
   for (int b = 0; b < No; ++b)
-    for (int a = 0; a < No; ++a)
-      for (int which = 0; which < 2; ++which) {
-        cpx<R>* g = grid(which == 0 ? TA : TB, a, b);
-        p3->exec_fused(TauRefLoad<R>{eig_d, uni_d, a, b, No, which, mu, beta, dtau, md},
-                       GridStore<R>{g, (unsigned)mesh.Nkall}, g, tmp.p, -1, (R)1, s);
-      }
+    for (int a = 0; a < No; ++a) tau_grids(grid(TA, a, b), grid(TB, a, b), a, b, eig_d, uni_d, mu, mu, false, temp, s);
   const double wref = 1.0 / (temp * (double)mesh.Nkall);
   const R invN = (R)(1.0 / (double)p4->transformed_points());
   for (auto& lm : op.irr) {
@@ -382,8 +408,7 @@ void ConvEngine<R>::chi0_split_tail(zd* chi, double temp, const double* eig_d, c
     if (!is_self(l, m)) pl.add(grid(F0g, a13, b13), grid(F0g, a42, b42), (R)w, 1);
     zd* dst = chi + NkNw * (m + (long)Nchi * l);
     const GatherStore<R> gst = gather_sf(dst, 1.0, zd(0, 0), nullptr, 0, s);
-    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s, true);
-    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s, true);
+    product_pass(pl, gst, s);
   }
   // 2) + the tau reference and the sawtooth term, FP64
   {
@@ -661,16 +686,11 @@ void ConvEngine<R>::set_reference(const zd* Gref, const zd* G0ref, bool tail, co
     }
   if (!tail) return;
   ensure(F0r); ensure(F0d); ensure(TAr); ensure(TBr); ensure(TAd); ensure(TBd);
-  const double beta = 1.0 / temp, dtau = beta / (double)mesh.M;
   for (int b = 0; b < No; ++b)
     for (int a = 0; a < No; ++a) {
       cpx<R>* g = grid(F0r, a, b);
       p4->exec_fused(GreenLoad<R>{G0ref, a, b, No, md}, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
-      for (int which = 0; which < 2; ++which) {
-        cpx<R>* t = grid(which == 0 ? TAr : TBr, a, b);
-        p3->exec_fused(TauRefLoad<R>{eig_d, uni_d, a, b, No, which, mu_ref, beta, dtau, md}, GridStore<R>{t, S}, t,
-                       tmp.p, -1, (R)1, s);
-      }
+      tau_grids(grid(TAr, a, b), grid(TBr, a, b), a, b, eig_d, uni_d, mu_ref, mu_ref, false, temp, s);
     }
 }
 
@@ -710,18 +730,12 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
     }
   DBuf<zd> slice;
   if (tail) {
-    const double beta = 1.0 / temp, dtau = beta / (double)mesh.M;
     for (int b = 0; b < No; ++b)
       for (int a = 0; a < No; ++a) {
         cpx<R>* g = grid(F0d, a, b);
         GreenDiffLoad<R> dl{GreenLoad<double>{G0, a, b, No, md}, GreenLoad<double>{G0prev, a, b, No, md}};
         p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
-        for (int which = 0; which < 2; ++which) {
-          cpx<R>* t = grid(which == 0 ? TAd : TBd, a, b);
-          TauRefDiffLoad<R> tl{TauRefLoad<double>{eig_d, uni_d, a, b, No, which, mu, beta, dtau, md},
-                               TauRefLoad<double>{eig_d, uni_d, a, b, No, which, mu_prev, beta, dtau, md}};
-          p3->exec_fused(tl, GridStore<R>{t, S}, t, tmp.p, -1, (R)1, s);
-        }
+        tau_grids(grid(TAd, a, b), grid(TBd, a, b), a, b, eig_d, uni_d, mu, mu_prev, true, temp, s);
       }
   }
   for (auto& lm : op.irr) {
@@ -751,7 +765,7 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
     if (self_tail) {
       // increment of conj(conv[G] + ref) - conv[G0], formed apart and then added
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
-      p4->exec_fused(pl, gather_sf(slice.p, 1.0, J, sawc_d, 0, s), work.p, tmp.p, +1, invN, s, true);
+      product_pass(pl, gather_sf(slice.p, 1.0, J, sawc_d, 0, s), s);
       conj_copy(slice.p, slice.p, NkNw, s);
       IncProductLoad<R> p0{};
       p0.m = md;
@@ -766,18 +780,12 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
       // accumulation (fchi.f90:610): the stored value is conj(conv[G]), so the
       // increment is conjugated before it is added
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
-      if (pl.nterm == 1)
-        p4->exec_fused(pl.template fixed<1>(), gather_sf(slice.p, w, zd(0, 0), nullptr, 0, s), work.p, tmp.p, +1, invN,
-                       s, true);
-      else
-        p4->exec_fused(pl, gather_sf(slice.p, w, zd(0, 0), nullptr, 0, s), work.p, tmp.p, +1, invN, s, true);
+      product_pass(pl, gather_sf(slice.p, w, zd(0, 0), nullptr, 0, s), s);
       conj_copy(slice.p, slice.p, NkNw, s);
       axpy(dst, slice.p, zd(1, 0), NkNw, s);
       continue;
     }
-    const GatherStore<R> gst = gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, s);
-    if (pl.nterm == 1) p4->exec_fused(pl.template fixed<1>(), gst, work.p, tmp.p, +1, invN, s, true);
-    else p4->exec_fused(pl, gst, work.p, tmp.p, +1, invN, s, true);
+    product_pass(pl, gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, s), s);
     if (pr[0] >= 0) conj_copy(chi_acc + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
 }

@@ -551,6 +551,16 @@ struct GatherStore {
 template <typename R>
 void mul_acc(cpx<R>* acc, const cpx<R>* A, const cpx<R>* B, bool first, long n, hipStream_t s);
 
+// The analytic tau reference kernels of one orbital pair (a, b) written out on
+// the full grid (the functions of TauRefLoad, fchi.f90:703-729, 791-812):
+// ta(sp, s) = sum_n c_n g0_n(tau_s), tb(sp, s) = sum_n c_n g1_n(tau_s), with
+// c_n = U_an conj(U_bn) (conjugated at -k).  diff: the differences between
+// chemical potentials mu and mu2.  The exponentials advance by one factor per
+// slot (No <= 8; FP64 arithmetic, stored at the grid precision).
+template <typename R>
+void tau_ref_grids(cpx<R>* ta, cpx<R>* tb, const double* eig, const zd* uni, int a, int b, int No, double mu,
+                   double mu2, bool diff, double beta, double dtau, const MeshDev& m, hipStream_t s);
+
 // ---- elementwise
 void conj_copy(zd* dst, const zd* src, long n, hipStream_t s);
 void conj_copy(zd* dst, const zd* src, long n, double sign, hipStream_t s);  // dst = sign conj(src), sign = +-1

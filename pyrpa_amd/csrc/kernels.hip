@@ -107,6 +107,92 @@ void mul_acc(cpx<R>* acc, const cpx<R>* A, const cpx<R>* B, bool first, long n, 
   GMOD_CHECK_LAUNCH();
 }
 
+// One thread per spatial point and chunk of L slots.  For band n with
+// a = |e_n - mu|: E1(s) = exp(-a tau_s) falls by r = exp(-a dtau) per slot,
+// E2(s) = exp(-a (beta - tau_s)) rises by 1/r; g0 = -(xi >= 0 ? E1 : E2) / d,
+// g1 = (xi >= 0 ? E2 : E1) / d, d = 1 + exp(-beta a).  E2 is evaluated
+// directly where its chunk start would underflow (deep bands, low T).
+template <typename R, int NO>
+__global__ void __launch_bounds__(256)
+k_tau_ref(cpx<R>* ta, cpx<R>* tb, const double* __restrict__ eig, const zd* __restrict__ U, int a, int b, double mu,
+          double mu2, int nv, double beta, double dtau, MeshDev m, int L) {
+  const long nch = (m.M + L - 1) / L;
+  const long tot = m.Nkall * nch;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    const long sp = e % m.Nkall;
+    const int s0 = (int)(e / m.Nkall) * L;
+    const int s1 = s0 + L < m.M ? s0 + L : m.M;
+    const long k = m.irr_s[sp];
+    const bool rev = m.flag_s[sp] != 0;
+    zd c[NO];
+    double E1[NO][2], E2[NO][2], r[NO][2], ir[NO][2], id[NO][2], ad[NO][2];
+    bool pos[NO][2], direct[NO][2];
+#pragma unroll
+    for (int n = 0; n < NO; ++n) {
+      zd cn = U[a + NO * (n + (long)NO * k)] * conj(U[b + NO * (n + (long)NO * k)]);
+      c[n] = rev ? conj(cn) : cn;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const double xi = eig[n + (long)NO * k] - (v == 0 ? mu : mu2);
+        const double aa = fabs(xi);
+        pos[n][v] = xi >= 0.0;
+        id[n][v] = 1.0 / (1.0 + exp(-beta * aa));
+        ad[n][v] = aa * dtau;
+        direct[n][v] = ad[n][v] * (double)(m.M - s0) > 600.0 || ad[n][v] > 300.0;
+        E1[n][v] = exp(-ad[n][v] * (double)s0);
+        E2[n][v] = direct[n][v] ? 0.0 : exp(-ad[n][v] * (double)(m.M - s0));
+        r[n][v] = exp(-ad[n][v]);
+        ir[n][v] = direct[n][v] ? 0.0 : exp(ad[n][v]);
+      }
+    }
+    for (int s = s0; s < s1; ++s) {
+      zd A(0, 0), B(0, 0);
+#pragma unroll
+      for (int n = 0; n < NO; ++n) {
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          if (v >= nv) continue;
+          const double e2 = direct[n][v] ? exp(-ad[n][v] * (double)(m.M - s)) : E2[n][v];
+          const double g0 = -(pos[n][v] ? E1[n][v] : e2) * id[n][v];
+          const double g1 = (pos[n][v] ? e2 : E1[n][v]) * id[n][v];
+          const double sg = v == 0 ? 1.0 : -1.0;
+          A += c[n] * (sg * g0);
+          B += c[n] * (sg * g1);
+          E1[n][v] *= r[n][v];
+          E2[n][v] *= ir[n][v];
+        }
+      }
+      ta[sp + m.Nkall * (long)s] = cpx<R>(A);
+      tb[sp + m.Nkall * (long)s] = cpx<R>(B);
+    }
+  }
+}
+
+template <typename R>
+void tau_ref_grids(cpx<R>* ta, cpx<R>* tb, const double* eig, const zd* uni, int a, int b, int No, double mu,
+                   double mu2, bool diff, double beta, double dtau, const MeshDev& m, hipStream_t s) {
+  const int L = 64;
+  const long tot = m.Nkall * (long)((m.M + L - 1) / L);
+  const int nv = diff ? 2 : 1;
+  const dim3 g((unsigned)std::min<long>((tot + 255) / 256, 1L << 20)), blk(256);
+  switch (No) {
+#define GMOD_TAU_CASE(NOV)                                                                                           \
+  case NOV:                                                                                                          \
+    hipLaunchKernelGGL((k_tau_ref<R, NOV>), g, blk, 0, s, ta, tb, eig, uni, a, b, mu, mu2, nv, beta, dtau, m, L); \
+    break;
+    GMOD_TAU_CASE(1) GMOD_TAU_CASE(2) GMOD_TAU_CASE(3) GMOD_TAU_CASE(4)
+    GMOD_TAU_CASE(5) GMOD_TAU_CASE(6) GMOD_TAU_CASE(7) GMOD_TAU_CASE(8)
+#undef GMOD_TAU_CASE
+    default: throw Error("tau reference: Norb > 8 is not supported");
+  }
+  GMOD_CHECK_LAUNCH();
+}
+
+template void tau_ref_grids<float>(cpx<float>*, cpx<float>*, const double*, const zd*, int, int, int, double, double,
+                                   bool, double, double, const MeshDev&, hipStream_t);
+template void tau_ref_grids<double>(cpx<double>*, cpx<double>*, const double*, const zd*, int, int, int, double,
+                                    double, bool, double, double, const MeshDev&, hipStream_t);
+
 // ------------------------------------------------------------------ elementwise
 __global__ void k_conj_copy(zd* dst, const zd* src, long n) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
