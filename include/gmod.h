@@ -265,8 +265,8 @@ void gmod_set_sigma_seed(const gmod_z* sigma, double mu, double mu_old, int64_t 
 /* Statistics of the last mkself call: [iterations, esterr, stoner, mu,
  * seconds, seconds in chi0, seconds in vertex, seconds in sigma, seconds in
  * mu search + Dyson, FP32-increment iterations, device-resident solve seconds
- * (inputs in HBM to converged Sigma and mu, no PCIe copies)]; returns the
- * number of values written. */
+ * (inputs in HBM to converged Sigma and mu, no PCIe copies), moves of the
+ * increments' reference iterate]; returns the number of values written. */
 int gmod_flex_stats(double* out, int n);
 /* Kernel profiling with hipEvents on the library stream.  gmod_profile(1)
  * resets and enables; gmod_profile_read returns, per kernel class

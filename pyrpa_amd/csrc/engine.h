@@ -61,30 +61,32 @@ class ConvEngine {
   void tail_constants(double temp, const double* eig_h, const zd* uni_h, const int64_t* invk, double mu,
                       std::vector<zd>& J, DBuf<zd>& sawc_d, hipStream_t s) const;
 
-  // ---- rolling increments (mixed-precision FLEX): the transforms of the
-  // previous iterate are kept at grid precision and advanced by increments.
-  // Transforms of G (and, with the tail, G0 and the tau kernels at mu).
+  // ---- increments from a fixed reference (mixed-precision FLEX): the
+  // transforms of a reference iterate are held at grid precision and every
+  // later iterate is reached by the transforms of its difference from it.
+  // Transforms of Gref (and, with the tail, G0ref and the tau kernels at mu_ref).
   void set_reference(const zd* Gref, const zd* G0ref, bool tail, const double* eig_d, const zd* uni_d, double mu_ref,
                      double temp, hipStream_t s);
-  // Transforms of the vertex for all Nchi^2 pairs (full grid incl. the nu->inf slot).
+  // Transforms of the reference vertex for all Nchi^2 pairs (full grid incl. the nu->inf slot).
   void set_vertex_reference(const zd* Vref, const double* S_h, const double* C_h, hipStream_t s);
-  // chi_acc += chi0[G] - chi0[Gprev] (tail: + the G0 and tau-reference terms at mu vs
-  // mu_prev and dJ sawc), with the reference's partner / self-partner conjugation.
-  // Leaves the transformed increments in Fd (F0d, TAd, TBd).
-  void chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, const zd* G0, const zd* G0prev, double temp,
-                      bool tail, const double* eig_d, const zd* uni_d, double mu, double mu_prev,
+  // chi <- chi_base + chi0[G] - chi0[Gref] (tail: + the G0 and tau-reference
+  // terms at mu vs mu_ref and dJ sawc), with the reference's partner /
+  // self-partner conjugation; chi_base holds chi0[Gref].  Every slot the
+  // gather keeps is written; the others keep their contents.  Leaves the
+  // transformed differences in Fd (F0d, TAd, TBd).
+  void chi0_increment(zd* chi, const zd* chi_base, const zd* G, const zd* Gref, const zd* G0, const zd* G0ref,
+                      double temp, bool tail, const double* eig_d, const zd* uni_d, double mu, double mu_ref,
                       const std::vector<zd>& dJ, const zd* sawc_d, hipStream_t s);
-  // sig_acc += Sigma[V, G] - Sigma[Vprev, Gprev] (unthresholded); uses Fr/Fd of the
-  // preceding chi0_increment and advances FVr to V.
-  // dv_ready: the FP32 vertex increment is already in dv_buffer() (written by
-  // the fused vertex kernel); else it is formed here from V and Vprev
-  void sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, double temp, hipStream_t s, bool dv_ready = false);
+  // sig <- sig_base + Sigma[V, G] - Sigma[Vref, Gref] (unthresholded); uses
+  // Fr/Fd of the preceding chi0_increment and FVr.
+  // dv_ready: the FP32 vertex difference V - Vref is already in dv_buffer()
+  // (written by the fused vertex kernel); else it is formed here from V and Vref
+  void sigma_increment(zd* sig, const zd* sig_base, const zd* V, const zd* Vref, double temp, hipStream_t s,
+                       bool dv_ready = false);
   cpx<float>* dv_buffer(long n) {
     if (dvbuf.n != (size_t)n) dvbuf.alloc(n);
     return dvbuf.p;
   }
-  // Fr += Fd (and the tail grids): the current iterate becomes the reference
-  void advance(bool tail, hipStream_t s);
   // Eliashberg: cache FFT[V_Delta] of the qualifying pairs (feliash.f90:540)
   void prepare_gap(const zd* Vd, const double* S_h, const double* C_h, bool singlet, hipStream_t s);
   // newdelta (un-weighted, Hermitised) <- sum V_Delta (*) F  (feliash.f90:486-635)
@@ -102,14 +104,14 @@ class ConvEngine {
   const FFTPlan<R>* p4;
   const FFTPlan<R>* p3;
   DBuf<cpx<R>> Fg, F0g, TA, TB, work, tmp, acc, Vft, Ff;
-  DBuf<cpx<R>> Fr, Fd, F0r, F0d, TAr, TBr, TAd, TBd, FVr;  // increment mode
+  DBuf<cpx<R>> Fr, Fd, F0r, F0d, TAr, TBr, TAd, TBd, FVr;  // increment mode (r: reference, d: difference)
   DBuf<cpx<R>> Mg;  // mirrored Fg (G(-r,-t)): the second factor of the bubble, read in place
   // Sigma through the accumulating last pass: the sources of one output
   // after their first passes, and the device table of their operand grids
   DBuf<cpx<R>> srcs;
   DBuf<cpx<R>*> accum_tab;
   std::vector<cpx<R>*> accum_tab_h;
-  DBuf<cpx<float>> dvbuf;                                  // vertex increment, k space
+  DBuf<cpx<float>> dvbuf;                                  // vertex difference, k space
   // boxes of p4's forward last pass (it spans the t axis) that hold a kept
   // spatial point: a spatial-first gather skips the others
   DBuf<unsigned char> tlive;

@@ -10,6 +10,7 @@
 //  * The tail correction conv[G] - conv[G0] + ref[G0] (fchi.f90:852-892) is
 //    combined in (r,tau) space before a single inverse FFT per pair.
 #include <cmath>
+#include <cstdlib>
 
 #include <array>
 
@@ -18,6 +19,11 @@
 namespace gmod {
 
 constexpr double PI_E = 3.141592653589793238462643383279;
+
+static bool sigma_unfused() {
+  const char* e = getenv("GMOD_SIGMA_UNFUSED");
+  return e && *e && *e != '0';
+}
 
 void OrbPairs::from_olist(const int64_t* olist, int Nchi_, int No_) {
   Nchi = Nchi_;
@@ -457,7 +463,8 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* stat_h, double tem
   const R invN = (R)(1.0 / (double)p4->transformed_points());
   const long NC = NkNw;
   // the accumulating last pass sums the pairs of one output in registers
-  const bool fused = p4->accum_ok(-1);
+  // (GMOD_SIGMA_UNFUSED=1: one pass per pair, accumulating through memory)
+  const bool fused = p4->accum_ok(-1) && !sigma_unfused();
   std::vector<std::array<int, 2>> pairs;
   if (fused) {
     accum_tab_h.clear();
@@ -712,9 +719,10 @@ void ConvEngine<R>::set_vertex_reference(const zd* Vref, const double* S_h, cons
 }
 
 template <typename R>
-void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, const zd* G0, const zd* G0prev,
-                                   double temp, bool tail, const double* eig_d, const zd* uni_d, double mu,
-                                   double mu_prev, const std::vector<zd>& dJ, const zd* sawc_d, hipStream_t s) {
+void ConvEngine<R>::chi0_increment(zd* chi, const zd* chi_base, const zd* G, const zd* Gref, const zd* G0,
+                                   const zd* G0ref, double temp, bool tail, const double* eig_d, const zd* uni_d,
+                                   double mu, double mu_ref, const std::vector<zd>& dJ, const zd* sawc_d,
+                                   hipStream_t s) {
   const int No = op.No, Nchi = op.Nchi;
   const long NkNw = mesh.Nk * (long)mesh.Nw;
   const double w = temp / (double)mesh.Nkall;
@@ -725,7 +733,7 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
   for (int b = 0; b < No; ++b)
     for (int a = 0; a < No; ++a) {
       cpx<R>* g = grid(Fd, a, b);
-      GreenDiffLoad<R> dl{GreenLoad<double>{G, a, b, No, md}, GreenLoad<double>{Gprev, a, b, No, md}};
+      GreenDiffLoad<R> dl{GreenLoad<double>{G, a, b, No, md}, GreenLoad<double>{Gref, a, b, No, md}};
       p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
     }
   DBuf<zd> slice;
@@ -733,9 +741,9 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
     for (int b = 0; b < No; ++b)
       for (int a = 0; a < No; ++a) {
         cpx<R>* g = grid(F0d, a, b);
-        GreenDiffLoad<R> dl{GreenLoad<double>{G0, a, b, No, md}, GreenLoad<double>{G0prev, a, b, No, md}};
+        GreenDiffLoad<R> dl{GreenLoad<double>{G0, a, b, No, md}, GreenLoad<double>{G0ref, a, b, No, md}};
         p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
-        tau_grids(grid(TAd, a, b), grid(TBd, a, b), a, b, eig_d, uni_d, mu, mu_prev, true, temp, s);
+        tau_grids(grid(TAd, a, b), grid(TBd, a, b), a, b, eig_d, uni_d, mu, mu_ref, true, temp, s);
       }
   }
   for (auto& lm : op.irr) {
@@ -761,7 +769,8 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
       pl.add(grid(TAr, a13, b13), grid(TAd, a13, b13), grid(TBr, a42, b42), grid(TBd, a42, b42), (R)(-wref), 0);
     }
     const zd J = tail ? dJ[l + Nchi * m] : zd(0, 0);
-    zd* dst = chi_acc + NkNw * (m + (long)Nchi * l);
+    zd* dst = chi + NkNw * (m + (long)Nchi * l);
+    const zd* base = chi_base + NkNw * (m + (long)Nchi * l);
     if (self_tail) {
       // increment of conj(conv[G] + ref) - conv[G0], formed apart and then added
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
@@ -772,7 +781,7 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
       p0.add(grid(F0r, a13, b13), grid(F0d, a13, b13), grid(F0r, a42, b42), grid(F0d, a42, b42), (R)w, 1);
       p4->exec_fused(p0.template fixed<1>(), gather_sf(slice.p, 1.0, zd(0, 0), nullptr, 1, s), work.p, tmp.p, +1, invN,
                      s, true);
-      axpy(dst, slice.p, zd(1, 0), NkNw, s);
+      lincomb(dst, base, 1.0, slice.p, 1.0, NkNw, s);
       continue;
     }
     if (self) {
@@ -782,17 +791,19 @@ void ConvEngine<R>::chi0_increment(zd* chi_acc, const zd* G, const zd* Gprev, co
       if (!mesh.complete) GMOD_HIP(hipMemsetAsync(slice.p, 0, NkNw * sizeof(zd), s));
       product_pass(pl, gather_sf(slice.p, w, zd(0, 0), nullptr, 0, s), s);
       conj_copy(slice.p, slice.p, NkNw, s);
-      axpy(dst, slice.p, zd(1, 0), NkNw, s);
+      lincomb(dst, base, 1.0, slice.p, 1.0, NkNw, s);
       continue;
     }
-    product_pass(pl, gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 1, s), s);
-    if (pr[0] >= 0) conj_copy(chi_acc + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
+    GatherStore<R> gst = gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 0, s);
+    gst.base = base;
+    product_pass(pl, gst, s);
+    if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
 }
 
 template <typename R>
-void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, double temp, hipStream_t s,
-                                    bool dv_ready) {
+void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, const zd* Vref, double temp,
+                                    hipStream_t s, bool dv_ready) {
   const int No = op.No, Nchi = op.Nchi;
   const long NkNw = mesh.Nk * (long)mesh.Nw;
   if (acc.n != (size_t)N) acc.alloc(N);
@@ -800,15 +811,15 @@ void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, d
   const unsigned S = (unsigned)mesh.Nkall;
   const R invN = (R)(1.0 / (double)p4->transformed_points());
   const long NC = NkNw;
-  // the vertex increment once in k space at grid precision: each element is
+  // the vertex difference once in k space at grid precision: each element is
   // then read (from a quarter of the bytes) by the four grid points that
   // unfold it, instead of two FP64 reads per grid point
   const long nC = NkNw * Nchi * Nchi;
   if (!dv_ready || dvbuf.n != (size_t)nC) {
     if (dvbuf.n != (size_t)nC) dvbuf.alloc(nC);
-    diff_to_f32(dvbuf.p, V, Vprev, nC, s);
+    diff_to_f32(dvbuf.p, V, Vref, nC, s);
   }
-  const bool fused = p4->accum_ok(-1);
+  const bool fused = p4->accum_ok(-1) && !sigma_unfused();
   if (fused) {
     accum_tab_h.clear();
     int nmax = 0;
@@ -857,20 +868,11 @@ void ConvEngine<R>::sigma_increment(zd* sig_acc, const zd* V, const zd* Vprev, d
         tab_off += nsrc;
       }
       if (first) continue;
-      zd* dst = sig_acc + NkNw * (m1 + (long)No * l1);
-      p4->exec_fused(GridLoad<R>{acc.p, S}, GatherStore<R>{dst, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 1, md},
-                     acc.p, tmp.p, +1, invN, s);
+      const long o = NkNw * (m1 + (long)No * l1);
+      GatherStore<R> gst{sig + o, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md};
+      gst.base = sig_base + o;
+      p4->exec_fused(GridLoad<R>{acc.p, S}, gst, acc.p, tmp.p, +1, invN, s);
     }
-}
-
-template <typename R>
-void ConvEngine<R>::advance(bool tail, hipStream_t s) {
-  const long nb = (long)op.No * op.No * N;
-  add_inplace(Fr.p, Fd.p, nb, s);
-  if (!tail) return;
-  add_inplace(F0r.p, F0d.p, nb, s);
-  add_inplace(TAr.p, TAd.p, nb, s);
-  add_inplace(TBr.p, TBd.p, nb, s);
 }
 
 template class ConvEngine<double>;

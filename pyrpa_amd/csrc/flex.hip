@@ -147,10 +147,13 @@ struct FlexIn {
 
 // Precision modes: R = double, mixed = false: FP64 throughout (the reference's
 // arithmetic).  R = float, mixed = false: FP32 transforms of the full fields.
-// R = double, mixed = true: the first `head` iterations in FP64, then FP32
-// transforms of rolling increments (ConvEngine::chi0_increment /
-// sigma_increment) around FP64 accumulators of chi0 and Sigma; vertex, Dyson,
-// mu search, mixing and the convergence test stay FP64.
+// R = double, mixed = true: the first `head` iterations in FP64, then chi0 and
+// Sigma as FP64 values at a reference iterate plus FP32 transforms of the
+// differences from it (ConvEngine::chi0_increment / sigma_increment); vertex,
+// Dyson, mu search, mixing and the convergence test stay FP64.  The FP32
+// rounding of a difference scales with its size, so the reference moves to
+// the current iterate once the convergence measure comes within
+// GMOD_REBASE (default 1e-5) times the distance of Sigma from the reference.
 template <typename R>
 static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mixed) {
   auto t_start = clk::now();
@@ -192,20 +195,27 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
     for (long i = 0; i < A.m_diis; ++i) { xout[i].alloc(nS); res[i].alloc(nS); }
   std::unique_ptr<ConvEngine<R>> eng(new ConvEngine<R>(mesh, op));
   std::unique_ptr<ConvEngine<float>> e32;
-  // mixed-precision state: FP64 accumulators and the previous iterate
-  DBuf<zd> chi_acc, Vprev, sig_acc, Gprev, G0, G0prev, sawc_d;
-  std::vector<zd> Jprev, Jcur, dJ;
-  double mu_prev = 0.0;
+  // mixed-precision state: chi0 and Sigma of the current iterate (chi_acc,
+  // sig_acc) and the reference iterate (chi_ref, sig_ref, Vref, Gref, G0ref,
+  // mu_ref, Jref).  In the increment iterations the vertex overwrites
+  // chi_acc in place, and chi_ref takes over the buffer `chi` held.
+  DBuf<zd> chi_acc, chi_ref, Vref, sig_acc, sig_ref, Gref, G0, G0ref, sawc_d;
+  std::vector<zd> Jref, Jcur, dJ;
+  double mu_ref = 0.0;
   long head = 1;
   if (const char* e = getenv("GMOD_MIXED_HEAD")) head = std::max(1L, atol(e));
+  double rebase_c = 1.0e-5;
+  if (const char* e = getenv("GMOD_REBASE")) rebase_c = atof(e);
+  double dist_ref = 0.0;  // max relative distance of Sigma from sig_ref
   if (mixed) {
     chi_acc.alloc(nC);
-    Vprev.alloc(nC);
+    Vref.alloc(nC);
     sig_acc.alloc(nS);
-    Gprev.alloc(nS);
-    if (A.sw_tail) { G0.alloc(nS); G0prev.alloc(nS); }
+    sig_ref.alloc(nS);
+    Gref.alloc(nS);
+    if (A.sw_tail) { G0.alloc(nS); G0ref.alloc(nS); }
   }
-  int n_inc = 0;
+  int n_inc = 0, n_rebase = 0;
 
   double mu = A.mu_init, mu_old = 0.0;
   // device-resident solve time: from the inputs resident in HBM to the
@@ -347,6 +357,23 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
     if (verbose) printf("mu  =%8.4f filling =%8.4f\n", mu, fill);
   };
 
+  // the reference of the increment iterations: the current iterate, whose
+  // chi0 (chi0_n), vertex (V_n) and Sigma (sig_acc) are FP64 values
+  auto take_reference = [&](const zd* chi0_n, const zd* V_n) {
+    copy(Gref.p, G.p, nS, s);
+    copy(Vref.p, V_n, nC, s);
+    copy(sig_ref.p, sig_acc.p, nS, s);
+    if (chi0_n != chi_ref.p) copy(chi_ref.p, chi0_n, nC, s);
+    mu_ref = mu;
+    if (A.sw_tail) {
+      green0(G0ref.p, eig_d.p, uni_d.p, mu, T, Nk, (int)Nw, No, s);
+      Jref = Jcur;
+    }
+    e32->set_reference(Gref.p, G0ref.p, A.sw_tail, eig_d.p, uni_d.p, mu_ref, T, s);
+    e32->set_vertex_reference(Vref.p, A.Smat, A.Cmat, s);
+    dist_ref = 0.0;
+  };
+
   double stoner = 0.0;
   double t_chi = 0, t_vert = 0, t_sig = 0, t_mu = 0;
   int iters = 0;
@@ -360,6 +387,9 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
     // (fself.f90:375) is applied where chi0 is read: by the Stoner slice and
     // by the vertex kernel
     const zd* chi_src = mixed ? chi_acc.p : chi.p;
+    // the increment iterations move the reference to this iterate when the
+    // last convergence measure came near the rounding of the differences
+    const bool rebase = inc && rebase_c > 0.0 && it > 1 && esterr < rebase_c * dist_ref;
     if (!inc) {
       eng->transform_green(G.p, false, s);
       eng->chi0(mixed ? chi_acc.p : chi.p, T, A.sw_tail, eig_d.p, uni_d.p, A.eig, A.uni, A.invk, mu, s, false);
@@ -368,24 +398,30 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
         green0(G0.p, eig_d.p, uni_d.p, mu, T, Nk, (int)Nw, No, s);
         e32->tail_constants(T, A.eig, A.uni, A.invk, mu, Jcur, sawc_d, s);
         dJ.resize(Jcur.size());
-        for (size_t i = 0; i < Jcur.size(); ++i) dJ[i] = Jcur[i] - Jprev[i];
+        for (size_t i = 0; i < Jcur.size(); ++i) dJ[i] = Jcur[i] - Jref[i];
       }
-      e32->chi0_increment(chi_acc.p, G.p, Gprev.p, G0.p, G0prev.p, T, A.sw_tail, eig_d.p, uni_d.p, mu, mu_prev, dJ,
-                          sawc_d.p, s);
+      // slots the gather does not keep hold the last vertex: restore them
+      if (!mesh.complete) copy(chi_acc.p, chi_ref.p, nC, s);
+      e32->chi0_increment(chi_acc.p, chi_ref.p, G.p, Gref.p, G0.p, G0ref.p, T, A.sw_tail, eig_d.p, uni_d.p, mu,
+                          mu_ref, dJ, sawc_d.p, s);
+      if (rebase) copy(chi_ref.p, chi_acc.p, nC, s);  // chi0 of the new reference, before the vertex overwrites it
     }
     stoner = stoner_factor(chi_src, Nk, Nw, Nchi, S_d.p, s, 1e-9, A.soc ? -1.0 : 1.0);
     t_chi += secs(t0);
     if (verbose) printf("SDW max eig(chi0 S) = %12.8f\n", stoner);
     const bool rescale = A.sw_rescale && stoner >= 1.0;
+    // the vertex: into `chi`, or in place over chi_acc in the increment
+    // iterations (chi0 is recomputed from chi_ref in the next one)
+    zd* Vout = inc ? chi_acc.p : chi.p;
     if (rescale) {
       const double rescale_t = 1.0 - 1.0e-4;  // fself.f90:371-380
       if (verbose) printf("[FLEX] Stoner factor=%10.6f >= 1: rescaling chi0 to%7.4f\n", stoner, rescale_t);
-      threshold_copy(chi.p, chi_src, nC, 1e-9, s);
-      scale(chi.p, nC, rescale_t / stoner, s);
+      threshold_copy(Vout, chi_src, nC, 1e-9, s);
+      scale(Vout, nC, rescale_t / stoner, s);
     }
     t0 = clk::now();
-    // the FP32 increment of the vertex comes out of the vertex kernel in the
-    // increment iterations (unless chi0 was rescaled)
+    // the FP32 vertex difference V - Vref comes out of the vertex kernel in
+    // the increment iterations (unless chi0 was rescaled)
     const bool dv_fused = inc && !rescale;
     {
       ProfScope ps(PROF_VERTEX, (double)nC * 2 * sizeof(zd), s);
@@ -393,47 +429,41 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
         soc_vertex(rescale ? chi.p : chi_src, chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, rescale ? 0.0 : 1e-9,
                    err.p, s);
       else if (rescale)
-        rpa_vertex(chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 0, err.p, s);
+        rpa_vertex(Vout, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 0, err.p, s);
       else
-        rpa_vertex_fused(chi_src, chi.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 1e-9,
-                         dv_fused ? Vprev.p : nullptr, dv_fused ? e32->dv_buffer(nC) : nullptr, err.p, s);
+        rpa_vertex_fused(chi_src, Vout, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 1e-9,
+                         dv_fused ? Vref.p : nullptr, dv_fused ? e32->dv_buffer(nC) : nullptr, err.p, s);
     }
     check_err(err, s, A.soc ? "get_V_soc_flex" : "get_vsigma_flex_nosoc");
     t_vert += secs(t0);
     if (verbose)
-      printf("Re V_sigma: max:%12.4E min:%12.4E\n", max_real(chi.p, nC, s), min_real(chi.p, nC, s));
+      printf("Re V_sigma: max:%12.4E min:%12.4E\n", max_real(Vout, nC, s), min_real(Vout, nC, s));
     t0 = clk::now();
     if (!inc) {
       if (A.soc) eng->sigma(sig.p, chi.p, soc_stat.data(), T, s, true);
       else eng->sigma(mixed ? sig_acc.p : sig.p, chi.p, A.Smat, A.Cmat, T, s, !mixed);
       if (mixed) {
         threshold_copy(sig.p, sig_acc.p, nS, 1e-9, s);
-        copy(Vprev.p, chi.p, nC, s);
-        copy(Gprev.p, G.p, nS, s);
-        mu_prev = mu;
-        if (A.sw_tail) {
-          green0(G0prev.p, eig_d.p, uni_d.p, mu, T, Nk, (int)Nw, No, s);
-          eng->tail_constants(T, A.eig, A.uni, A.invk, mu, Jprev, sawc_d, s);
-        }
         if (it >= head) {
           if (verbose) printf("[FLEX] switching to FP32 increments\n");
+          if (A.sw_tail) eng->tail_constants(T, A.eig, A.uni, A.invk, mu, Jcur, sawc_d, s);
           eng.reset();  // FP64 grids are no longer needed
           e32.reset(new ConvEngine<float>(mesh, op));
-          e32->set_reference(Gprev.p, G0prev.p, A.sw_tail, eig_d.p, uni_d.p, mu_prev, T, s);
-          e32->set_vertex_reference(Vprev.p, A.Smat, A.Cmat, s);
+          // chi_ref takes the buffer of the vertex, which is copied to Vref first
+          std::swap(chi_ref, chi);
+          take_reference(chi_acc.p, chi_ref.p);
         }
       }
     } else {
-      e32->sigma_increment(sig_acc.p, chi.p, Vprev.p, T, s, dv_fused);
-      e32->advance(A.sw_tail, s);
+      e32->sigma_increment(sig_acc.p, sig_ref.p, Vout, Vref.p, T, s, dv_fused);
       threshold_copy(sig.p, sig_acc.p, nS, 1e-9, s);
-      std::swap(Vprev, chi);
-      std::swap(Gprev, G);  // G is rewritten by the Dyson step below
-      if (A.sw_tail) {
-        std::swap(G0prev, G0);
-        Jprev = Jcur;
+      if (rebase) {
+        if (verbose) printf("[FLEX] reference moved to this iterate (esterr %9.2E, distance %9.2E)\n", esterr, dist_ref);
+        take_reference(chi_ref.p, Vout);
+        ++n_rebase;
+      } else {
+        dist_ref = max_rel_err(sig_ref.p, sig_acc.p, nS, 1.0e-10, s, nullptr);
       }
-      mu_prev = mu;
       ++n_inc;
     }
     if (A.sub_sigma > 0) subtract_static(sig.p, Nk, (int)Nw, No, (int)A.sub_sigma, s);
@@ -514,9 +544,9 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
     write_record(f, meta, 32);
     if (!f) throw Error("io_sigma: cannot write sigma.bin");
   }
-  double st[11] = {(double)iters, esterr, stoner,  mu,       secs(t_start),  t_chi,
-                   t_vert,        t_sig,  t_mu,    (double)n_inc, t_solve};
-  std::copy(st, st + 11, g_stats);
+  double st[12] = {(double)iters, esterr, stoner, mu,    secs(t_start), t_chi, t_vert, t_sig, t_mu, (double)n_inc,
+                   t_solve,       (double)n_rebase};
+  std::copy(st, st + 12, g_stats);
 }
 
 // ---------------------------------------------------------------- Eliashberg
@@ -1509,7 +1539,7 @@ extern "C" void gmod_set_sigma_seed(const gmod_z* sigma, double mu, double mu_ol
 }
 
 extern "C" int gmod_flex_stats(double* out, int n) {
-  int m = std::min(n, 11);
+  int m = std::min(n, 12);
   std::copy(g_stats, g_stats + m, out);
   return m;
 }

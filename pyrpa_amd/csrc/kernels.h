@@ -393,14 +393,14 @@ struct IncProductLoad {
 
 // Sigma increment per vertex pair: acc (+)= dV~ (Fr + Fd) + FVr Fd, where dV~
 // is the transformed vertex increment, Fr / Fd the transformed reference /
-// increment of G_ab and FVr the transformed reference vertex of the pair;
-// FVr then advances to the current vertex (FVr += dV~).
+// increment of G_ab and FVr the transformed reference vertex of the pair
+// (V G - Vr Gr with V = Vr + dV, G = Gr + dG)
 template <typename R>
 struct SigmaIncStore {
   cpx<R>* acc;
   const cpx<R>* Fr;
   const cpx<R>* Fd;
-  cpx<R>* FVr;
+  const cpx<R>* FVr;
   int first;
   unsigned S;
   // prep (all loads) / commit (arithmetic and stores): the box FFT issues the
@@ -421,11 +421,10 @@ struct SigmaIncStore {
   __device__ void commit(unsigned sp, unsigned t, const Pre& q, cpx<R> v) const {
     const size_t ad = sp + (size_t)S * t;
     acc[ad] = q.acc + v * (q.fr + q.fd) + q.fv * q.fd;
-    FVr[ad] = q.fv + v;
   }
   __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
-  __host__ double algo_bytes(double N) const {  // acc (read unless first), Fr, Fd, FVr read; acc, FVr written
-    return ((first ? 3.0 : 4.0) + 2.0) * N * sizeof(cpx<R>);
+  __host__ double algo_bytes(double N) const {  // acc (read unless first), Fr, Fd, FVr read; acc written
+    return ((first ? 3.0 : 4.0) + 1.0) * N * sizeof(cpx<R>);
   }
   __device__ bool skip(unsigned) const { return false; }
 };
@@ -461,7 +460,7 @@ struct MulAccStore {
 // Accumulator functors of FFTPlan::exec_accum_last.  The per-source operand
 // pointers sit in a device table (tab), so that a run-time source index does
 // not index a kernel argument.
-//   SigmaIncAccum: sum_c [dV~_c (Fr_c + Fd_c) + FVr_c Fd_c], FVr_c += dV~_c
+//   SigmaIncAccum: sum_c [dV~_c (Fr_c + Fd_c) + FVr_c Fd_c]
 //   (SigmaIncStore's increment summed over the sources of one output)
 //   MulAccAccum:   sum_c v_c F_c  (MulAccStore's sum)
 template <typename R>
@@ -475,11 +474,9 @@ struct SigmaIncAccum {
     return Pre{tab[3 * c][ad], tab[3 * c + 1][ad], tab[3 * c + 2][ad]};
   }
   __device__ cpx<R> contrib(int, const Pre& q, cpx<R> v) const { return v * (q.fr + q.fd) + q.fv * q.fd; }
-  __device__ void side(int c, unsigned sp, unsigned t, const Pre& q, cpx<R> v) const {  // FVr_c += v
-    tab[3 * c + 2][sp + __umul24(S, t)] = q.fv + v;
-  }
+  __device__ void side(int, unsigned, unsigned, const Pre&, cpx<R>) const {}
   __device__ void finish(unsigned sp, unsigned t, cpx<R> sum) const { out[sp + __umul24(S, t)] = sum; }
-  __host__ double src_bytes(double N) const { return 4.0 * N * sizeof(cpx<R>); }  // Fr, Fd, FVr read, FVr written
+  __host__ double src_bytes(double N) const { return 3.0 * N * sizeof(cpx<R>); }  // Fr, Fd, FVr read
   __host__ double sum_bytes(double N) const { return N * sizeof(cpx<R>); }
 };
 template <typename R>

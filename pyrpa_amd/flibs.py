@@ -672,10 +672,10 @@ def set_sigma_seed(sigmak, mu, mu_old):
 
 
 def flex_stats():
-    out = np.zeros(11)
-    _L().gmod_flex_stats(out, 11)
+    out = np.zeros(12)
+    _L().gmod_flex_stats(out, 12)
     keys = ("iterations", "esterr", "stoner", "mu", "seconds", "t_chi0", "t_vertex", "t_sigma", "t_mu_dyson",
-            "increment_iterations", "solve_seconds")
+            "increment_iterations", "solve_seconds", "reference_moves")
     return dict(zip(keys, out.tolist()))
 
 

@@ -152,9 +152,10 @@ def _flex_case(case):
 
 @pytest.mark.parametrize("case", ["1orb_conv", "1orb_tail", "2orb_diis_sub2", "3orb_3d_tail"])
 def test_mkself_mixed_precision_matches_fp64(F, case):
-    """FP32 rolling increments around FP64 accumulators: converges to the
-    reference criterion esterr < 1e-8 and lands on the FP64 fixed point
-    (max|dSigma| / max|Sigma| < 1e-6, |dmu| < 1e-6)."""
+    """FP64 values at a reference iterate plus FP32 transforms of the
+    differences from it: converges to the reference criterion esterr < 1e-8
+    and lands on the FP64 fixed point (max|dSigma| / max|Sigma| < 1e-6,
+    |dmu| < 1e-6)."""
     args, kw = _flex_case(case)
     opts = dict(eps=1e-8, pp=0.5, m_diis=1, scf_loop=300)
     opts.update(kw)
