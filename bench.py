@@ -57,7 +57,7 @@ SW_TAIL = True
 # device copy of known size (FETCH_SIZE reads half the bytes on this device),
 # per functor class, weighted by the classes' dispatch counts in the full
 # solve.  Measured offline: counter runs cannot be part of the timed run.
-FP32_BOX_TRAFFIC_BYTES = {3: 1812791481}
+FP32_BOX_TRAFFIC_BYTES = {3: 1462511170}
 
 
 def log(*a):
