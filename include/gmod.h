@@ -173,6 +173,34 @@ void remap_delta_(gmod_z* delta, const gmod_z* delta0, const double* prt, const 
                   const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Norb,
                   const int64_t* gap_sym);
 
+/* ---------------- nonlinear FLEX-Eliashberg ------------------------------- */
+/* libs/src/ffeliash.f90:2-313 [libs/flibs/_eliashberg.py:230-304].  delta (in:
+ * the seed, out: the gap), sigmak (out; zero without sw_sigma) and Gk (in: the
+ * normal G the loop starts from, out: the last normal G, as the reference's
+ * intent(inout) leaves it) are (Nk,Nw,Norb,Norb).  Precision 0 FP64, 1 and 2
+ * FP32 convolutions. */
+void eliashberg(gmod_z* delta, gmod_z* sigmak, gmod_z* Gk, const gmod_z* hamk, const double* Smat,
+                const double* Cmat, const int64_t* olist, const double* prt, const int64_t* kmap, const int64_t* invk,
+                const double* mu, const double* temp, const double* eps, const int64_t* Nkall, const int64_t* Nk,
+                const int64_t* Nw, const int64_t* Nchi, const int64_t* Norb, const int64_t* Nx, const int64_t* Ny,
+                const int64_t* Nz, const int64_t* itemax, const int64_t* gap_sym, const bool* sw_sigma,
+                const bool* sw_Vconst, const int64_t* m_diis, const double* gap_min, const bool* sw_amp_newton);
+/* No counterpart in the reference (its loop prints these): of the last
+ * eliashberg call, [iterations, status (0 itemax reached, 1 converged,
+ * 2 gap collapsed), re-anchors, last max|D_new - D| / max|D|]. */
+int gmod_eliashberg_stats(double* out, int n);
+/* libs/src/ffeliash.f90:459-489 (get_chi0sc_, internal to eliashberg):
+ * Vsigma <- chi^GG + chi^FF, Vdelta <- chi^GG - chi^FF, (Nk,Nw,Nchi,Nchi) */
+void get_chi0sc_(gmod_z* Vsigma, gmod_z* Vdelta, const gmod_z* Gk, const gmod_z* fk, const int64_t* kmap,
+                 const int64_t* invk, const int32_t* irr_chi, const int32_t* chi_map, const int64_t* olist,
+                 const double* prt, const double* temp, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+                 const int64_t* Nw, const int64_t* Nk, const int64_t* Nkall, const int64_t* Norb,
+                 const int64_t* Nchi, const bool* sw_pair);
+/* libs/src/ffeliash.f90:386-457 (mkv_flex_nosoc_, internal to eliashberg): in
+ * place, Vsigma chi0_S -> V_sigma and Vdelta chi0_C -> V_Delta */
+void mkv_flex_nosoc_(gmod_z* Vdelta, gmod_z* Vsigma, const double* Smat, const double* Cmat, const int64_t* Nk,
+                     const int64_t* Nw, const int64_t* Nchi, const bool* sw_pair);
+
 /* ---------------- spinful basis (spin-orbit coupling) -------------------- */
 /* Orbitals carry spin: slist(Norb) = +-1, invs(Norb) the 1-based spin-flipped
  * partner, olist over all Norb^2 pairs, Nchi = Norb^2. */

@@ -2038,3 +2038,248 @@ def conv_delta_orb_to_band_soc(delta, uni, invk, invs, slist):
             continue
         out[i] = L @ Df[i, 0] @ R
     return np.ascontiguousarray(out.T)
+
+
+# --------------------------------------------------------------------------- #
+#  nonlinear FLEX-Eliashberg (ffeliash.f90)
+# --------------------------------------------------------------------------- #
+def _full_ff_leg(Ff, pos_d, neg_d, pos_r, neg_r, c2, cj, kmap, invk, Nx, Ny, Nz):
+    """One leg of the anomalous bubble on the full grid (ffeliash.f90:556-580):
+    slot j <- F(k_irr, w_j, pos), slot M-1-j <- c2 F(k_irr, w_j, neg), the
+    (a, b) pairs pos_d / neg_d at flag-0 points and pos_r / neg_r at -k points
+    exactly as the reference indexes them; cj: conjugated leg (F^+14)."""
+    Nk, Nw = Ff.shape[:2]
+    M = 2 * Nw
+    out = np.zeros((Nx * Ny * Nz, M), dtype=complex)
+    irr = invk[:, 0] - 1
+    fl = invk[:, 1]
+    sp = _spatial(kmap, Nx, Ny, Nz)
+    d = fl == 0
+    r = fl == 1
+    out[sp[d], :Nw] = Ff[irr[d], :, pos_d[0], pos_d[1]]
+    out[sp[d], Nw:] = (c2 * Ff[irr[d], :, neg_d[0], neg_d[1]])[:, ::-1]
+    out[sp[r], :Nw] = Ff[irr[r], :, pos_r[0], pos_r[1]]
+    out[sp[r], Nw:] = (c2 * Ff[irr[r], :, neg_r[0], neg_r[1]])[:, ::-1]
+    if cj:
+        out = np.conj(out)
+    return out.reshape(Nx, Ny, Nz, M)
+
+
+def get_chi0_conv_ff(fk, kmap, invk, olist, prt, temp, Nx, Ny, Nz, singlet=True):
+    """Anomalous bubble chi^FF(q) = -(T/Nkall) sum F^+14(k+q) F23(k) by FFT
+    (ffeliash.f90:491-612): per irreducible pair (l, m) the legs F^+14 and F23
+    with the parity / spin factors sgn = dprt = +1 (singlet) or -1 (triplet),
+    the product with the (-r, -tau) reversal, the 1e-9 threshold at the
+    gather and chi(chi_map(m,l)) = conj(chi(m,l)).  Python in/out orders."""
+    Ff = np.asarray(fk).T
+    prt = np.asarray(prt, dtype=float)
+    ol = olist_f(olist)
+    cmap, irr = get_chi_map(olist)
+    Nk, Nw = Ff.shape[:2]
+    Nchi = len(ol)
+    Nkall = len(kmap)
+    sgn = 1.0 if singlet else -1.0
+    dprt = sgn
+    weight = temp / Nkall
+    chif = np.zeros((Nk, Nw, Nchi, Nchi), dtype=complex)
+    for l, m in irr:
+        l1, l2, m1, m2 = ol[l, 0], ol[l, 1], ol[m, 0], ol[m, 1]
+        c14 = prt[l1] * prt[m2] * sgn * dprt
+        c23 = prt[m1] * prt[l2] * sgn * dprt
+        f14 = fft4(_full_ff_leg(Ff, (m2, l1), (l1, m2), (m1, l1), (l1, m2), c14, True, kmap, invk, Nx, Ny, Nz))
+        f23 = fft4(_full_ff_leg(Ff, (l2, m1), (m1, l2), (l2, m2), (l2, m1), c23, False, kmap, invk, Nx, Ny, Nz))
+        tmp = ifft4(-f14 * _reverse(f23))
+        chif[:, :, m, l] = _threshold(_gather(tmp, kmap, invk, Nk, Nw) * weight)
+        pm, pl = cmap[m, l]
+        chif[:, :, pm, pl] = np.conj(chif[:, :, m, l])
+    return np.ascontiguousarray(chif.T)
+
+
+def get_chi0sc(Gk, fk, kmap, invk, olist, prt, temp, Nx, Ny, Nz, singlet=True):
+    """chi0_S = chi^GG + chi^FF, chi0_C = chi^GG - chi^FF (ffeliash.f90:459-489)."""
+    gg = get_chi0_conv(Gk, kmap, invk, olist, temp, Nx, Ny, Nz)
+    ff = get_chi0_conv_ff(fk, kmap, invk, olist, prt, temp, Nx, Ny, Nz, singlet)
+    return gg + ff, gg - ff
+
+
+def mkV_flex_nosoc(chiS, chiC, Smat, Cmat, singlet=True):
+    """V_sigma and V_Delta of the superconducting loop (ffeliash.f90:386-457):
+    chi_s = (I - chi0_S S)^-1 chi0_S S, chi_c = (I + chi0_C C)^-1 chi0_C C,
+    V_sigma = 3/2 S - 1/2 C + 3/2 S chi_s + 1/2 C chi_c
+              - 1/4 (S + C)(chi0_S S + chi0_C C),
+    V_Delta = (S + C)/2 + 3/2 S chi_s - 1/2 C chi_c (singlet) or
+              (S - C)/2 - 1/2 S chi_s - 1/2 C chi_c (triplet).  Python orders."""
+    AS = np.asarray(chiS).T
+    AC = np.asarray(chiC).T
+    S = np.asarray(Smat).T
+    C = np.asarray(Cmat).T
+    I = np.eye(S.shape[0])
+    XS = AS @ S
+    XC = AC @ C
+    chis = np.linalg.solve(I - XS, XS)
+    chic = np.linalg.solve(I + XC, XC)
+    Vs = (1.5 * S - 0.5 * C) + 1.5 * (S @ chis) + 0.5 * (C @ chic) - 0.25 * ((S + C) @ (XS + XC))
+    if singlet:
+        Vd = 0.5 * (S + C) + 1.5 * (S @ chis) - 0.5 * (C @ chic)
+    else:
+        Vd = 0.5 * (S - C) - 0.5 * (S @ chis) - 0.5 * (C @ chic)
+    return np.ascontiguousarray(Vs.T), np.ascontiguousarray(Vd.T)
+
+
+def _hermitize_f(F):
+    No = F.shape[-1]
+    for l in range(No):
+        F[..., l, l] = F[..., l, l].real
+        for m in range(l + 1, No):
+            F[..., l, m] = np.conj(F[..., m, l])
+    return F
+
+
+def mkgreliah_trs_nsoc(Gk0, fk, delta):
+    """G = G0 (I + Delta F^dagger) per (k, iw) (ffeliash.f90:315-343)."""
+    G0 = np.asarray(Gk0).T
+    Ff = np.asarray(fk).T
+    Df = np.asarray(delta).T
+    No = G0.shape[-1]
+    G = G0 @ (np.eye(No) + Df @ np.conj(np.swapaxes(Ff, -1, -2)))
+    return np.ascontiguousarray(G.T)
+
+
+def mkfkeliash_trs_nsoc(Gk0, Gk, delta):
+    """F = -G0 Delta G^dagger, then diagonal real and upper = conj(lower)
+    (ffeliash.f90:345-384)."""
+    G0 = np.asarray(Gk0).T
+    Gf = np.asarray(Gk).T
+    Df = np.asarray(delta).T
+    F = -(G0 @ (Df @ np.conj(np.swapaxes(Gf, -1, -2))))
+    return np.ascontiguousarray(_hermitize_f(F).T)
+
+
+def eliashberg(delta, Gk, hamk, Smat, Cmat, olist, prt, kmap, invk, mu, temp, eps, Nx, Ny, Nz, itemax, gap_sym,
+               sw_sigma=False, sw_Vconst=False, m_diis=5, gap_min=1.0e-2, sw_amp_newton=True, info=None):
+    """Non-linear FLEX-Eliashberg loop (ffeliash.f90:2-313), step for step:
+    gap equation, convergence and gap-collapse tests, amplitude phase
+    (probe / pitchfork fit / anchoring) or Pulay-DIIS (linear mixing pp = 0.3
+    as fallback), FLEX Sigma and the dressed G0, the superconducting Dyson
+    updates of G and F, chi0_S / chi0_C and the vertices.  Python in/out
+    orders; returns (delta, sigmak, Gk) where Gk is the final normal G (the
+    reference's Gk argument is intent(inout))."""
+    D = np.array(np.asarray(delta).T, dtype=complex)
+    G = np.array(np.asarray(Gk).T, dtype=complex)
+    Nk, Nw, No, _ = D.shape
+    Nkall = len(kmap)
+    singlet = gap_sym >= 0
+    weight = temp / Nkall
+    pp, eps_reg, amp_tol, n_polish_max = 0.3, 1.0e-12, 2.0e-2, 10
+    m_eff = max(int(m_diis), 1)
+    xout = [None] * m_eff
+    res = [None] * m_eff
+    n_hist = i_hist = 0
+    amp_phase, npts, n_polish, n_anchor = 1, 0, 0, 0
+    a1 = a2 = g1 = g2 = 0.0
+    a_cur = np.sqrt(np.vdot(D, D).real) + 1.0e-300
+    v0 = D / a_cur
+    if not sw_amp_newton:
+        amp_phase = 2
+    P = lambda X: np.ascontiguousarray(X.T)      # Fortran-order work array -> Python order
+    Fo = lambda X: np.asarray(X).T               # Python order -> Fortran view
+    G0d = G.copy()
+    delta0_max = np.abs(D).max() + 1.0e-30
+    if sw_Vconst:
+        fk = np.zeros_like(G)
+        cS, cC = get_chi0sc(P(G), P(fk), kmap, invk, olist, prt, temp, Nx, Ny, Nz, singlet)
+        fk = Fo(mkfk_trs_nsoc(P(G), P(D))).copy()
+    else:
+        fk = Fo(mkfk_trs_nsoc(P(G), P(D))).copy()
+        cS, cC = get_chi0sc(P(G), P(fk), kmap, invk, olist, prt, temp, Nx, Ny, Nz, singlet)
+    stoner = ckchi(cS, Smat, Cmat)
+    Vs, Vd = mkV_flex_nosoc(cS, cC, Smat, Cmat, singlet)
+    sig = np.zeros_like(G)
+    it_done = 0
+    status = "itemax"
+    for it in range(1, itemax + 1):
+        it_done = it
+        nd = mkdelta_nsoc(P(fk), Vd, Smat, Cmat, kmap, invk, prt, olist, Nx, Ny, Nz, singlet) * weight  # Fortran order
+        diff = np.abs(nd - D).max()
+        dnorm = np.abs(D).max() + 1.0e-30
+        new_max = np.abs(nd).max()
+        if info is not None:
+            info.setdefault("trace", []).append((it, diff / dnorm, new_max, amp_phase))
+        if diff / dnorm < eps:
+            D = nd.copy()
+            status = "converged"
+            break
+        if gap_min > 0.0 and new_max < gap_min * delta0_max:
+            D = nd.copy()
+            status = "collapsed"
+            break
+        if sw_amp_newton and amp_phase == 2 and n_polish >= n_polish_max:
+            a_cur = np.sqrt(np.vdot(D, D).real) + 1.0e-300
+            v0 = D / a_cur
+            npts, a1, a2, g1, g2 = 0, 0.0, 0.0, 0.0, 0.0
+            amp_phase = 1
+            n_anchor += 1
+        if amp_phase == 1:
+            proj = np.vdot(v0, nd).real
+            g_cur = proj / a_cur
+            a2, g2, a1, g1 = a1, g1, a_cur, g_cur
+            npts = min(npts + 1, 2)
+            if npts >= 2 and abs(a1 * a1 - a2 * a2) > 1.0e-30:
+                cfit = (g1 - g2) / (a2 * a2 - a1 * a1)
+                lam0 = g1 + cfit * a1 * a1
+                if cfit > 1.0e-12 and lam0 > 1.0:
+                    a_nxt = np.sqrt((lam0 - 1.0) / cfit)
+                    a_nxt = min(max(a_nxt, 0.3 * a_cur), 3.0 * a_cur)
+                else:
+                    a_nxt = 0.5 * a_cur
+            else:
+                a_nxt = 0.9 * a_cur if abs(g_cur - 1.0) < 0.2 else 0.6 * a_cur
+            if abs(g_cur - 1.0) < amp_tol:
+                amp_phase = 2
+                a_nxt = a_cur
+                n_polish = 0
+                n_hist = i_hist = 0
+            D = a_nxt * v0
+            a_cur = a_nxt
+        else:
+            i_hist = i_hist % m_eff + 1
+            n_hist = min(n_hist + 1, m_eff)
+            n = n_hist
+            xout[i_hist - 1] = nd.copy()
+            res[i_hist - 1] = nd - D
+            do_pulay = n >= 2 and m_eff >= 2
+            if do_pulay:
+                idx = [(i_hist - n + ih - 1) % m_eff for ih in range(1, n + 1)]
+                B = np.zeros((n + 1, n + 1))
+                for a in range(n):
+                    for b in range(n):
+                        B[a, b] = np.vdot(res[idx[a]], res[idx[b]]).real
+                    B[a, a] += eps_reg
+                    B[a, n] = B[n, a] = 1.0
+                rhs = np.zeros(n + 1)
+                rhs[n] = 1.0
+                if info is not None:
+                    info.setdefault("diis_cond", []).append(float(np.linalg.cond(B)))
+                try:
+                    c = np.linalg.solve(B, rhs)
+                except np.linalg.LinAlgError:
+                    do_pulay = False
+            if do_pulay:
+                D = np.zeros_like(D)
+                for a in range(n):
+                    D = D + c[a] * xout[idx[a]]
+            else:
+                D = pp * nd + (1.0 - pp) * D
+            n_polish += 1
+        if sw_sigma:
+            sig = Fo(calc_sigma(P(G), Vs, Smat, Cmat, kmap, invk, olist, temp, Nx, Ny, Nz)).copy()
+            G0d = Fo(gen_green(P(sig), hamk, mu, temp)).copy()
+        G = Fo(mkgreliah_trs_nsoc(P(G0d), P(fk), P(D))).copy()
+        fk = Fo(mkfkeliash_trs_nsoc(P(G0d), P(G), P(D))).copy()
+        if not sw_Vconst:
+            cS, cC = get_chi0sc(P(G), P(fk), kmap, invk, olist, prt, temp, Nx, Ny, Nz, singlet)
+            stoner = ckchi(cS, Smat, Cmat)
+            Vs, Vd = mkV_flex_nosoc(cS, cC, Smat, Cmat, singlet)
+    if info is not None:
+        info.update(iters=it_done, status=status, anchors=n_anchor, stoner=stoner)
+    return P(D), P(sig), P(G)

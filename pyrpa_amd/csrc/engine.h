@@ -52,6 +52,12 @@ class ConvEngine {
   // stat_h[m + Nchi*l] (-Vmat(m,l) in a spinful basis, fselfwithsoc.f90:592)
   void sigma(zd* sig, const zd* V, const double* stat_h, double temp, hipStream_t s, bool apply_threshold = true);
 
+  // chi^FF(Nk,Nw,Nchi,Nchi) <- the anomalous bubble of F (ffeliash.f90:491-612):
+  // per irreducible pair, the legs F^+14 and F23 (FFLoad) transformed, their
+  // product with the (-r,-tau) reversal inverse-transformed and gathered with
+  // T/Nkall, the partner slot conjugated; thresholds at 1e-9
+  void chi0_ff(zd* chi, const zd* F, const double* prt, bool singlet, double temp, hipStream_t s);
+
   // the spatial transforms of the tau reference kernels TA, TB of pair (a, b)
   // at mu, or (diff) of their change from mu2 to mu (fchi.f90:703-812)
   void tau_grids(cpx<R>* ta, cpx<R>* tb, int a, int b, const double* eig_d, const zd* uni_d, double mu, double mu2,
@@ -106,6 +112,7 @@ class ConvEngine {
   DBuf<cpx<R>> Fg, F0g, TA, TB, work, tmp, acc, Vft, Ff;
   DBuf<cpx<R>> Fr, Fd, F0r, F0d, TAr, TBr, TAd, TBd, FVr;  // increment mode (r: reference, d: difference)
   DBuf<cpx<R>> Mg;  // mirrored Fg (G(-r,-t)): the second factor of the bubble, read in place
+  DBuf<cpx<R>> Lff;  // the two legs of one anomalous-bubble pair
   // Sigma through the accumulating last pass: the sources of one output
   // after their first passes, and the device table of their operand grids
   DBuf<cpx<R>> srcs;

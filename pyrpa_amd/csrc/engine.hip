@@ -360,6 +360,39 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
 }
 
 template <typename R>
+void ConvEngine<R>::chi0_ff(zd* chi, const zd* F, const double* prt, bool singlet, double temp, hipStream_t s) {
+  const int No = op.No, Nchi = op.Nchi;
+  const long NkNw = mesh.Nk * (long)mesh.Nw;
+  const double w = temp / (double)mesh.Nkall;
+  // F(-k) = sgn F^T(k) and the gap's frequency parity dprt: both +1 for a
+  // singlet, -1 for a triplet (ffeliash.f90:541-547)
+  const double sgn = singlet ? 1.0 : -1.0, dprt = sgn;
+  if (Lff.n != (size_t)2 * N) Lff.alloc((size_t)2 * N);
+  cpx<R>* A = Lff.p;
+  cpx<R>* B = Lff.p + N;
+  const MeshDev md = mesh_dev(mesh);
+  const unsigned S = (unsigned)mesh.Nkall;
+  auto off = [&](int a, int b) { return NkNw * (long)(a + No * b); };
+  if (!mesh.complete) GMOD_HIP(hipMemsetAsync(chi, 0, NkNw * Nchi * Nchi * sizeof(zd), s));
+  for (auto& lm : op.irr) {
+    const int l = lm[0], m = lm[1];
+    const int l1 = op.ol[l][0], l2 = op.ol[l][1], m1 = op.ol[m][0], m2 = op.ol[m][1];
+    const FFLoad<R> f14{F, off(m2, l1), off(l1, m2), off(m1, l1), off(l1, m2), prt[l1] * prt[m2] * sgn * dprt, 1, md};
+    const FFLoad<R> f23{F, off(l2, m1), off(m1, l2), off(l2, m2), off(l2, m1), prt[m1] * prt[l2] * sgn * dprt, 0, md};
+    p4->exec_fused(f14, GridStore<R>{A, S}, A, tmp.p, -1, (R)1, s);
+    p4->exec_fused(f23, GridStore<R>{B, S}, B, tmp.p, -1, (R)1, s);
+    ProductLoad<R> pl{};
+    pl.m = md;
+    pl.add(A, B, (R)-1, 1);
+    zd* dst = chi + NkNw * (m + (long)Nchi * l);
+    product_pass(pl, gather_sf(dst, w, zd(0, 0), nullptr, 0, s), s);
+    auto pr = op.partner[m + Nchi * l];
+    if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
+  }
+  threshold(chi, NkNw * Nchi * Nchi, 1e-9, s);
+}
+
+template <typename R>
 void ConvEngine<R>::tau_reference(zd* chi, double temp, const double* eig_d, const zd* uni_d, double mu,
                                   const std::vector<zd>& J, const zd* sawc_d, hipStream_t s) {
   const int No = op.No, Nchi = op.Nchi;

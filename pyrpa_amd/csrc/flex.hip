@@ -901,6 +901,208 @@ static double eliash_soc_driver(const EliashSocIn& A, zd* delta_out) {
   return lam;
 }
 
+// ---------------------------------------------------------------- nonlinear Eliashberg
+struct NonlinIn {
+  const zd *Gk, *hamk;
+  const double *Smat, *Cmat, *prt;
+  const int64_t *olist, *kmap, *invk;
+  double mu, temp, eps, gap_min;
+  long Nkall, Nk, Nw, Nx, Ny, Nz, itemax, gap_sym, m_diis;
+  bool sw_sigma, sw_Vconst, sw_amp_newton;
+  int No, Nchi;
+};
+static double g_nonlin_stats[6];  // iterations, status (0 itemax, 1 converged, 2 collapsed), anchors, last rel. change
+
+// Non-linear FLEX-Eliashberg loop (ffeliash.f90:2-313).  Per iteration: the
+// gap equation (the Eliashberg gap convolution), the convergence and
+// gap-collapse tests, the amplitude phase (Delta = a v0, a from a probe or the
+// pitchfork fit g(a) = lambda0 - c a^2 of the v0-projected gain) or a
+// Pulay-DIIS / linear-mixing (pp = 0.3) step, the FLEX Sigma and the dressed
+// G0, one step of the superconducting Dyson equations for G and F, then (unless
+// the vertex is held constant) chi0_S = chi^GG + chi^FF, chi0_C = chi^GG -
+// chi^FF and the vertices V_sigma, V_Delta.  Delta (in/out), Sigma and the
+// final normal G are returned; every field stays on the device.
+template <typename R>
+static void nonlin_eliash_driver(const NonlinIn& A, zd* delta_io, zd* sigma_out, zd* Gk_io) {
+  hipStream_t s = ctx().stream;
+  const int verbose = ctx().verbose;
+  const int No = A.No, Nchi = A.Nchi;
+  const long Nk = A.Nk, Nw = A.Nw, NkNw = Nk * Nw;
+  const size_t nS = (size_t)NkNw * No * No, nC = (size_t)NkNw * Nchi * Nchi;
+  const bool singlet = A.gap_sym >= 0;
+  const double T = A.temp, weight = T / (double)A.Nkall;
+  const double pp = 0.3, eps_reg = 1.0e-12, amp_tol = 2.0e-2;  // ffeliash.f90:76-77
+  const int n_polish_max = 10;
+  Mesh mesh;
+  mesh.build(A.kmap, A.invk, A.Nkall, Nk, (int)A.Nx, (int)A.Ny, (int)A.Nz, (int)Nw, s);
+  OrbPairs op;
+  op.from_olist(A.olist, Nchi, No);
+  Upload up{s};
+  DBuf<zd> D, G, ham, G0d(nS), F(nS), nd(nS), sig(nS), v0(nS), VS(nC), VD(nC);
+  DBuf<double> S_d, C_d;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(D, delta_io, nS);
+  up(G, Gk_io, nS);
+  up(ham, A.hamk, (size_t)No * No * Nk);
+  up(S_d, A.Smat, (size_t)Nchi * Nchi);
+  up(C_d, A.Cmat, (size_t)Nchi * Nchi);
+  ConvEngine<R> eng(mesh, op);
+  const int m_eff = (int)std::max<long>(A.m_diis, 1);
+  std::vector<DBuf<zd>> xout(m_eff), res(m_eff);
+  for (int i = 0; i < m_eff; ++i) { xout[i].alloc(nS); res[i].alloc(nS); }
+  auto sumsq = [&](const zd* a) { return dot_w(a, a, nullptr, (long)nS, 1, s).re; };
+  // VS <- chi0_S, VD <- chi0_C from G and F (ffeliash.f90:459-489)
+  auto chi0sc = [&]() {
+    eng.transform_green(G.p, false, s);
+    eng.chi0(VS.p, T, false, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, s, true);
+    eng.chi0_ff(VD.p, F.p, A.prt, singlet, T, s);
+    sum_diff(VS.p, VD.p, (long)nC, s);
+  };
+  auto vertices = [&]() {
+    if (verbose) printf("SDW max eig(chi0_S S) = %12.8f\n", stoner_factor(VS.p, Nk, Nw, Nchi, S_d.p, s, 0.0, 1.0));
+    sc_vertex(VS.p, VD.p, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, singlet, err.p, s);
+    check_err(err, s, "mkV_flex_nosoc");
+    eng.prepare_gap(VD.p, A.Smat, A.Cmat, singlet, s);
+  };
+  // amplitude state (ffeliash.f90:88-101)
+  int amp_phase = A.sw_amp_newton ? 1 : 2, npts = 0, n_polish = 0, n_anchor = 0;
+  double a1 = 0, a2 = 0, g1 = 0, g2 = 0;
+  double a_cur = std::sqrt(sumsq(D.p)) + 1.0e-300;
+  copy(v0.p, D.p, nS, s);
+  scale(v0.p, nS, 1.0 / a_cur, s);
+  int n_hist = 0, i_hist = 0;
+  if (verbose) printf("%s\n", singlet ? "singlet" : "triplet");
+  copy(G0d.p, G.p, nS, s);
+  const double delta0_max = max_abs_diff(D.p, nullptr, (long)nS, s) + 1.0e-30;
+  if (A.sw_Vconst) {
+    GMOD_HIP(hipMemsetAsync(F.p, 0, nS * sizeof(zd), s));
+    chi0sc();
+    anomalous_f(F.p, G.p, D.p, Nk, (int)Nw, No, s);
+  } else {
+    anomalous_f(F.p, G.p, D.p, Nk, (int)Nw, No, s);
+    chi0sc();
+  }
+  vertices();
+  GMOD_HIP(hipMemsetAsync(sig.p, 0, nS * sizeof(zd), s));
+  long it_done = 0;
+  int status = 0;
+  double rel = 0.0;
+  for (long it = 1; it <= A.itemax; ++it) {
+    it_done = it;
+    if (verbose) printf("iter=%5ld\n", it);
+    eng.gap(nd.p, F.p, A.prt, singlet, s);
+    scale(nd.p, nS, weight, s);
+    const double diff = max_abs_diff(nd.p, D.p, (long)nS, s);
+    const double dnorm = max_abs_diff(D.p, nullptr, (long)nS, s) + 1.0e-30;
+    const double new_max = max_abs_diff(nd.p, nullptr, (long)nS, s);
+    rel = diff / dnorm;
+    if (verbose) printf("  |D_new-D|max=%12.4E  rel=%12.4E  |D_new|max=%12.4E\n", diff, rel, new_max);
+    if (rel < A.eps) {
+      copy(D.p, nd.p, nS, s);
+      status = 1;
+      if (verbose) printf("converged at iter=%5ld\n", it);
+      break;
+    }
+    if (A.gap_min > 0.0 && new_max < A.gap_min * delta0_max) {
+      copy(D.p, nd.p, nS, s);
+      status = 2;
+      if (verbose) printf("gap collapsed: |D_new|max=%12.4E < %12.4E, stopping\n", new_max, A.gap_min * delta0_max);
+      break;
+    }
+    if (A.sw_amp_newton && amp_phase == 2 && n_polish >= n_polish_max) {
+      a_cur = std::sqrt(sumsq(D.p)) + 1.0e-300;
+      copy(v0.p, D.p, nS, s);
+      scale(v0.p, nS, 1.0 / a_cur, s);
+      npts = 0; a1 = a2 = g1 = g2 = 0.0;
+      amp_phase = 1;
+      ++n_anchor;
+      if (verbose) printf("  re-anchor #%d\n", n_anchor);
+    }
+    if (amp_phase == 1) {
+      const double proj = dot_w(v0.p, nd.p, nullptr, (long)nS, 1, s).re;
+      const double g_cur = proj / a_cur;
+      a2 = a1; g2 = g1; a1 = a_cur; g1 = g_cur;
+      npts = std::min(npts + 1, 2);
+      double a_nxt;
+      if (npts >= 2 && std::fabs(a1 * a1 - a2 * a2) > 1.0e-30) {
+        const double cfit = (g1 - g2) / (a2 * a2 - a1 * a1);
+        const double lam0 = g1 + cfit * a1 * a1;
+        if (cfit > 1.0e-12 && lam0 > 1.0) {
+          a_nxt = std::sqrt((lam0 - 1.0) / cfit);
+          a_nxt = std::min(std::max(a_nxt, 0.3 * a_cur), 3.0 * a_cur);
+        } else {
+          a_nxt = 0.5 * a_cur;
+        }
+      } else {
+        a_nxt = std::fabs(g_cur - 1.0) < 0.2 ? 0.9 * a_cur : 0.6 * a_cur;
+      }
+      if (verbose) printf("  amp: g=%9.4f  a=%11.4E -> %11.4E\n", g_cur, a_cur, a_nxt);
+      if (std::fabs(g_cur - 1.0) < amp_tol) {
+        amp_phase = 2;
+        a_nxt = a_cur;
+        n_polish = 0;
+        n_hist = i_hist = 0;
+      }
+      copy(D.p, v0.p, nS, s);
+      scale(D.p, nS, a_nxt, s);
+      a_cur = a_nxt;
+    } else {
+      i_hist = i_hist % m_eff + 1;
+      n_hist = std::min(n_hist + 1, m_eff);
+      const int n = n_hist;
+      copy(xout[i_hist - 1].p, nd.p, nS, s);
+      lincomb(res[i_hist - 1].p, nd.p, 1.0, D.p, -1.0, nS, s);
+      bool pulay = n >= 2 && m_eff >= 2;
+      std::vector<double> c(n + 1, 0.0);
+      std::vector<int> idx(n);
+      for (int ih = 1; ih <= n; ++ih) idx[ih - 1] = ((i_hist - n + ih - 1) % m_eff + m_eff) % m_eff;
+      if (pulay) {
+        std::vector<double> B((n + 1) * (n + 1), 0.0);
+        for (int a = 0; a < n; ++a) {
+          for (int b = a; b < n; ++b) {
+            const double v = dot_w(res[idx[a]].p, res[idx[b]].p, nullptr, (long)nS, 1, s).re;
+            B[a + (n + 1) * b] = B[b + (n + 1) * a] = v;
+          }
+          B[a + (n + 1) * a] += eps_reg;
+          B[a + (n + 1) * n] = B[n + (n + 1) * a] = 1.0;
+        }
+        c[n] = 1.0;
+        if (!host_dgesv(B, n + 1, c)) {
+          if (verbose) printf("DIIS: dgesv failed, falling back to linear mixing pp=%4.2f.\n", pp);
+          pulay = false;
+        }
+      }
+      if (pulay) {
+        GMOD_HIP(hipMemsetAsync(D.p, 0, nS * sizeof(zd), s));
+        for (int a = 0; a < n; ++a) axpy(D.p, xout[idx[a]].p, zd(c[a], 0), nS, s);
+      } else {
+        lincomb(D.p, nd.p, pp, D.p, 1.0 - pp, nS, s);
+      }
+      ++n_polish;
+    }
+    if (A.sw_sigma) {
+      eng.transform_green(G.p, false, s);
+      eng.sigma(sig.p, VS.p, A.Smat, A.Cmat, T, s, true);
+      dyson(G0d.p, sig.p, ham.p, A.mu, T, Nk, (int)Nw, No, err.p, s);
+      check_err(err, s, "getinv");
+    }
+    sc_green_update(G.p, F.p, G0d.p, D.p, Nk, (int)Nw, No, s);
+    if (!A.sw_Vconst) {
+      chi0sc();
+      vertices();
+    }
+  }
+  D.download(delta_io, nS, s);
+  sig.download(sigma_out, nS, s);
+  G.download(Gk_io, nS, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  g_nonlin_stats[0] = (double)it_done;
+  g_nonlin_stats[1] = (double)status;
+  g_nonlin_stats[2] = (double)n_anchor;
+  g_nonlin_stats[3] = rel;
+}
+
 }  // namespace gmod
 
 using namespace gmod;
@@ -1565,6 +1767,97 @@ extern "C" void lin_eliash(gmod_z* delta, gmod_z* chi, const gmod_z* Gk, const g
   *lambda_out = 0.0;
   if (ctx().precision >= 1) *lambda_out = eliash_driver<float>(A, (zd*)delta);
   else *lambda_out = eliash_driver<double>(A, (zd*)delta);
+  GMOD_CATCH
+}
+
+extern "C" void eliashberg(gmod_z* delta, gmod_z* sigmak, gmod_z* Gk, const gmod_z* hamk, const double* Smat,
+                           const double* Cmat, const int64_t* olist, const double* prt, const int64_t* kmap,
+                           const int64_t* invk, const double* mu, const double* temp, const double* eps,
+                           const int64_t* Nkall, const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi,
+                           const int64_t* Norb, const int64_t* Nx, const int64_t* Ny, const int64_t* Nz,
+                           const int64_t* itemax, const int64_t* gap_sym, const bool* sw_sigma,
+                           const bool* sw_Vconst, const int64_t* m_diis, const double* gap_min,
+                           const bool* sw_amp_newton) {
+  GMOD_TRY
+  auto g = api_lock();
+  NonlinIn A;
+  A.Gk = (const zd*)Gk; A.hamk = (const zd*)hamk;
+  A.Smat = Smat; A.Cmat = Cmat; A.prt = prt; A.olist = olist; A.kmap = kmap; A.invk = invk;
+  A.mu = *mu; A.temp = *temp; A.eps = *eps; A.gap_min = *gap_min;
+  A.Nkall = *Nkall; A.Nk = *Nk; A.Nw = *Nw; A.Nx = *Nx; A.Ny = *Ny; A.Nz = *Nz;
+  A.itemax = *itemax; A.gap_sym = *gap_sym; A.m_diis = *m_diis;
+  A.sw_sigma = *sw_sigma; A.sw_Vconst = *sw_Vconst; A.sw_amp_newton = *sw_amp_newton;
+  A.No = (int)*Norb; A.Nchi = (int)*Nchi;
+  if (ctx().precision >= 1) nonlin_eliash_driver<float>(A, (zd*)delta, (zd*)sigmak, (zd*)Gk);
+  else nonlin_eliash_driver<double>(A, (zd*)delta, (zd*)sigmak, (zd*)Gk);
+  GMOD_CATCH
+}
+
+extern "C" int gmod_eliashberg_stats(double* out, int n) {
+  const int m = std::min(n, 4);
+  std::copy(g_nonlin_stats, g_nonlin_stats + m, out);
+  return m;
+}
+
+// the superconducting bubbles chi0_S = chi^GG + chi^FF, chi0_C = chi^GG - chi^FF
+// (ffeliash.f90:459-489; its irr_chi / chi_map tables as get_chi0_conv_ takes them)
+extern "C" void get_chi0sc_(gmod_z* Vsigma, gmod_z* Vdelta, const gmod_z* Gk, const gmod_z* fk, const int64_t* kmap,
+                            const int64_t* invk, const int32_t* irr_chi, const int32_t* chi_map,
+                            const int64_t* olist, const double* prt, const double* temp, const int64_t* Nx,
+                            const int64_t* Ny, const int64_t* Nz, const int64_t* Nw, const int64_t* Nk,
+                            const int64_t* Nkall, const int64_t* Norb, const int64_t* Nchi, const bool* sw_pair) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const int No = (int)*Norb, Nc = (int)*Nchi;
+  const size_t nS = (size_t)*Nk * *Nw * No * No, nC = (size_t)*Nk * *Nw * Nc * Nc;
+  OrbPairs op;
+  op.from_olist(olist, Nc, No);
+  op.from_maps(irr_chi, chi_map);
+  Mesh mesh;
+  mesh.build(kmap, invk, *Nkall, *Nk, (int)*Nx, (int)*Ny, (int)*Nz, (int)*Nw, s);
+  Upload up{s};
+  DBuf<zd> G, F, cs(nC), cc(nC);
+  up(G, (const zd*)Gk, nS);
+  up(F, (const zd*)fk, nS);
+  auto run = [&](auto eng) {
+    eng->transform_green(G.p, false, s);
+    eng->chi0(cs.p, *temp, false, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, s, true);
+    eng->chi0_ff(cc.p, F.p, prt, *sw_pair, *temp, s);
+  };
+  if (ctx().precision >= 1) run(std::make_unique<ConvEngine<float>>(mesh, op));
+  else run(std::make_unique<ConvEngine<double>>(mesh, op));
+  sum_diff(cs.p, cc.p, (long)nC, s);
+  cs.download((zd*)Vsigma, nC, s);
+  cc.download((zd*)Vdelta, nC, s);
+  GMOD_HIP(hipStreamSynchronize(s));
+  GMOD_CATCH
+}
+
+// the vertices of the superconducting loop in place (ffeliash.f90:386-457):
+// Vsigma chi0_S -> V_sigma, Vdelta chi0_C -> V_Delta
+extern "C" void mkv_flex_nosoc_(gmod_z* Vdelta, gmod_z* Vsigma, const double* Smat, const double* Cmat,
+                                const int64_t* Nk, const int64_t* Nw, const int64_t* Nchi, const bool* sw_pair) {
+  GMOD_TRY
+  auto g = api_lock();
+  hipStream_t s = ctx().stream;
+  const long NkNw = *Nk * *Nw;
+  const int n = (int)*Nchi;
+  const size_t nC = (size_t)NkNw * n * n;
+  Upload up{s};
+  DBuf<zd> vs, vd;
+  DBuf<double> S, C;
+  DBuf<int> err(1);
+  err.zero(s);
+  up(vs, (const zd*)Vsigma, nC);
+  up(vd, (const zd*)Vdelta, nC);
+  up(S, Smat, (size_t)n * n);
+  up(C, Cmat, (size_t)n * n);
+  sc_vertex(vs.p, vd.p, NkNw, BatchLayout{1, NkNw}, n, S.p, C.p, *sw_pair, err.p, s);
+  check_err(err, s, "mkV_flex_nosoc");
+  vs.download((zd*)Vsigma, nC, s);
+  vd.download((zd*)Vdelta, nC, s);
+  GMOD_HIP(hipStreamSynchronize(s));
   GMOD_CATCH
 }
 

@@ -231,6 +231,32 @@ struct AnomLoad {
   }
 };
 
+// one leg of the anomalous bubble chi^FF (ffeliash.f90:556-580): slot t < Nw
+// reads F(k_irr, w_t) at orbital pair od (flag-0 point) or or_ (-k point),
+// slot M-1-j reads c2 F(k_irr, w_j) at pair nd / nr; cj conjugates the leg
+// (F^+14).  The four pairs are the reference's own index choices.
+template <typename R>
+struct FFLoad {
+  const zd* F;
+  long od, nd, or_, nr;  // element offsets of the (a, b) slices
+  double c2;
+  int cj;
+  MeshDev m;
+  __device__ cpx<R> operator()(unsigned sp, unsigned tu) const {
+    const int t = (int)tu;
+    const long k = m.irr_s[sp];
+    const int f = m.flag_s[sp];
+    zd v;
+    if (t < m.Nw) {
+      v = F[k + m.Nk * (long)t + (f == 0 ? od : or_)];
+    } else {
+      const int j = m.M - 1 - t;
+      v = F[k + m.Nk * (long)j + (f == 0 ? nd : nr)] * c2;
+    }
+    return cpx<R>(cj ? conj(v) : v);
+  }
+};
+
 // sum_n U_an conj(U_bn) g_n(k, +-tau_s); -k uses the conjugated coefficient
 // (fchi.f90:703-729, 791-812), overflow-safe forms of the reference
 template <typename R>
@@ -583,6 +609,8 @@ void add_inplace(cpx<R>* y, const cpx<R>* x, long n, hipStream_t s);  // y += x
 zd dot_w(const zd* u, const zd* v, const double* kw, long Nk, long rest, hipStream_t s);  // sum kw conj(u) v
 double max_rel_err(const zd* s0, const zd* sd, long n, double floor, hipStream_t s, long* where);
 double max_real(const zd* a, long n, hipStream_t s);
+double max_abs_diff(const zd* a, const zd* b, long n, hipStream_t s);                      // max |a - b| (b null: max |a|)
+void sum_diff(zd* a, zd* b, long n, hipStream_t s);                                        // a, b <- a + b, a - b
 double min_real(const zd* a, long n, hipStream_t s);
 
 }  // namespace gmod

@@ -333,6 +333,19 @@ void lincomb(zd* y, const zd* a, double ca, const zd* b, double cb, long n, hipS
   GMOD_CHECK_LAUNCH();
 }
 
+// a <- a + b, b <- a - b (with the old a)
+__global__ void k_sum_diff(zd* a, zd* b, long n) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const zd x = a[e], y = b[e];
+    a[e] = x + y;
+    b[e] = x - y;
+  }
+}
+void sum_diff(zd* a, zd* b, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_diff, dim3(nblocks(n)), dim3(TPB), 0, s, a, b, n);
+  GMOD_CHECK_LAUNCH();
+}
+
 // ------------------------------------------------------------------ reductions
 constexpr int RED_BLOCKS = 1024;
 
@@ -435,6 +448,31 @@ double max_rel_err(const zd* s0, const zd* sd, long n, double floor, hipStream_t
   hipLaunchKernelGGL(k_max_rel, dim3(RED_BLOCKS), dim3(TPB), 0, s, s0, sd, n, floor, part.p);
   GMOD_CHECK_LAUNCH();
   if (where) *where = -1;
+  return host_reduce(part.p, RED_BLOCKS, s, 0);
+}
+
+// max |a - b| (b null: max |a|)
+__global__ void k_max_absdiff(const zd* a, const zd* b, long n, double* part) {
+  __shared__ double sh[TPB];
+  double best = 0.0;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const zd d = b ? a[e] - b[e] : a[e];
+    const double r = hypot(d.re, d.im);
+    if (r > best) best = r;
+  }
+  sh[threadIdx.x] = best;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+double max_abs_diff(const zd* a, const zd* b, long n, hipStream_t s) {
+  DBuf<double> part(RED_BLOCKS);
+  hipLaunchKernelGGL(k_max_absdiff, dim3(RED_BLOCKS), dim3(TPB), 0, s, a, b, n, part.p);
+  GMOD_CHECK_LAUNCH();
   return host_reduce(part.p, RED_BLOCKS, s, 0);
 }
 

@@ -61,6 +61,13 @@ void green_inv(zd* G, const zd* sigma, const zd* hamk, double mu, double temp, l
                hipStream_t s);
 // F = -G Delta G^dagger, Hermitised (feliash.f90:445-484)
 void anomalous_f(zd* F, const zd* G, const zd* D, long Nk, int Nw, int No, hipStream_t s);
+// superconducting Dyson step (ffeliash.f90:315-384): G <- G0 (I + D F^+) with
+// the F given, then F <- -G0 D G^+ (Hermitised), in place on G and F
+void sc_green_update(zd* G, zd* F, const zd* G0, const zd* D, long Nk, int Nw, int No, hipStream_t s);
+// vertices of the superconducting loop (ffeliash.f90:386-457): vs holds chi0_S
+// and vd chi0_C on entry, V_sigma and V_Delta on exit; err set on a singular system
+void sc_vertex(zd* vs, zd* vd, long batch, BatchLayout L, int n, const double* S, const double* C, bool singlet,
+               int* err, hipStream_t s);
 // Hermitise: upper = conj(lower), diagonal real  (feliash.f90:618-634)
 void hermitize_lower(zd* D, long Nk, int Nw, int No, hipStream_t s);
 // spinful Eliashberg on the full zone, X(Nkall, Nw, No, No); sl = slist,

@@ -143,6 +143,74 @@ k_rpa_vertex(zd* chi, BatchLayout L, long batch, int n, const double* S, const d
   }
 }
 
+// Vertices of the superconducting loop (ffeliash.f90:386-457) per (q, i nu):
+// on entry vs holds chi0_S and vd chi0_C; on exit vs holds V_sigma and vd
+// V_Delta (singlet or triplet).  chi_s, chi_c by pivoted LUs in LDS as in
+// k_rpa_vertex, the S system on chi0_S and the C system on chi0_C.
+__global__ void __launch_bounds__(64)
+k_sc_vertex(zd* vs, zd* vd, BatchLayout L, long batch, int n, const double* S, const double* C, int singlet,
+            int* err) {
+  extern __shared__ unsigned char sm[];
+  const int nn = n * n;
+  double* Ss = reinterpret_cast<double*>(sm);
+  double* Cs = Ss + nn;
+  zd* A = reinterpret_cast<zd*>(Cs + nn);
+  zd* M1 = A + nn;
+  zd* X1 = M1 + nn;
+  zd* M2 = X1 + nn;
+  zd* X2 = M2 + nn;
+  __shared__ int shi[2];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < nn; e += nt) { Ss[e] = S[e]; Cs[e] = C[e]; }
+  for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+    zd* bs = vs + b * L.bs;
+    zd* bd = vd + b * L.bs;
+    for (int e = tid; e < nn; e += nt) A[e] = bs[e * L.es];  // chi0_S
+    __syncthreads();
+    for (int e = tid; e < nn; e += nt) {
+      int p = e % n, q = e / n;
+      zd as(0, 0);
+      for (int r = 0; r < n; ++r) as += A[p + n * r] * Ss[r + n * q];
+      M1[e] = zd(p == q ? 1.0 : 0.0, 0) - as;
+      X1[e] = as;
+    }
+    __syncthreads();
+    for (int e = tid; e < nn; e += nt) A[e] = bd[e * L.es];  // chi0_C
+    __syncthreads();
+    for (int e = tid; e < nn; e += nt) {
+      int p = e % n, q = e / n;
+      zd ac(0, 0);
+      for (int r = 0; r < n; ++r) ac += A[p + n * r] * Cs[r + n * q];
+      M2[e] = zd(p == q ? 1.0 : 0.0, 0) + ac;
+      X2[e] = ac;
+    }
+    __syncthreads();
+    for (int e = tid; e < nn; e += nt) A[e] = X1[e] + X2[e];  // chi0_S S + chi0_C C
+    __syncthreads();
+    bool ok = lds_lu_solve(M1, X1, n, n, shi);
+    ok = ok && lds_lu_solve(M2, X2, n, n, shi);
+    if (!ok) {
+      if (tid == 0) atomicOr(err, 1);
+      __syncthreads();
+      continue;
+    }
+    for (int e = tid; e < nn; e += nt) {
+      int p = e % n, q = e / n;
+      zd v = zd(1.5 * Ss[e] - 0.5 * Cs[e], 0);
+      zd d = singlet ? zd(0.5 * (Ss[e] + Cs[e]), 0) : zd(0.5 * (Ss[e] - Cs[e]), 0);
+      for (int r = 0; r < n; ++r) {
+        const double sv = Ss[p + n * r], cv = Cs[p + n * r];
+        const zd xs = X1[r + n * q], xc = X2[r + n * q];
+        v += xs * (1.5 * sv) + xc * (0.5 * cv) - A[r + n * q] * (0.25 * (sv + cv));
+        d += singlet ? xs * (1.5 * sv) - xc * (0.5 * cv) : zd(0, 0) - xs * (0.5 * sv) - xc * (0.5 * cv);
+      }
+      bs[e * L.es] = v;
+      bd[e * L.es] = d;
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------- register LU
 // Column-per-lane variant for Nchi = N in {1,4,9,16,25}.  Each solve uses 2N
 // lanes: lane c < N holds column c of M = I -+ chi0 K (K = S or C), lane
@@ -691,6 +759,16 @@ void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, cons
   GMOD_CHECK_LAUNCH();
 }
 
+void sc_vertex(zd* vs, zd* vd, long batch, BatchLayout L, int n, const double* S, const double* C, bool singlet,
+               int* err, hipStream_t s) {
+  if (batch <= 0) return;
+  size_t shm = 2 * n * n * sizeof(double) + 5 * n * n * sizeof(zd);
+  check_lds(shm);
+  unsigned grid = (unsigned)std::min<long>(batch, 256L * 64);
+  hipLaunchKernelGGL(k_sc_vertex, dim3(grid), dim3(64), shm, s, vs, vd, L, batch, n, S, C, singlet ? 1 : 0, err);
+  GMOD_CHECK_LAUNCH();
+}
+
 // ---------------------------------------------------------------- spinful vertex
 // V_sigma = -V + V X with X = (I + chi0 V)^-1 chi0 V per (q, i nu) (fUJ.f90:367-399),
 // chi0 read with the 1e-9 threshold of the loop applied (thr > 0).  One
@@ -1226,6 +1304,75 @@ __global__ void __launch_bounds__(128) k_anom_f(zd* F, const zd* G, const zd* D,
   }
 }
 
+// one step of the superconducting Dyson equations per (k, i w)
+// (ffeliash.f90:315-384): G <- G0 (I + D F^+) with the previous F, then
+// F <- -G0 D G^+ with the new G, diagonal real and upper = conj(lower)
+template <int NO>
+__global__ void __launch_bounds__(128) k_sc_green(zd* G, zd* F, const zd* G0, const zd* D, long Nk, int Nw) {
+  const long tot = Nk * (long)Nw;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+    zd g0[NO][NO], d[NO][NO], t[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        g0[p][q] = G0[e + tot * (p + NO * q)];
+        d[p][q] = D[e + tot * (p + NO * q)];
+      }
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        zd acc(p == q ? 1.0 : 0.0, 0);
+#pragma unroll
+        for (int r = 0; r < NO; ++r) acc += d[p][r] * conj(F[e + tot * (q + NO * r)]);
+        t[p][q] = acc;  // I + D F^+
+      }
+    zd g[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        zd acc(0, 0);
+#pragma unroll
+        for (int r = 0; r < NO; ++r) acc += g0[p][r] * t[r][q];
+        g[p][q] = acc;
+      }
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        zd acc(0, 0);
+#pragma unroll
+        for (int r = 0; r < NO; ++r) acc += d[p][r] * conj(g[q][r]);
+        t[p][q] = acc;  // D G^+
+      }
+    zd f[NO][NO];
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        zd acc(0, 0);
+#pragma unroll
+        for (int r = 0; r < NO; ++r) acc += g0[p][r] * t[r][q];
+        f[p][q] = zd(0, 0) - acc;
+      }
+#pragma unroll
+    for (int l = 0; l < NO; ++l) {
+      f[l][l].im = 0.0;
+#pragma unroll
+      for (int m = l + 1; m < NO; ++m) f[l][m] = conj(f[m][l]);
+    }
+#pragma unroll
+    for (int p = 0; p < NO; ++p)
+#pragma unroll
+      for (int q = 0; q < NO; ++q) {
+        G[e + tot * (p + NO * q)] = g[p][q];
+        F[e + tot * (p + NO * q)] = f[p][q];
+      }
+  }
+}
+
 __global__ void k_hermitize_lower(zd* D, long tot, int No) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
     for (int l = 0; l < No; ++l) {
@@ -1327,6 +1474,12 @@ void green0(zd* G, const double* eig, const zd* uni, double mu, double temp, lon
 void anomalous_f(zd* F, const zd* G, const zd* D, long Nk, int Nw, int No, hipStream_t s) {
   long tot = Nk * (long)Nw;
   LAUNCH_NO(No, k_anom_f, dim3(nblk(tot, 128)), dim3(128), F, G, D, Nk, Nw);
+  GMOD_CHECK_LAUNCH();
+}
+
+void sc_green_update(zd* G, zd* F, const zd* G0, const zd* D, long Nk, int Nw, int No, hipStream_t s) {
+  long tot = Nk * (long)Nw;
+  LAUNCH_NO(No, k_sc_green, dim3(nblk(tot, 128)), dim3(128), G, F, G0, D, Nk, Nw);
   GMOD_CHECK_LAUNCH();
 }
 

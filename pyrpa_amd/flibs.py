@@ -57,6 +57,13 @@ _SIGS = {
     "get_initial_delta_": [_z, _d, _z, _i, _i, _pi, _pi, _pi, _pi, _pi],
     "conv_delta_orb_to_band": [_z, _z, _z, _d, _i, _pi, _pi, _pi, _pi, _pi],
     "remap_delta_": [_z, _z, _d, _i, _pi, _pi, _pi, _pi, _pi],
+    # libs/flibs/_eliashberg.py:230-304 (nonlinear FLEX-Eliashberg) and the
+    # loop's internal bubbles / vertices (ffeliash.f90:386-489)
+    "eliashberg": [_z, _z, _z, _z, _d, _d, _i, _d, _i, _i, _pd, _pd, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi,
+                   _pi, _pi, _pi, _pb, _pb, _pi, _pd, _pb],
+    "gmod_eliashberg_stats": [_d, c_int],
+    "get_chi0sc_": [_z, _z, _z, _z, _i, _i, _i32, _i32, _i, _d, _pd, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pi, _pb],
+    "mkv_flex_nosoc_": [_z, _z, _d, _d, _pi, _pi, _pi, _pb],
     # libs/flibs/_interaction.py:7-45
     "get_scmat": [_d, _d, _i, _i, _pd, _pd, _pi],
     "get_scmat_orb": [_d, _d, _i, _i, _d, _d, _pi, _pi],
@@ -615,6 +622,62 @@ def linearized_eliashberg(chi, Gk, uni, init_delta, Smat, Cmat, olist, plist, km
           np.asarray(plist, dtype=np.float64), kmap, invk, dbl(temp), dbl(eps), i64(Nkall), i64(Nk), i64(Nw),
           i64(Nchi), i64(Norb), i64(Nx), i64(Ny), i64(Nz), i64(itemax), i64(gap_sym), i64(arnoldi_m), byref(lam))
     return delta, lam.value
+
+
+def nonlinear_eliashberg(delta_init, Gk, hamk, Smat, Cmat, olist, plist, kmap, invk, mu, temp, gap_sym, Nx, Ny, Nz,
+                         sw_sigma=False, sw_Vconst=False, eps=1.0e-4, itemax=100, m_diis=5, gap_min=1.0e-2,
+                         sw_amp_newton=True):
+    """(delta[No,No,Nw,Nk], sigmak[No,No,Nw,Nk])  [_eliashberg.py:230-304].
+    delta_init is updated in place when it is C-contiguous complex128 (else a
+    copy is returned); Gk, when C-contiguous complex128, leaves holding the last
+    normal G, as the reference's intent(inout) argument does."""
+    if delta_init.dtype != np.complex128 or not delta_init.flags['C_CONTIGUOUS']:
+        delta = np.ascontiguousarray(delta_init, dtype=np.complex128)
+    else:
+        delta = delta_init
+    Norb, _, Nw, Nk = delta.shape
+    Nkall, Nchi = len(kmap), len(Smat)
+    sigmak = np.zeros((Norb, Norb, Nw, Nk), dtype=np.complex128)
+    G = Gk if (Gk.dtype == np.complex128 and Gk.flags['C_CONTIGUOUS']) else np.ascontiguousarray(Gk, np.complex128)
+    _call("eliashberg", delta, sigmak, G, np.ascontiguousarray(hamk), Smat, Cmat, olist,
+          np.asarray(plist, dtype=np.float64), kmap, invk, dbl(mu), dbl(temp), dbl(eps), i64(Nkall), i64(Nk), i64(Nw),
+          i64(Nchi), i64(Norb), i64(Nx), i64(Ny), i64(Nz), i64(itemax), i64(gap_sym), byref(c_bool(sw_sigma)),
+          byref(c_bool(sw_Vconst)), i64(m_diis), dbl(gap_min), byref(c_bool(sw_amp_newton)))
+    return delta, sigmak
+
+
+def eliashberg_stats():
+    """Of the last nonlinear_eliashberg call: iterations, status ("itemax",
+    "converged", "collapsed"), re-anchors of the amplitude phase, last relative
+    change max|D_new - D| / max|D|."""
+    out = np.zeros(4)
+    _L().gmod_eliashberg_stats(out, 4)
+    return dict(iterations=int(out[0]), status=("itemax", "converged", "collapsed")[int(out[1])],
+                anchors=int(out[2]), rel_change=float(out[3]))
+
+
+def get_chi0sc(Gk, fk, kmap, invk, olist, plist, temp, Nx, Ny, Nz, gap_sym):
+    """(chi0_S, chi0_C)[Nchi,Nchi,Nw,Nk] = chi^GG +- chi^FF of the
+    superconducting loop (ffeliash.f90:459-489)."""
+    Nkall, Nk = len(kmap), Gk.shape[3]
+    Norb, Nchi, Nw = len(Gk), len(olist), Gk.shape[2]
+    cs = np.zeros((Nchi, Nchi, Nw, Nk), dtype=np.complex128)
+    cc = np.zeros_like(cs)
+    chi_map, irr_chi = _get_chi_map(olist)
+    _call("get_chi0sc_", cs, cc, np.ascontiguousarray(Gk), np.ascontiguousarray(fk), kmap, invk, irr_chi, chi_map,
+          olist, np.asarray(plist, dtype=np.float64), dbl(temp), i64(Nx), i64(Ny), i64(Nz), i64(Nw), i64(Nk),
+          i64(Nkall), i64(Norb), i64(Nchi), byref(c_bool(gap_sym >= 0)))
+    return cs, cc
+
+
+def mkV_flex_nosoc(chiS, chiC, Smat, Cmat, gap_sym):
+    """(V_sigma, V_Delta) of the superconducting loop from chi0_S, chi0_C
+    (ffeliash.f90:386-457); inputs are not modified."""
+    Nchi, _, Nw, Nk = chiS.shape
+    vs = np.array(chiS, dtype=np.complex128, order='C')
+    vd = np.array(chiC, dtype=np.complex128, order='C')
+    _call("mkv_flex_nosoc_", vd, vs, Smat, Cmat, i64(Nk), i64(Nw), i64(Nchi), byref(c_bool(gap_sym >= 0)))
+    return vs, vd
 
 
 def gen_Fk(Gk, delta, invk):

@@ -35,7 +35,8 @@ def test_header_declares_the_hot_path_entry_points():
                  "get_scmat", "get_scmat_orb", "generate_irr_kpoint_inv", "gen_ham", "get_eig", "get_ffermi",
                  "openmp_params", "gmod_last_error", "gmod_get_chis_chic_all", "gmod_nn_create",
                  "gmod_nn_train_epoch", "mkself_soc", "get_chi0_soc", "get_chis_chic_soc", "get_vmat_soc",
-                 "get_vmat_soc_orb", "get_chi_irr", "chiq_map", "pade_with_trace", "gmod_wannier_r"]:
+                 "get_vmat_soc_orb", "get_chi_irr", "chiq_map", "pade_with_trace", "gmod_wannier_r",
+                 "eliashberg", "get_chi0sc_", "mkv_flex_nosoc_"]:
         assert name in syms
 
 
