@@ -140,3 +140,60 @@ def test_config_eliashberg_one_orbital_128x128_2048(F):
         anti = np.abs(g + g[p]).max() / np.abs(g).max()
         print(f"[C4] precision {prec}: max|Delta(kx,ky) + Delta(ky,kx)| / max|Delta| = {anti:.2e}")
         assert anti < 1e-4
+    # the nonlinear loop as libs/plibs/_calc.py:494-498 drives it: the linearized
+    # gap scaled to the BCS amplitude 1.764 T, the vertex held constant, the
+    # reference's defaults (eps 1e-4, DIIS 5, amplitude phase).  The loop builds
+    # its vertex from the sharp-cutoff chi0 (ffeliash.f90:131-137) while the
+    # gate above used the tail-corrected one, so whether a gap survives follows
+    # the sharp-cutoff eigenvalue
+    chs, _ = F.get_chi0(st['Smat'], st['Cmat'], G, st['olist'], st['kmap'], st['invk'], st['temp'], 128, 128, 1)
+    _, l_sharp = F.linearized_eliashberg(chs, *args, eps=1e-7, itemax=400, arnoldi_m=0)
+    del chs
+    gap0 = res[0][0] * (1.764 * st['temp'] / np.abs(res[0][0]).max())
+    t0 = time.perf_counter()
+    d, _ = F.nonlinear_eliashberg(gap0.copy(), np.array(G), st['hamk'], st['Smat'], st['Cmat'], st['olist'],
+                                  st['plist'], st['kmap'], st['invk'], mu, st['temp'], 1, 128, 128, 1,
+                                  sw_sigma=False, sw_Vconst=True, eps=1e-4, itemax=300)
+    stt = F.eliashberg_stats()
+    print(f"[C4] lambda with the sharp-cutoff chi0 {l_sharp:.10f}; nonlinear Eliashberg: {stt['status']} after "
+          f"{stt['iterations']} iterations, {time.perf_counter() - t0:.2f} s, max|Delta| {np.abs(d).max():.3e}")
+    assert stt['status'] == ('converged' if l_sharp > 1.0 else 'collapsed')
+
+
+def test_nonlinear_eliashberg_one_orbital_64x64_1024_below_tc(F):
+    """The nonlinear loop converged at the 64x64 x 1024 mesh on the bare
+    one-orbital G (the square lattice with t' = -0.25 at T = 0.02, U = 1.8 put
+    the sharp-cutoff lambda above 1): FP64 and FP32 convolutions reach the same
+    d-wave gap."""
+    from pyrpa_amd import lattice as L
+    st = L.config(1, U=1.8)
+    G = F.gen_Green0(st['eig'], st['uni'], st['mu'], st['temp'], st['Nw'])
+    chi, stoner = F.get_chi0(st['Smat'], st['Cmat'], G, st['olist'], st['kmap'], st['invk'], st['temp'], 64, 64, 1)
+    init = L.gap_symms(st['klist'], 1, 1)
+    args = (G, st['uni'], init, st['Smat'], st['Cmat'], st['olist'], st['plist'], st['kmap'], st['invk'], 64, 64, 1,
+            st['temp'], 1)
+    gap, lam = F.linearized_eliashberg(chi, *args, eps=1e-8, itemax=600, arnoldi_m=0)
+    print(f"\n[nonlinear 64x64x1024] Stoner {stoner:.6f}, lambda {lam:.8f}")
+    assert stoner < 1.0 and lam > 1.0
+    gap0 = gap * (1.764 * st['temp'] / np.abs(gap).max())
+    out = {}
+    for prec in (0, 1):
+        try:
+            F.set_precision(prec)
+            t0 = time.perf_counter()
+            d, _ = F.nonlinear_eliashberg(gap0.copy(), np.array(G), st['hamk'], st['Smat'], st['Cmat'], st['olist'],
+                                          st['plist'], st['kmap'], st['invk'], st['mu'], st['temp'], 1, 64, 64, 1,
+                                          sw_sigma=False, sw_Vconst=True, eps=1e-6, itemax=600)
+            out[prec] = (d, F.eliashberg_stats(), time.perf_counter() - t0)
+        finally:
+            F.set_precision(0)
+        d, stt, tt = out[prec]
+        print(f"[nonlinear 64x64x1024] precision {prec}: {stt['status']} after {stt['iterations']} iterations "
+              f"({stt['anchors']} re-anchors), {tt:.2f} s, max|Delta| {np.abs(d).max():.6f}")
+        assert stt['status'] == 'converged'
+    p = swap_partner(st['klist'], 64)
+    g = out[0][0][0, 0, 0]
+    assert np.abs(g + g[p]).max() / np.abs(g).max() < 1e-6
+    e = float(np.abs(out[1][0] - out[0][0]).max() / np.abs(out[0][0]).max())
+    print(f"[nonlinear 64x64x1024] FP32 convolutions vs FP64: max|d Delta|/max|Delta| {e:.2e}")
+    assert e < 1e-3
