@@ -476,6 +476,63 @@ __device__ __forceinline__ long long gj_row_max(long long k) {
   return gj_row_max_step<1>(k);
 }
 
+// Gauss-Jordan inversion of M (row r in lane r of a 16-lane group, in
+// mrow) in place with partial pivoting and no row exchanges.  Column k of the
+// row storage turns into a column of the inverse once column k is
+// eliminated.  Rows stay unnormalised: the pivot lane publishes its row as is
+// (prow, the group's LDS slot) and keeps 1 in the pivot column, the others
+// form the multiplier from the published pivot, and each lane divides its row
+// by its own pivot once at the end.  A lane's candidate for column k + 1 is
+// formed right after column k + 1 of its row is updated.  At the end the lane
+// that was pivot of column k holds row k of M^-1 (xrow = k), entry j in
+// column piv[j] (the pivot row of column j).  Returns true if a pivot was
+// zero or not finite.
+template <int N>
+__device__ __forceinline__ bool gj_invert(zd (&mrow)[N], int r, zd* prow, int* piv, int& xrow) {
+  bool used = false;
+  bool singular = false;
+  zd dpiv(1.0, 0);
+  long long key = gj_key(abs1(mrow[0]), r);  // my candidate for the next column
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    wave_sync();
+    const long long kb = gj_row_max(key);
+    const int pr = 15 - (int)(kb & 0xF);
+    const double best = __longlong_as_double(kb & ~0xFLL);
+    singular = singular || !(best > 0.0 && best < 1e300);  // zero, or a clamped inf / NaN
+    if (r == pr) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) prow[j] = mrow[j];
+      piv[k] = r;
+      dpiv = mrow[k];
+      mrow[k] = zd(1.0, 0);
+      used = true;
+      xrow = k;
+    }
+    wave_sync();
+    if (r != pr) {
+      const zd f = mrow[k] * crecip(prow[k]);
+      if (k + 1 < N) {
+        csub_mul(mrow[k + 1], f, prow[k + 1]);
+        key = gj_key(used ? -1.0 : abs1(mrow[k + 1]), r);
+      }
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        if (j != k && j != k + 1) csub_mul(mrow[j], f, prow[j]);
+      mrow[k] = zd(0, 0) - f;
+    } else if (k + 1 < N) {
+      key = gj_key(-1.0, r);
+    }
+  }
+  {
+    const zd inv = crecip(dpiv);
+#pragma unroll
+    for (int j = 0; j < N; ++j) mrow[j] = mrow[j] * inv;
+  }
+  wave_sync();
+  return singular;
+}
+
 constexpr int GJ_WAVES = 4;
 constexpr int GJ_MAT = 2 * GJ_WAVES;  // matrices per block
 
@@ -586,15 +643,8 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
       if (j == r) mrow[j] += zd(1.0, 0);
     }
     __syncthreads();  // all reads of A are complete before X overwrites it
-    // ---- Gauss-Jordan inversion in place with partial pivoting and no row
-    // exchanges.  Column k of the row storage turns into a column of the
-    // inverse once column k is eliminated.  Rows stay unnormalised: the
-    // pivot lane publishes its row as is and keeps 1 in the pivot column,
-    // the others form the multiplier from the published pivot, and each
-    // lane divides its row by its own pivot once at the end.  A lane's
-    // candidate for column k + 1 is formed right after column k + 1 of its
-    // row is updated.  At the end the lane that was pivot of column k holds row
-    // k of M^-1, entry j in column piv[j] (the pivot row of column j).
+    // ---- Gauss-Jordan inversion (the steps of gj_invert, kept inline here:
+    // the headline kernel's register allocation is tuned on this form)
     bool used = false;
     int xrow = 0;  // the row of M^-1 this lane holds at the end
     bool singular = false;
@@ -673,6 +723,166 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
         const long o = b + (long)e * es;
         chi_out[o] = v;
         if (dv) dv[o] = cpx<float>(v - vprev[o]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// The vertices of the superconducting loop (ffeliash.f90:386-457) for
+// Nchi = 16 with sparse S, C: the Gauss-Jordan scheme of k_vertex_gj with the
+// S system solved on chi0_S and the C system on chi0_C (read from vs and vd),
+// V_sigma = 3/2 S - 1/2 C + 3/2 S chi_s + 1/2 C chi_c - 1/4 W (chi0_S S +
+// chi0_C C) written to vs and V_Delta (singlet: (S+C)/2 + 3/2 S chi_s - 1/2 C
+// chi_c; triplet: (S-C)/2 - 1/2 S chi_s - 1/2 C chi_c) to vd.  Two waves and
+// four matrices per work-group: chi0_S (later X), chi0_C (later V_Delta) and
+// V_sigma take three 16 KB LDS buffers.
+constexpr int SCGJ_WAVES = 2;
+constexpr int SCGJ_MAT = 2 * SCGJ_WAVES;
+
+template <int N>
+__global__ void __launch_bounds__(64 * SCGJ_WAVES)
+k_vertex_gj_sc(zd* vs, zd* vd, long batch, long es, const VSparse* __restrict__ spg, const double* __restrict__ Sd,
+               const double* __restrict__ Cd, int singlet, int* err) {
+  static_assert(N == 16, "a 16-lane group per system");
+  constexpr int NN = N * N;
+  constexpr int NT = 64 * SCGJ_WAVES;
+  constexpr int PER = SCGJ_MAT * NN / NT;
+  __shared__ VSparse sp;
+  __shared__ double2 Abuf_[SCGJ_MAT * NN];  // chi0_S, later X_s then X_c
+  __shared__ double2 Cbuf_[SCGJ_MAT * NN];  // chi0_C, then chi0_S S + chi0_C C, then V_Delta
+  __shared__ double2 Vbuf_[SCGJ_MAT * NN];  // V_sigma
+  constexpr int PROW = 2 * N + 1;
+  __shared__ double2 Prow_[SCGJ_WAVES * 4 * PROW];
+  __shared__ int Piv_[SCGJ_WAVES * 4 * N];
+  zd* Abuf = reinterpret_cast<zd*>(Abuf_);
+  zd* Cbuf = reinterpret_cast<zd*>(Cbuf_);
+  zd* Vbuf = reinterpret_cast<zd*>(Vbuf_);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  {
+    const int* src = reinterpret_cast<const int*>(spg);
+    int* dst = reinterpret_cast<int*>(&sp);
+    for (int i = tid; i < (int)(sizeof(VSparse) / sizeof(int)); i += NT) dst[i] = src[i];
+  }
+  const int g = lane >> 4, r = lane & 15;
+  const int mloc = wave * 2 + (g >> 1);
+  const int sys = g & 1;  // 0: S system on chi0_S, 1: C system on chi0_C
+  zd* Am = (sys == 0 ? Abuf : Cbuf) + mloc * NN;
+  zd* prow = reinterpret_cast<zd*>(Prow_) + (wave * 4 + g) * PROW;
+  int* piv = Piv_ + (wave * 4 + g) * N;
+  const double d1 = singlet ? 1.5 : -0.5;  // V_Delta: coefficient of S X_s
+  const double d2 = -0.5;                  // and of C X_c
+  const long bstep = (long)gridDim.x * SCGJ_MAT;
+  zd nxs[PER], nxc[PER];
+  auto fetch = [&](long b0) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * NT;
+      const int m = i % SCGJ_MAT, e = i / SCGJ_MAT;
+      const long b = b0 + m;
+      const long o = b < batch ? b + (long)e * es : 0;
+      nxs[u] = vs[o];
+      nxc[u] = vd[o];
+    }
+  };
+  __syncthreads();
+  // (a prefetch of the next batch across the loop, as k_vertex_gj does, puts
+  // this kernel's state in scratch memory: the loads are issued per batch)
+  for (long b0 = (long)blockIdx.x * SCGJ_MAT; b0 < batch; b0 += bstep) {
+    fetch(b0);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * NT;
+      const int m = i % SCGJ_MAT, e = i / SCGJ_MAT;
+      const bool in = b0 + m < batch;
+      Abuf[m * NN + e] = in ? nxs[u] : zd(0, 0);
+      Cbuf[m * NN + e] = in ? nxc[u] : zd(0, 0);
+    }
+    __syncthreads();
+    // ---- my row of M = I - chi0_S S (S system) or I + chi0_C C (C system)
+    zd mrow[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      zd acc(0, 0);
+#pragma unroll
+      for (int t = 0; t < VW_NZ; ++t) acc += Am[r + N * sp.idx[sys][j][t]] * sp.val[sys][j][t];
+      mrow[j] = sys == 0 ? zd(0, 0) - acc : acc;
+      if (j == r) mrow[j] += zd(1.0, 0);
+    }
+    // ---- T = chi0_S S + chi0_C C, block-wide
+    zd tt[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * NT;
+      const int m = i / NN, e = i % NN, p = e % N, q = e / N;
+      zd acc(0, 0);
+#pragma unroll
+      for (int t = 0; t < VW_NZ; ++t) {
+        acc += Abuf[m * NN + p + N * sp.idx[0][q][t]] * sp.val[0][q][t];
+        acc += Cbuf[m * NN + p + N * sp.idx[1][q][t]] * sp.val[1][q][t];
+      }
+      tt[u] = acc;
+    }
+    __syncthreads();  // chi0_S and chi0_C are read
+#pragma unroll
+    for (int u = 0; u < PER; ++u) Cbuf[tid + u * NT] = tt[u];
+    __syncthreads();
+    // ---- V_sigma = st - 1/4 W T; V_Delta = its static part
+    zd vsig[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * NT;
+      const int m = i / NN, e = i % NN, p = e % N, q = e / N;
+      zd acc(0, 0);
+#pragma unroll
+      for (int t = 0; t < VW_NZ; ++t) acc += Cbuf[m * NN + sp.idx[2][p][t] + N * q] * sp.val[2][p][t];
+      vsig[u] = zd(1.5 * Sd[e] - 0.5 * Cd[e], 0) - acc * 0.25;
+    }
+    __syncthreads();  // every read of T is complete
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = tid + u * NT;
+      const int e = i % NN;
+      Vbuf[i] = vsig[u];
+      Cbuf[i] = singlet ? zd(0.5 * (Sd[e] + Cd[e]), 0) : zd(0.5 * (Sd[e] - Cd[e]), 0);
+    }
+    int xrow = 0;
+    const bool singular = gj_invert<N>(mrow, r, prow, piv, xrow);
+    if (singular && r == 0 && b0 + mloc < batch) atomicOr(err, 1);
+    // ---- X_s then X_c through my matrix's A slot; both vertices accumulate
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      if (sys == pass) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const int q = piv[j];
+          zd x = mrow[j];
+          if (q == xrow) x -= zd(1.0, 0);
+          Abuf[mloc * NN + xrow + N * q] = sys == 0 ? x : zd(0, 0) - x;
+        }
+      }
+      __syncthreads();
+      const double cs = pass == 0 ? 1.5 : 0.5;
+      const double cd = pass == 0 ? d1 : d2;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int i = tid + u * NT;
+        const int m = i / NN, e = i % NN, p = e % N, q = e / N;
+        zd acc(0, 0);
+#pragma unroll
+        for (int t = 0; t < VW_NZ; ++t) acc += Abuf[m * NN + sp.idx[pass][p][t] + N * q] * sp.val[pass][p][t];
+        Vbuf[i] += acc * cs;
+        Cbuf[i] += acc * cd;
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < SCGJ_MAT * NN; i += NT) {
+      const int m = i % SCGJ_MAT, e = i / SCGJ_MAT;
+      const long b = b0 + m;
+      if (b < batch) {
+        const long o = b + (long)e * es;
+        vs[o] = Vbuf[m * NN + e];
+        vd[o] = Cbuf[m * NN + e];
       }
     }
     __syncthreads();
@@ -762,6 +972,18 @@ void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, cons
 void sc_vertex(zd* vs, zd* vd, long batch, BatchLayout L, int n, const double* S, const double* C, bool singlet,
                int* err, hipStream_t s) {
   if (batch <= 0) return;
+  VSparse sp;
+  if (L.bs == 1 && n == 16 && vertex_sparse(S, C, n, sp, s)) {
+    DBuf<VSparse> spd(1);
+    spd.upload(&sp, 1, s);
+    const long nb = (batch + SCGJ_MAT - 1) / SCGJ_MAT;
+    const unsigned grid = (unsigned)std::min<long>(nb, 256L * 48);
+    hipLaunchKernelGGL(k_vertex_gj_sc<16>, dim3(grid), dim3(64 * SCGJ_WAVES), 0, s, vs, vd, batch, L.es, spd.p, S, C,
+                       singlet ? 1 : 0, err);
+    GMOD_CHECK_LAUNCH();
+    GMOD_HIP(hipStreamSynchronize(s));  // spd is released at return
+    return;
+  }
   size_t shm = 2 * n * n * sizeof(double) + 5 * n * n * sizeof(zd);
   check_lds(shm);
   unsigned grid = (unsigned)std::min<long>(batch, 256L * 64);

@@ -39,6 +39,8 @@ MODELS = {
     "1orb_6x6": lambda: MD.one_orbital(Nx=6, Ny=6, Nw=8, temp=0.07, mu=-0.2, tx=1.0, ty=1.0, U=2.0),
     "2orb_4x4": lambda: MD.two_orbital(Nx=4, Ny=4, Nw=8, temp=0.08, mu=0.1, U=1.2, J=0.2),
     "3orb_4x4x2": lambda: MD.random_model(3, 4, 4, 2, 6, 0.1, 0.0, 0.8, 0.1, seed=7, real=True),
+    # Nchi = 16 with the sparse Kanamori S, C: the Gauss-Jordan vertex kernel
+    "4orb_2x2x2": lambda: MD.random_model(4, 2, 2, 2, 4, 0.1, 0.0, 1.0, 0.2, seed=5, real=True),
 }
 
 
@@ -54,7 +56,7 @@ def test_chi0sc_matches_oracle(F, name, gs):
     assert rel(cs, cc) > 1e-6  # the anomalous bubble is not negligible here
 
 
-@pytest.mark.parametrize("name", ["2orb_4x4", "3orb_4x4x2"])
+@pytest.mark.parametrize("name", ["2orb_4x4", "3orb_4x4x2", "4orb_2x2x2"])
 @pytest.mark.parametrize("gs", [0, -1])
 def test_sc_vertices_match_oracle(F, name, gs):
     st = MODELS[name]()
