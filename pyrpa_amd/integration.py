@@ -18,9 +18,17 @@ import os
 
 # the libfmod.so symbols libgmod.so replaces (include/gmod.h)
 HOT = frozenset((
+    # FLEX, chi0 and the linearized gap equation
     "mkself", "get_chi0", "get_chi0_tail", "get_chi0_conv_", "get_vsigma_flex_nosoc_", "get_chis", "get_chis_chic",
     "get_eig_or_tr_chi", "gen_green0_", "gen_green_inv_", "getinv_", "lin_eliash", "mkfk_trs_nsoc_",
     "get_initial_delta_", "conv_delta_orb_to_band", "remap_delta_",
+    # spinful basis
+    "mkself_soc", "get_chi0_soc", "get_chis_chic_soc", "lin_eliash_soc", "mkfk_trs_soc_", "conv_delta_orb_to_band_soc",
+    "get_vmat_soc", "get_vmat_soc_orb",
+    # real-frequency maps and Pade continuation
+    "get_chi_irr", "chiq_map", "get_tr_chi", "get_qshift_", "get_a_", "get_qp_", "pade_with_trace",
+    # nonlinear FLEX-Eliashberg
+    "eliashberg",
 ))
 
 
