@@ -266,10 +266,12 @@ void generate_irr_kpoint_inv(double* klist, int64_t* kmap, int64_t* invk, const 
                              const int64_t* Ny, const int64_t* Nz);
 /* libs/src/finv.f90:292-305 [libs/flibs/_hamiltonian.py:384-395] */
 void get_kweight(double* weight, const int64_t* invk, const int64_t* Nk, const int64_t* Nkall);
-/* libs/src/fmod.f90:162-207 [libs/flibs/_hamiltonian.py:134-158] */
+/* libs/src/fmod.f90:162-207 [libs/flibs/_hamiltonian.py:134-158]; formed on the device */
 void gen_ham(gmod_z* ham_k, const double* klist, const gmod_z* ham_r, const double* rvec, const int64_t* Nk,
              const int64_t* Nr, const int64_t* Norb);
-/* libs/src/fmod.f90:209-248 [libs/flibs/_hamiltonian.py:160-184] */
+/* libs/src/fmod.f90:209-248 [libs/flibs/_hamiltonian.py:160-184]; batched Hermitian
+ * Jacobi on the device (eigenvalues ascending as zheev; eigenvector columns up to a
+ * phase), Norb <= 64 */
 void get_eig(double* eig, gmod_z* uni, const gmod_z* ham_k, const int64_t* Nk, const int64_t* Norb);
 /* libs/src/fmod.f90:312-340 [libs/flibs/_hamiltonian.py:186-200] */
 void get_ffermi(double* ffermi, const double* eig, const double* mu, const double* temp, const int64_t* Nk,

@@ -1,6 +1,7 @@
 // Library runtime: device context, caching pool, sticky error channel,
 // hipEvent kernel profiling, and the host-side setup entry points of the
-// reference (k-mesh, H(k), band eigenvectors, interaction vertices).
+// reference (k-mesh, Fermi factors, interaction vertices).  H(k) and the band
+// eigenvectors are formed on the device (setup.hip).
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -346,111 +347,6 @@ extern "C" void get_kweight(double* w, const int64_t* invk, const int64_t* Nk, c
     long k = invk[3 * i] - 1;
     if (k >= 0 && k < *Nk) w[k] += 1.0;
   }
-}
-
-extern "C" void gen_ham(gmod_z* ham_k, const double* klist, const gmod_z* ham_r, const double* rvec,
-                        const int64_t* Nk_, const int64_t* Nr_, const int64_t* Norb_) {
-  const long Nk = *Nk_, Nr = *Nr_, No = *Norb_;
-  const double pi = 3.141592653589793238462643383279;
-  for (long i = 0; i < Nk; ++i) {
-    gmod_z* H = ham_k + No * No * i;
-    for (long e = 0; e < No * No; ++e) H[e] = {0, 0};
-    for (long j = 0; j < Nr; ++j) {
-      double ph = 2 * pi * (klist[3 * i] * rvec[3 * j] + klist[3 * i + 1] * rvec[3 * j + 1] +
-                            klist[3 * i + 2] * rvec[3 * j + 2]);
-      double c = std::cos(ph), s = -std::sin(ph);
-      for (long l = 0; l < No; ++l)
-        for (long m = l; m < No; ++m) {
-          gmod_z h = ham_r[m + No * (l + No * j)];
-          H[m + No * l].re += h.re * c - h.im * s;
-          H[m + No * l].im += h.re * s + h.im * c;
-        }
-    }
-    for (long l = 0; l < No; ++l) {
-      H[l + No * l].im = 0;
-      for (long m = l + 1; m < No; ++m) H[l + No * m] = {H[m + No * l].re, -H[m + No * l].im};
-    }
-  }
-}
-
-// cyclic complex Jacobi for one Hermitian matrix (eigenvalues ascending,
-// eigenvectors in columns) -- same mathematical output as zheev('V','U')
-// (eigenvector phases may differ; every hot-path consumer uses phase-free
-// combinations U_an conj(U_bn)).
-static void herm_jacobi(std::vector<std::complex<double>>& A, int n, std::vector<double>& w,
-                        std::vector<std::complex<double>>& V) {
-  using C = std::complex<double>;
-  V.assign(n * n, C(0));
-  for (int i = 0; i < n; ++i) V[i + n * i] = 1;
-  // use the upper triangle, as zheev('U')
-  for (int j = 0; j < n; ++j)
-    for (int i = j + 1; i < n; ++i) A[i + n * j] = std::conj(A[j + n * i]);
-  for (int i = 0; i < n; ++i) A[i + n * i] = A[i + n * i].real();
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0, tot = 0;
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) {
-        double a = std::norm(A[i + n * j]);
-        tot += a;
-        if (i != j) off += a;
-      }
-    if (off <= 1e-32 * tot || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        C apq = A[p + n * q];
-        double mag = std::abs(apq);
-        if (mag < 1e-300) continue;
-        double app = A[p + n * p].real(), aqq = A[q + n * q].real();
-        C ph = apq / mag;  // A_pq = |A_pq| e^{i phi}
-        double theta = 0.5 * std::atan2(2 * mag, aqq - app);
-        double c = std::cos(theta), s = std::sin(theta);
-        // rotation J: columns p,q of identity replaced by [c, -s conj(ph)... ]
-        // J = [[c, s*ph], [-s*conj(ph), c]] acting on (p,q)
-        for (int k = 0; k < n; ++k) {  // A <- A J
-          C akp = A[k + n * p], akq = A[k + n * q];
-          A[k + n * p] = c * akp - s * std::conj(ph) * akq;
-          A[k + n * q] = s * ph * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {  // A <- J^H A
-          C apk = A[p + n * k], aqk = A[q + n * k];
-          A[p + n * k] = c * apk - s * ph * aqk;
-          A[q + n * k] = s * std::conj(ph) * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          C vkp = V[k + n * p], vkq = V[k + n * q];
-          V[k + n * p] = c * vkp - s * std::conj(ph) * vkq;
-          V[k + n * q] = s * ph * vkp + c * vkq;
-        }
-        A[p + n * q] = A[q + n * p] = 0;
-      }
-  }
-  std::vector<int> idx(n);
-  for (int i = 0; i < n; ++i) idx[i] = i;
-  std::vector<double> d(n);
-  for (int i = 0; i < n; ++i) d[i] = A[i + n * i].real();
-  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
-  w.resize(n);
-  std::vector<C> V2(n * n);
-  for (int c = 0; c < n; ++c) {
-    w[c] = d[idx[c]];
-    for (int r = 0; r < n; ++r) V2[r + n * c] = V[r + n * idx[c]];
-  }
-  V = V2;
-}
-
-extern "C" void get_eig(double* eig, gmod_z* uni, const gmod_z* ham_k, const int64_t* Nk_, const int64_t* Norb_) {
-  try {
-    const long Nk = *Nk_;
-    const int No = (int)*Norb_;
-    std::vector<std::complex<double>> A(No * No), V;
-    std::vector<double> w;
-    for (long i = 0; i < Nk; ++i) {
-      for (int e = 0; e < No * No; ++e) A[e] = {ham_k[No * No * i + e].re, ham_k[No * No * i + e].im};
-      herm_jacobi(A, No, w, V);
-      for (int n = 0; n < No; ++n) eig[No * i + n] = w[n];
-      for (int e = 0; e < No * No; ++e) uni[No * No * i + e] = {V[e].real(), V[e].imag()};
-    }
-  } catch (std::exception& e) { set_error(e.what()); }
 }
 
 extern "C" void get_ffermi(double* ffermi, const double* eig, const double* mu, const double* temp,

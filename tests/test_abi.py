@@ -78,14 +78,7 @@ def test_irreducible_mesh_full_size_consistent(run):
 def test_setup_host_entry_points_match_oracle(run):
     import pyrpa_amd.flibs as F
     st = MD.random_model(4, 4, 4, 2, 4, 0.1, 0.0, 1.0, 0.2, seed=1)
-    hk = F.gen_ham(st['klist'], st['ham_r'], st['rvec'])
-    assert np.abs(hk - st['hamk']).max() < 1e-14
-    e, u = F.get_eig(hk)
-    assert np.abs(e - st['eig']).max() < 1e-12
-    # eigenvectors: phase-free reconstruction U diag(e) U^H = H
-    U = np.transpose(u, (0, 2, 1))
-    rec = np.einsum('kmn,kn,kln->kml', U, e, np.conj(U))
-    assert np.abs(rec - np.transpose(hk, (0, 2, 1))).max() < 1e-12
+    e = st['eig']  # H(k) and the band eigenpairs are formed on the device (tests/test_gpu_setup.py)
     ol, site = O.get_chi_orb_list(2, [2])
     for a, b in zip(F.gen_SCmatrix(ol, site, 1.0, 0.2), O.get_scmat(ol, site, 1.0, 0.2)):
         assert np.array_equal(a, b)

@@ -1,8 +1,9 @@
 """The product's host helpers against the oracle and the mathematics:
 olist (libs/plibs/_response.py:382-413), the initial-gap form factors
 (_response.py:415-445) and the initial chemical potential (_bands.py:50-78,
-_calc.py:298-309).  Host-only code, so each test runs in the CPU suite and
-again, through the gpu-marked case of RUN, in the GPU suite."""
+_calc.py:298-309).  The host-only helpers run in the CPU suite and again,
+through the gpu-marked case of RUN, in the GPU suite; the initial chemical
+potential needs the bands, which are formed on the device."""
 import numpy as np
 import pytest
 
@@ -56,8 +57,8 @@ def test_gap_form_factors(run):
     assert np.allclose(L.gap_symms(kl, 2, 0), 1.0)
 
 
-@RUN
-def test_initial_mu(run):
+@pytest.mark.gpu
+def test_initial_mu():
     from pyrpa_amd import lattice as L
     # particle-hole symmetric nearest-neighbour square lattice at half filling: mu = 0
     rvec, ham_r = L.square_lattice(1.0, 0.0)
