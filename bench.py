@@ -327,8 +327,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": FP32_BOX_TRAFFIC_BYTES.get(args.config) if cls == "fft_pass_fp32" else None,
-                     "kernel": f"box_fft_kernel<{'float' if cls == 'fft_pass_fp32' else 'double'}> and "
-                               f"box_accum_kernel (the FFT passes), all instantiations", "avg_launch_us": round(1e3 * avg_ms, 2),
+                     "kernel": f"box_fft_kernel<{'float' if cls == 'fft_pass_fp32' else 'double'}>, "
+                               f"box_accum_kernel (the FFT passes, all instantiations) and the Sigma increment's "
+                               f"reference-vertex contraction", "avg_launch_us": round(1e3 * avg_ms, 2),
                      "launches": int(pc["launches"]), "algorithmic_bytes_per_launch": bytes_per,
                      "bytes_model": "per pass: N b per full grid read or written (fused operands included), "
                                     "irreducible (k,w) slices at their size; average over the launches"},

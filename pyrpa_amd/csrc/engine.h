@@ -117,6 +117,10 @@ class ConvEngine {
   // after their first passes, and the device table of their operand grids
   DBuf<cpx<R>> srcs;
   DBuf<cpx<R>*> accum_tab;
+  // the grouped Sigma increment: Y of every output, then Gn = Fr + Fd of every
+  // orbital pair; ytab_d: per output and pair the reference-vertex grid
+  DBuf<cpx<R>> ysum;
+  DBuf<int> ytab_d;
   std::vector<cpx<R>*> accum_tab_h;
   DBuf<cpx<float>> dvbuf;                                  // vertex difference, k space
   // boxes of p4's forward last pass (it spans the t axis) that hold a kept

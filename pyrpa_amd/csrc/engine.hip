@@ -24,6 +24,12 @@ static bool sigma_unfused() {
   const char* e = getenv("GMOD_SIGMA_UNFUSED");
   return e && *e && *e != '0';
 }
+// GMOD_SIGMA_UNGROUPED=1: the Sigma increment's accumulating pass reads the
+// three operands of every source instead of the grouped form
+static bool sigma_ungrouped() {
+  const char* e = getenv("GMOD_SIGMA_UNGROUPED");
+  return e && *e && *e != '0';
+}
 
 void OrbPairs::from_olist(const int64_t* olist, int Nchi_, int No_) {
   Nchi = Nchi_;
@@ -853,6 +859,25 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
     diff_to_f32(dvbuf.p, V, Vref, nC, s);
   }
   const bool fused = p4->accum_ok(-1) && !sigma_unfused();
+  // grouped form (every output has one source per orbital pair (a, b), No <= 4):
+  // the reference-vertex term Y_o = sum FVr Fd and Gn = Fr + Fd in one
+  // streaming kernel, then the accumulating pass with one operand per source
+  bool grouped = fused && No <= 4 && !sigma_ungrouped();
+  std::vector<int> ytab;
+  if (grouped) {
+    const int NAB = No * No;
+    ytab.assign((size_t)NAB * NAB, -1);
+    for (int l = 0; l < Nchi && grouped; ++l)
+      for (int m = 0; m < Nchi; ++m) {
+        const int o = op.ol[m][0] + No * op.ol[l][0], ab = op.ol[m][1] + No * op.ol[l][1];
+        int& slot = ytab[(size_t)o * NAB + ab];
+        if (slot >= 0) { grouped = false; break; }
+        slot = m + Nchi * l;
+      }
+    for (int v : ytab) grouped = grouped && v >= 0;
+  }
+  // ysum: Y of the No^2 outputs, then Gn of the No^2 orbital pairs
+  if (grouped && ysum.n != (size_t)2 * No * No * N) ysum.alloc((size_t)2 * No * No * N);
   if (fused) {
     accum_tab_h.clear();
     int nmax = 0;
@@ -863,9 +888,13 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
           for (int m = 0; m < Nchi; ++m)
             if (op.ol[m][0] == m1 && op.ol[l][0] == l1) {
               const int a = op.ol[m][1], b = op.ol[l][1];
-              accum_tab_h.push_back(grid(Fr, a, b));
-              accum_tab_h.push_back(grid(Fd, a, b));
-              accum_tab_h.push_back(FVr.p + (size_t)(m + Nchi * l) * N);
+              if (grouped) {
+                accum_tab_h.push_back(ysum.p + (size_t)No * No * N + (size_t)(a + No * b) * N);  // Gn_ab
+              } else {
+                accum_tab_h.push_back(grid(Fr, a, b));
+                accum_tab_h.push_back(grid(Fd, a, b));
+                accum_tab_h.push_back(FVr.p + (size_t)(m + Nchi * l) * N);
+              }
               ++n;
             }
         nmax = std::max(nmax, n);
@@ -874,6 +903,14 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
     accum_tab.alloc(accum_tab_h.size());
     accum_tab.upload(accum_tab_h.data(), accum_tab_h.size(), s);
     GMOD_HIP(hipStreamSynchronize(s));
+  }
+  if (grouped) {
+    if (ytab_d.n != ytab.size()) ytab_d.alloc(ytab.size());
+    ytab_d.upload(ytab.data(), ytab.size(), s);
+    GMOD_HIP(hipStreamSynchronize(s));
+    const double NAB = (double)No * No;
+    ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, (NAB * NAB + 4.0 * NAB) * (double)N * sizeof(cpx<R>), s);
+    sigma_ref_contract<R>(ysum.p, ysum.p + (size_t)No * No * N, Fr.p, Fd.p, FVr.p, ytab_d.p, No * No, No, N, s);
   }
   long tab_off = 0;
   for (int l1 = 0; l1 < No; ++l1)
@@ -897,7 +934,12 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
           ++nsrc;
         }
       if (fused && nsrc > 0) {
-        p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccum<R>{accum_tab.p + 3 * tab_off, acc.p, S}, -1, (R)1, s);
+        if (grouped) {
+          const cpx<R>* Y = ysum.p + (size_t)(m1 + No * l1) * N;
+          p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccumG<R>{accum_tab.p + tab_off, Y, acc.p, S}, -1, (R)1, s);
+        } else {
+          p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccum<R>{accum_tab.p + 3 * tab_off, acc.p, S}, -1, (R)1, s);
+        }
         tab_off += nsrc;
       }
       if (first) continue;

@@ -505,6 +505,36 @@ struct SigmaIncAccum {
   __host__ double src_bytes(double N) const { return 3.0 * N * sizeof(cpx<R>); }  // Fr, Fd, FVr read
   __host__ double sum_bytes(double N) const { return N * sizeof(cpx<R>); }
 };
+// SigmaIncAccumG: sum_c dV~_c Gn_c + Y, the Sigma increment when the
+// reference-vertex term Y = sum_c FVr_c Fd_c and Gn = Fr + Fd were formed
+// beforehand (sigma_ref_contract)
+template <typename R>
+struct SigmaIncAccumG {
+  cpx<R>* const* tab;  // [c] = Gn_c
+  const cpx<R>* Y;
+  cpx<R>* out;
+  unsigned S;
+  struct Pre { cpx<R> g; };
+  __device__ Pre prep(int c, unsigned sp, unsigned t) const { return Pre{tab[c][sp + __umul24(S, t)]}; }
+  __device__ cpx<R> contrib(int, const Pre& q, cpx<R> v) const { return v * q.g; }
+  __device__ void side(int, unsigned, unsigned, const Pre&, cpx<R>) const {}
+  __device__ void finish(unsigned sp, unsigned t, cpx<R> sum) const {
+    const size_t ad = sp + __umul24(S, t);
+    out[ad] = sum + Y[ad];
+  }
+  __host__ double src_bytes(double N) const { return N * sizeof(cpx<R>); }      // Gn read
+  __host__ double sum_bytes(double N) const { return 2.0 * N * sizeof(cpx<R>); }  // Y read, the sum written
+};
+
+// For every point e of the full grid and Sigma output o (nout of them):
+// Y_o = sum_{ab} FVr[tab[o NAB + ab]] Fd_ab, where ab = a + No b runs over
+// all No^2 orbital pairs (NAB = No^2); also Gn_ab = Fr_ab + Fd_ab.  Grids of
+// N elements, the orbital-pair grids consecutive (Fr, Fd, Gn), the vertex
+// grids at FVr + c N, the outputs at Y + o N.
+template <typename R>
+void sigma_ref_contract(cpx<R>* Y, cpx<R>* Gn, const cpx<R>* Fr, const cpx<R>* Fd, const cpx<R>* FVr,
+                        const int* tab, int nout, int No, long N, hipStream_t s);
+
 template <typename R>
 struct MulAccAccum {
   cpx<R>* const* tab;  // [c] = F_c

@@ -1,4 +1,5 @@
 // Grid kernels: scatter / product / gather / elementwise / reductions.
+#include <algorithm>
 #include <cmath>
 
 #include "kernels.h"
@@ -490,10 +491,52 @@ double min_real(const zd* a, long n, hipStream_t s) {
   return host_reduce(part.p, RED_BLOCKS, s, 1);
 }
 
+// ------------------------------------------------------------------ Sigma reference-vertex term
+// One thread per grid point: the No^2 increments Fd_ab in registers (read
+// once), then every output's sum over its No^2 reference-vertex grids; the
+// vertex grids are read once each, coalesced along the point index.
+template <typename R, int NAB>
+__global__ void __launch_bounds__(256)
+k_sigma_ref_contract(cpx<R>* __restrict__ Y, cpx<R>* __restrict__ Gn, const cpx<R>* __restrict__ Fr,
+                     const cpx<R>* __restrict__ Fd, const cpx<R>* __restrict__ FVr, const int* __restrict__ tab,
+                     int nout, long N) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+    cpx<R> fd[NAB];
+#pragma unroll
+    for (int ab = 0; ab < NAB; ++ab) {
+      fd[ab] = Fd[ab * N + e];
+      Gn[ab * N + e] = Fr[ab * N + e] + fd[ab];
+    }
+    for (int o = 0; o < nout; ++o) {
+      const int* t = tab + o * NAB;
+      cpx<R> acc(0, 0);
+#pragma unroll
+      for (int ab = 0; ab < NAB; ++ab) acc += FVr[(long)t[ab] * N + e] * fd[ab];
+      Y[(long)o * N + e] = acc;
+    }
+  }
+}
+
+template <typename R>
+void sigma_ref_contract(cpx<R>* Y, cpx<R>* Gn, const cpx<R>* Fr, const cpx<R>* Fd, const cpx<R>* FVr,
+                        const int* tab, int nout, int No, long N, hipStream_t s) {
+  const unsigned grid = (unsigned)std::min<long>((N + 255) / 256, 256L * 32);
+  switch (No) {
+    case 1: hipLaunchKernelGGL((k_sigma_ref_contract<R, 1>), dim3(grid), dim3(256), 0, s, Y, Gn, Fr, Fd, FVr, tab, nout, N); break;
+    case 2: hipLaunchKernelGGL((k_sigma_ref_contract<R, 4>), dim3(grid), dim3(256), 0, s, Y, Gn, Fr, Fd, FVr, tab, nout, N); break;
+    case 3: hipLaunchKernelGGL((k_sigma_ref_contract<R, 9>), dim3(grid), dim3(256), 0, s, Y, Gn, Fr, Fd, FVr, tab, nout, N); break;
+    case 4: hipLaunchKernelGGL((k_sigma_ref_contract<R, 16>), dim3(grid), dim3(256), 0, s, Y, Gn, Fr, Fd, FVr, tab, nout, N); break;
+    default: throw Error("sigma_ref_contract: Norb > 4");
+  }
+  GMOD_CHECK_LAUNCH();
+}
+
 // ------------------------------------------------------------------ instantiations
 #define INST(R) \
   template void mul_acc<R>(cpx<R>*, const cpx<R>*, const cpx<R>*, bool, long, hipStream_t); \
-  template void add_inplace<R>(cpx<R>*, const cpx<R>*, long, hipStream_t);
+  template void add_inplace<R>(cpx<R>*, const cpx<R>*, long, hipStream_t); \
+  template void sigma_ref_contract<R>(cpx<R>*, cpx<R>*, const cpx<R>*, const cpx<R>*, const cpx<R>*, const int*, int, \
+                                      int, long, hipStream_t);
 INST(double)
 INST(float)
 

@@ -193,8 +193,15 @@ def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, monkeyp
     try:
         F.set_precision(prec)
         monkeypatch.delenv("GMOD_SIGMA_UNFUSED", raising=False)
+        monkeypatch.delenv("GMOD_SIGMA_UNGROUPED", raising=False)
         sf, mf = F.mkself(*args, **opts)
         nf = F.flex_stats()
+        if prec == 1:
+            # the increments' accumulating pass reading all three operands per
+            # source instead of the grouped reference-vertex term
+            monkeypatch.setenv("GMOD_SIGMA_UNGROUPED", "1")
+            sg, mg = F.mkself(*args, **opts)
+            monkeypatch.delenv("GMOD_SIGMA_UNGROUPED")
         monkeypatch.setenv("GMOD_SIGMA_UNFUSED", "1")
         su, mu_ = F.mkself(*args, **opts)
         nu = F.flex_stats()
@@ -206,6 +213,8 @@ def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, monkeyp
     tol = 1e-12 if prec == 0 else 1e-6
     assert rel(sf, su) < tol
     assert abs(mf - mu_) < tol
+    if prec == 1:
+        assert rel(sg, su) < tol and abs(mg - mu_) < tol
 
 
 def test_mkself_warm_start_from_memory_and_sigma_bin(F, tmp_path, monkeypatch):
