@@ -50,14 +50,15 @@ HBM_PEAK_GBS = 8000.0
 # both arms compute chi0 with the FP64 Matsubara tail correction
 # (libs/src/fchi.f90:852-892; sw_tail as main.py:165 sets it)
 SW_TAIL = True
-# HBM bytes per FP32 FFT-pass launch (box_fft_kernel and box_accum_kernel) on
+# HBM bytes per launch of the FP32 convolution kernels (box_fft_kernel,
+# box_accum_kernel and the Sigma increment's reference-vertex contraction) on
 # the 4-orbital 32x32x16 x 1024-Matsubara workload (tail-corrected chi0, warm
-# start), averaged over the FP32 pass dispatches of the timed mixed solve:
-# rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE (KiB x 1024), both calibrated on a
-# device copy of known size (FETCH_SIZE reads half the bytes on this device),
-# per functor class, weighted by the classes' dispatch counts in the full
-# solve.  Measured offline: counter runs cannot be part of the timed run.
-FP32_BOX_TRAFFIC_BYTES = {3: 1462511170}
+# start), averaged over the dispatches of the timed mixed solve: rocprofv3
+# FETCH_SIZE x 2 + WRITE_SIZE (KiB x 1024), both calibrated on a device copy of
+# known size (FETCH_SIZE reads half the bytes on this device), per kernel
+# class, weighted by the classes' dispatch counts in the full solve.  Measured
+# offline: counter runs cannot be part of the timed run.
+FP32_BOX_TRAFFIC_BYTES = {3: 1390197668}
 
 
 def log(*a):

@@ -260,12 +260,11 @@ void BoxPlan<R>::build(int nx, int ny, int nz, int m, int mask, long budget, siz
       }
     }
   }
-  // boxes that are not forced to the whole spatial volume may be capped at
-  // GMOD_FFT_SMALL_BOX elements so that two work-groups share a CU.  Measured
-  // on the 4-orbital 32x32x16 x 2048 grid it does not pay (FP64 unchanged,
-  // FP32 13 % slower with the 32 B rows of a 4-column t box): no cap.
-  long small = budget;
-  if (const char* sb = getenv("GMOD_FFT_SMALL_BOX")) small = std::max(1L, std::min(budget, atol(sb)));
+  // boxes are not capped below the LDS budget: halving them so that two
+  // work-groups share a CU measured no faster on the 4-orbital 32x32x16 x
+  // 2048 grid (FP64 unchanged, FP32 13 % slower with the 32 B rows of a
+  // 4-column t box)
+  const long small = budget;
   std::vector<BoxArgs> spatial;
   for (auto& gdims : groups) {
     int ext[4] = {1, 1, 1, 1};
