@@ -136,6 +136,19 @@ class ConvEngine {
   const zd* Vd_dev = nullptr;
   std::vector<double> S_cache, C_cache;
   cpx<R>* grid(DBuf<cpx<R>>& b, int a, int c) { return b.p + (long)(a + op.No * c) * N; }
+  // The Sigma outputs in the order their accumulating passes run: an output
+  // (m1, l1) and its transpose (l1, m1) back to back, sharing the head passes
+  // (the spatial transforms) of their vertex sources when `share`: the
+  // bosonic grid of pair (l, m) is conj B_ml(x, -t), so its head passes are
+  // conj X_ml(-r, -t), read through the accumulating pass's source map.
+  struct SigmaPlan {
+    std::vector<int> outs;                               // m1 + No l1
+    std::vector<std::vector<std::array<int, 3>>> src;    // per output: (m, l, source-map entry)
+    std::vector<std::vector<std::array<int, 3>>> heads;  // per output, run before it: (m, l, slot)
+    int nmax = 0;                                        // head slots
+  };
+  SigmaPlan plan_sigma(bool share) const;
+  DBuf<int> srcmap_d;
   // an inverse transform whose first pass forms a product of 1-3 terms (PL:
   // ProductLoad or IncProductLoad), the term count fixed at compile time so
   // the operand pointers stay in scalar registers (a run-time term index puts

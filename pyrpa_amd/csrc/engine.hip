@@ -13,6 +13,7 @@
 #include <cstdlib>
 
 #include <array>
+#include <map>
 
 #include "engine.h"
 
@@ -483,6 +484,48 @@ void ConvEngine<R>::chi0_split_tail(zd* chi, double temp, const double* eig_d, c
 }
 
 template <typename R>
+typename ConvEngine<R>::SigmaPlan ConvEngine<R>::plan_sigma(bool share) const {
+  const int No = op.No, Nchi = op.Nchi;
+  SigmaPlan P;
+  std::vector<char> done((size_t)No * No, 0);
+  for (int l1 = 0; l1 < No; ++l1)
+    for (int m1 = 0; m1 < No; ++m1) {
+      const int o = m1 + No * l1;
+      if (done[o]) continue;
+      std::vector<int> batch{o};
+      if (share && m1 != l1) batch.push_back(l1 + No * m1);
+      std::map<std::pair<int, int>, int> slot;  // (m, l) -> head slot of this batch
+      for (int ob : batch) {
+        const int bm1 = ob % No, bl1 = ob / No;
+        std::vector<std::array<int, 3>> srcl, heads;
+        for (int l = 0; l < Nchi; ++l)
+          for (int m = 0; m < Nchi; ++m) {
+            if (op.ol[m][0] != bm1 || op.ol[l][0] != bl1) continue;
+            int e;
+            auto it = slot.find({m, l});
+            if (it != slot.end()) {
+              e = 2 * it->second;
+            } else if (share && (it = slot.find({l, m})) != slot.end()) {
+              e = 2 * it->second + 1;
+            } else {
+              const int sl = (int)slot.size();
+              slot[{m, l}] = sl;
+              heads.push_back({m, l, sl});
+              e = 2 * sl;
+            }
+            srcl.push_back({m, l, e});
+          }
+        P.outs.push_back(ob);
+        P.src.push_back(srcl);
+        P.heads.push_back(heads);
+        done[ob] = 1;
+      }
+      P.nmax = std::max(P.nmax, (int)slot.size());
+    }
+  return P;
+}
+
+template <typename R>
 void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* S_h, const double* C_h, double temp, hipStream_t s,
                           bool apply_threshold) {
   // the nu -> infinity slot of V_sigma is 1.5 S - 0.5 C (fself.f90:208)
@@ -504,60 +547,69 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* stat_h, double tem
   // the accumulating last pass sums the pairs of one output in registers
   // (GMOD_SIGMA_UNFUSED=1: one pass per pair, accumulating through memory)
   const bool fused = p4->accum_ok(-1) && !sigma_unfused();
-  std::vector<std::array<int, 2>> pairs;
+  auto gather_out = [&](int m1, int l1, bool any) {
+    zd* dst = sig + NkNw * (m1 + (long)No * l1);
+    if (!any) {
+      GMOD_HIP(hipMemsetAsync(dst, 0, NkNw * sizeof(zd), s));
+      return;
+    }
+    if (!mesh.complete) GMOD_HIP(hipMemsetAsync(dst, 0, NkNw * sizeof(zd), s));
+    p4->exec_fused(GridLoad<R>{acc.p, (unsigned)mesh.Nkall},
+                   GatherStore<R>{dst, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md}, acc.p, tmp.p, +1, invN, s);
+  };
   if (fused) {
+    // transposed outputs share their sources' head passes when the static
+    // slot is symmetric (the conjugate-reversal relation of the sources)
+    bool share = p4->accum_last_is_t(-1);
+    for (int l = 0; l < Nchi && share; ++l)
+      for (int m = 0; m < Nchi; ++m)
+        if (stat_h[m + Nchi * l] != stat_h[l + Nchi * m]) { share = false; break; }
+    const SigmaPlan P = plan_sigma(share);
     accum_tab_h.clear();
-    int nmax = 0;
-    for (int l1 = 0; l1 < No; ++l1)
-      for (int m1 = 0; m1 < No; ++m1) {
-        int n = 0;
-        for (int l = 0; l < Nchi; ++l)
-          for (int m = 0; m < Nchi; ++m)
-            if (op.ol[m][0] == m1 && op.ol[l][0] == l1) {
-              accum_tab_h.push_back(grid(Fg, op.ol[m][1], op.ol[l][1]));
-              ++n;
-            }
-        nmax = std::max(nmax, n);
+    std::vector<int> smap;
+    for (size_t i = 0; i < P.outs.size(); ++i)
+      for (auto& e : P.src[i]) {
+        accum_tab_h.push_back(grid(Fg, op.ol[e[0]][1], op.ol[e[1]][1]));
+        smap.push_back(e[2]);
       }
-    if (srcs.n != (size_t)nmax * N) srcs.alloc((size_t)nmax * N);
+    if (srcs.n != (size_t)P.nmax * N) srcs.alloc((size_t)P.nmax * N);
     accum_tab.alloc(accum_tab_h.size());
     accum_tab.upload(accum_tab_h.data(), accum_tab_h.size(), s);
+    srcmap_d.alloc(smap.size());
+    srcmap_d.upload(smap.data(), smap.size(), s);
     GMOD_HIP(hipStreamSynchronize(s));
-  }
-  long tab_off = 0;
-  for (int l1 = 0; l1 < No; ++l1)
-    for (int m1 = 0; m1 < No; ++m1) {
-      bool first = true;
-      int nsrc = 0;
-      for (int l = 0; l < Nchi; ++l)
-        for (int m = 0; m < Nchi; ++m) {
-          if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
-          const zd stat(stat_h[m + Nchi * l], 0);
-          BosonLoad<R> bl{V, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md};
-          if (fused) {
-            p4->exec_head(bl, srcs.p + (long)nsrc * N, tmp.p, -1, s);
-          } else {
+    long off = 0;
+    for (size_t i = 0; i < P.outs.size(); ++i) {
+      for (auto& h : P.heads[i]) {
+        const int m = h[0], l = h[1];
+        const zd stat(stat_h[m + Nchi * l], 0);
+        p4->exec_head(BosonLoad<R>{V, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md},
+                      srcs.p + (long)h[2] * N, tmp.p, -1, s);
+      }
+      const int nsrc = (int)P.src[i].size();
+      if (nsrc > 0)
+        p4->exec_accum_last(srcs.p, N, nsrc, MulAccAccum<R>{accum_tab.p + off, acc.p, (unsigned)mesh.Nkall}, -1,
+                            (R)1, s, srcmap_d.p + off, md.rev);
+      off += nsrc;
+      gather_out(P.outs[i] % No, P.outs[i] / No, nsrc > 0);
+    }
+  } else {
+    for (int l1 = 0; l1 < No; ++l1)
+      for (int m1 = 0; m1 < No; ++m1) {
+        bool first = true;
+        for (int l = 0; l < Nchi; ++l)
+          for (int m = 0; m < Nchi; ++m) {
+            if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
+            const zd stat(stat_h[m + Nchi * l], 0);
+            BosonLoad<R> bl{V, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md};
             p4->exec_fused(bl, MulAccStore<R>{acc.p, grid(Fg, op.ol[m][1], op.ol[l][1]), first ? 1 : 0,
                                               (unsigned)mesh.Nkall},
                            work.p, tmp.p, -1, (R)1, s);
+            first = false;
           }
-          first = false;
-          ++nsrc;
-        }
-      if (fused && nsrc > 0) {
-        p4->exec_accum_last(srcs.p, N, nsrc, MulAccAccum<R>{accum_tab.p + tab_off, acc.p, (unsigned)mesh.Nkall},
-                            -1, (R)1, s);
-        tab_off += nsrc;
+        gather_out(m1, l1, !first);
       }
-      zd* dst = sig + NkNw * (m1 + (long)No * l1);
-      if (first) {
-        GMOD_HIP(hipMemsetAsync(dst, 0, NkNw * sizeof(zd), s));
-        continue;
-      }
-      if (!mesh.complete) GMOD_HIP(hipMemsetAsync(dst, 0, NkNw * sizeof(zd), s));
-      p4->exec_fused(GridLoad<R>{acc.p, (unsigned)mesh.Nkall}, GatherStore<R>{dst, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md},
-                     acc.p, tmp.p, +1, invN, s);
-    }
+  }
   if (apply_threshold) threshold(sig, NkNw * No * No, 1e-9, s);
 }
 
@@ -878,76 +930,83 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
   }
   // ysum: Y of the No^2 outputs, then Gn of the No^2 orbital pairs
   if (grouped && ysum.n != (size_t)2 * No * No * N) ysum.alloc((size_t)2 * No * No * N);
-  if (fused) {
-    accum_tab_h.clear();
-    int nmax = 0;
+  auto gather_out = [&](int m1, int l1) {
+    const long o = NkNw * (m1 + (long)No * l1);
+    GatherStore<R> gst{sig + o, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md};
+    gst.base = sig_base + o;
+    p4->exec_fused(GridLoad<R>{acc.p, S}, gst, acc.p, tmp.p, +1, invN, s);
+  };
+  if (!fused) {
     for (int l1 = 0; l1 < No; ++l1)
       for (int m1 = 0; m1 < No; ++m1) {
-        int n = 0;
+        bool first = true;
         for (int l = 0; l < Nchi; ++l)
-          for (int m = 0; m < Nchi; ++m)
-            if (op.ol[m][0] == m1 && op.ol[l][0] == l1) {
-              const int a = op.ol[m][1], b = op.ol[l][1];
-              if (grouped) {
-                accum_tab_h.push_back(ysum.p + (size_t)No * No * N + (size_t)(a + No * b) * N);  // Gn_ab
-              } else {
-                accum_tab_h.push_back(grid(Fr, a, b));
-                accum_tab_h.push_back(grid(Fd, a, b));
-                accum_tab_h.push_back(FVr.p + (size_t)(m + Nchi * l) * N);
-              }
-              ++n;
-            }
-        nmax = std::max(nmax, n);
+          for (int m = 0; m < Nchi; ++m) {
+            if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
+            const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
+            const int a = op.ol[m][1], b = op.ol[l][1];
+            SigmaIncStore<R> st{acc.p, grid(Fr, a, b), grid(Fd, a, b), FVr.p + (size_t)(m + Nchi * l) * N,
+                                first ? 1 : 0, S};
+            p4->exec_fused(BosonLoad<R, float>{dvbuf.p, oml, olm, zd(0, 0), md}, st, work.p, tmp.p, -1, (R)1, s);
+            first = false;
+          }
+        if (!first) gather_out(m1, l1);
       }
-    if (srcs.n != (size_t)nmax * N) srcs.alloc((size_t)nmax * N);
-    accum_tab.alloc(accum_tab_h.size());
-    accum_tab.upload(accum_tab_h.data(), accum_tab_h.size(), s);
-    GMOD_HIP(hipStreamSynchronize(s));
+    return;
   }
+  // transposed outputs share the head passes of their sources (the vertex
+  // differences' static slot is zero, so the conjugate-reversal relation holds)
+  const SigmaPlan P = plan_sigma(p4->accum_last_is_t(-1));
+  accum_tab_h.clear();
+  std::vector<int> smap;
+  for (size_t i = 0; i < P.outs.size(); ++i)
+    for (auto& e : P.src[i]) {
+      const int m = e[0], l = e[1], a = op.ol[m][1], b = op.ol[l][1];
+      if (grouped) {
+        accum_tab_h.push_back(ysum.p + (size_t)No * No * N + (size_t)(a + No * b) * N);  // Gn_ab
+      } else {
+        accum_tab_h.push_back(grid(Fr, a, b));
+        accum_tab_h.push_back(grid(Fd, a, b));
+        accum_tab_h.push_back(FVr.p + (size_t)(m + Nchi * l) * N);
+      }
+      smap.push_back(e[2]);
+    }
+  if (srcs.n != (size_t)P.nmax * N) srcs.alloc((size_t)P.nmax * N);
+  accum_tab.alloc(accum_tab_h.size());
+  accum_tab.upload(accum_tab_h.data(), accum_tab_h.size(), s);
+  srcmap_d.alloc(smap.size());
+  srcmap_d.upload(smap.data(), smap.size(), s);
   if (grouped) {
     if (ytab_d.n != ytab.size()) ytab_d.alloc(ytab.size());
     ytab_d.upload(ytab.data(), ytab.size(), s);
-    GMOD_HIP(hipStreamSynchronize(s));
+  }
+  GMOD_HIP(hipStreamSynchronize(s));
+  if (grouped) {
     const double NAB = (double)No * No;
     ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, (NAB * NAB + 4.0 * NAB) * (double)N * sizeof(cpx<R>), s);
     sigma_ref_contract<R>(ysum.p, ysum.p + (size_t)No * No * N, Fr.p, Fd.p, FVr.p, ytab_d.p, No * No, No, N, s);
   }
-  long tab_off = 0;
-  for (int l1 = 0; l1 < No; ++l1)
-    for (int m1 = 0; m1 < No; ++m1) {
-      bool first = true;
-      int nsrc = 0;
-      for (int l = 0; l < Nchi; ++l)
-        for (int m = 0; m < Nchi; ++m) {
-          if (op.ol[m][0] != m1 || op.ol[l][0] != l1) continue;
-          const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
-          BosonLoad<R, float> bl{dvbuf.p, oml, olm, zd(0, 0), md};
-          if (fused) {
-            p4->exec_head(bl, srcs.p + (long)nsrc * N, tmp.p, -1, s);
-          } else {
-            const int a = op.ol[m][1], b = op.ol[l][1];
-            SigmaIncStore<R> st{acc.p, grid(Fr, a, b), grid(Fd, a, b), FVr.p + (size_t)(m + Nchi * l) * N,
-                                first ? 1 : 0, S};
-            p4->exec_fused(bl, st, work.p, tmp.p, -1, (R)1, s);
-          }
-          first = false;
-          ++nsrc;
-        }
-      if (fused && nsrc > 0) {
-        if (grouped) {
-          const cpx<R>* Y = ysum.p + (size_t)(m1 + No * l1) * N;
-          p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccumG<R>{accum_tab.p + tab_off, Y, acc.p, S}, -1, (R)1, s);
-        } else {
-          p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccum<R>{accum_tab.p + 3 * tab_off, acc.p, S}, -1, (R)1, s);
-        }
-        tab_off += nsrc;
-      }
-      if (first) continue;
-      const long o = NkNw * (m1 + (long)No * l1);
-      GatherStore<R> gst{sig + o, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md};
-      gst.base = sig_base + o;
-      p4->exec_fused(GridLoad<R>{acc.p, S}, gst, acc.p, tmp.p, +1, invN, s);
+  long off = 0;
+  for (size_t i = 0; i < P.outs.size(); ++i) {
+    for (auto& h : P.heads[i]) {
+      const int m = h[0], l = h[1];
+      const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
+      p4->exec_head(BosonLoad<R, float>{dvbuf.p, oml, olm, zd(0, 0), md}, srcs.p + (long)h[2] * N, tmp.p, -1, s);
     }
+    const int nsrc = (int)P.src[i].size();
+    const int m1 = P.outs[i] % No, l1 = P.outs[i] / No;
+    if (nsrc == 0) continue;
+    if (grouped) {
+      const cpx<R>* Y = ysum.p + (size_t)P.outs[i] * N;
+      p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccumG<R>{accum_tab.p + off, Y, acc.p, S}, -1, (R)1, s,
+                          srcmap_d.p + off, md.rev);
+    } else {
+      p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccum<R>{accum_tab.p + 3 * off, acc.p, S}, -1, (R)1, s,
+                          srcmap_d.p + off, md.rev);
+    }
+    off += nsrc;
+    gather_out(m1, l1);
+  }
 }
 
 template class ConvEngine<double>;

@@ -233,10 +233,21 @@ class FFTPlan {
   // 2 or 3 of them, a tiled last pass of exactly BOX_TILE_REGS tiles with
   // radices <= 16).
   bool accum_ok(int sign) const;
+  // the accumulating (last) pass transforms the t axis alone: the sources it
+  // reads hold spatial transforms only
+  bool accum_last_is_t(int sign) const {
+    const std::vector<BoxArgs>& bp = sign < 0 ? box_.fwd : box_.bwd;
+    if (!box_.ok || bp.empty()) return false;
+    const BoxArgs& a = bp.back();
+    return a.nrad[0] == 0 && a.nrad[1] == 0 && a.nrad[2] == 0 && a.nrad[3] > 0;
+  }
   template <class LD>
   void exec_head(LD ld, cpx<R>* out, cpx<R>* tmp, int sign, hipStream_t s) const;
+  // srcmap (device, nsrc entries, optional): source c is buffer srcmap[c] >> 1,
+  // read conjugated at (-r, -t) through rev when srcmap[c] & 1
   template <class AC>
-  void exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale, hipStream_t s) const;
+  void exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale, hipStream_t s,
+                       const int* srcmap = nullptr, const int* rev = nullptr) const;
   // Boxes of the forward list's last pass (the last pass of a sign -1 or a
   // spatial-first transform) that a gather keeping the spatial points with
   // kept_sp[sp] >= 0 needs: per box index (after box_remap) 1 if the box
@@ -415,7 +426,8 @@ void FFTPlan<R>::exec_head(LD ld, cpx<R>* out, cpx<R>* tmp, int sign, hipStream_
 template <typename R>
 template <class AC>
 void FFTPlan<R>::exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale,
-                                 hipStream_t s) const {
+                                 hipStream_t s, const int* srcmap, const int* rev) const {
+  if (srcmap && !rev) throw Error("exec_accum_last: a source map needs the reversal map");
   const int sg = sign < 0 ? 0 : 1;
   const std::vector<BoxArgs>& bp = sg == 0 ? box_.fwd : box_.bwd;
   const int i = (int)bp.size() - 1;
@@ -433,7 +445,8 @@ void FFTPlan<R>::exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac
   long grid = a.blocks();
   if (grid > persist_) grid = persist_;
   hipLaunchKernelGGL((box_accum_kernel<R, 16, AC>), dim3((unsigned)grid), dim3(BOX_THREADS), shm, s, src, stride,
-                     nsrc, ac, (const BoxArgs*)(argsd_[sg] + i), twb_[sg], (const int*)tabs_, scale, sg == 0 ? -1 : 1);
+                     nsrc, ac, (const BoxArgs*)(argsd_[sg] + i), twb_[sg], (const int*)tabs_, scale, sg == 0 ? -1 : 1,
+                     srcmap, rev);
   GMOD_CHECK_LAUNCH();
 }
 

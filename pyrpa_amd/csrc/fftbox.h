@@ -721,6 +721,27 @@ __device__ __forceinline__ void box_persistent(cpx<R>* lds, const LD& ld, const 
 template <typename R>
 struct GridLoad;  // fft.h
 
+// A source of the accumulating pass: the earlier passes' output stored at p,
+// read as stored, or (cr) conjugated at the reversed point (-r, -t) through
+// the spatial reversal map rev: the grid whose transform is conj X(-r, -t)
+template <typename R>
+struct AccumSrcLoad {
+  const cpx<R>* p;
+  const int* rev;
+  unsigned S, M;
+  int cr;
+  // key / value: the fetch issues the reversal-map loads of all its elements
+  // before their value loads
+  using Key = unsigned;  // the spatial index read
+  __host__ __device__ Key key(unsigned sp) const { return cr ? (unsigned)rev[sp] : sp; }
+  __host__ __device__ cpx<R> value(Key k, unsigned, unsigned t) const {
+    const unsigned tt = (cr && t != 0) ? M - t : t;
+    const cpx<R> v = p[k + mul24(S, tt)];
+    return cr ? cpx<R>(v.re, -v.im) : v;
+  }
+  __host__ __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
+};
+
 // Accumulating last pass: nsrc transforms that share this last pass (their
 // earlier passes left them in src + c src_stride, c < nsrc) are summed, each
 // with its own operands, into one grid.  A persistent work-group transforms
@@ -731,10 +752,13 @@ struct GridLoad;  // fft.h
 // provides prep(c, sp, t) (its operand loads for source c), contrib(c, pre,
 // v) (the contribution of transformed value v), side(c, sp, t, pre, v) (its
 // writes besides the sum) and finish(sp, t, sum).  MAXR caps the radices compiled in.
+// srcmap (optional): source c is buffer srcmap[c] >> 1 of src, conjugated at
+// (-r, -t) when srcmap[c] & 1 (rev: the spatial reversal map); else buffer c.
 template <typename R, int MAXR, class AC>
 __global__ void __launch_bounds__(BOX_THREADS)
 box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC ac, const BoxArgs* __restrict__ ap,
-                 const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale, int sign) {
+                 const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale, int sign,
+                 const int* __restrict__ srcmap, const int* __restrict__ rev) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const BoxArgs& a = *ap;
@@ -763,11 +787,15 @@ box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC a
   constexpr bool PF = sizeof(R) == 4;
   constexpr int TV = PF ? TM : 1;
   cpx<R> v[TV], sum[TM];
+  auto srcld = [&](int cc) {
+    const int e = srcmap ? srcmap[cc] : 2 * cc;
+    return AccumSrcLoad<R>{src + (long)(e >> 1) * src_stride, rev, a.S, a.M, e & 1};
+  };
   box_twiddles_to_lds<R>(lds, a, tw, tid, BOX_THREADS);  // ordered before use by the loop's first barrier
   unsigned b = blockIdx.x;
   int c = 0;
   if constexpr (PF) {
-    if (b < nb) box_fetch<R, TM>(v, GridLoad<R>{src, a.S}, a, b, nb, csp, ct, tsp, tt);
+    if (b < nb) box_fetch<R, TM>(v, srcld(0), a, b, nb, csp, ct, tsp, tt);
   }
   while (b < nb) {
     if constexpr (PF) {
@@ -778,12 +806,12 @@ box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC a
       const BoxOrigin O = box_origin(a, box_remap(b, nb, a.xcd_group));
       const unsigned sp0 = box_sp(a, O, 0, 0, 0) + csp;
       const unsigned t0 = mul24((unsigned)O.bt, (unsigned)a.tl_mul) + ct;
-      const cpx<R>* sc = src + (long)c * src_stride;
+      const AccumSrcLoad<R> sc = srcld(c);
 #pragma unroll
       for (int j0 = 0; j0 < TM; j0 += 8) {
         cpx<R> w[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) w[u] = sc[sp0 + (unsigned)tsp[j0 + u] + __umul24(a.S, t0 + (unsigned)tt[j0 + u])];
+        for (int u = 0; u < 8; ++u) w[u] = sc(sp0 + (unsigned)tsp[j0 + u], t0 + (unsigned)tt[j0 + u]);
 #pragma unroll
         for (int u = 0; u < 8; ++u) lds[l0 + (unsigned)tld[j0 + u]] = w[u];
       }
@@ -792,7 +820,7 @@ box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC a
     const bool more = c + 1 < nsrc;
     const unsigned b2 = more ? b : b + gridDim.x;
     const int c2 = more ? c + 1 : 0;
-    const GridLoad<R> ld2{src + (long)c2 * src_stride, a.S};
+    const AccumSrcLoad<R> ld2 = srcld(c2);
     for (int i = 0; i < a.nsub; ++i) {
       if constexpr (PF) {
         if (i == a.pf_at && b2 < nb) box_fetch<R, TM>(v, ld2, a, b2, nb, csp, ct, tsp, tt);

@@ -1589,7 +1589,7 @@ extern "C" void get_chis_chic_soc(gmod_z* chic, gmod_z* chiszz, gmod_z* chispm, 
 
 // fselfwithsoc.f90:1-539: the FLEX loop in a spinful basis.  Precision 0 runs
 // FP64 throughout; 1 and 2 run the transforms in FP32 (full fields: the
-// rolling-increment loop covers the spinless basis only).
+// increment loop around a reference iterate covers the spinless basis only).
 extern "C" void mkself_soc(gmod_z* sigmak, double* mu, const double* Vmat, const int64_t* kmap, const int64_t* invk,
                            const int64_t* invs, const int64_t* olist, const int64_t* slist, const gmod_z* hamk,
                            const double* eig, const gmod_z* uni, const double* mu_init, const double* rfill,
