@@ -560,7 +560,7 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* stat_h, double tem
   if (fused) {
     // transposed outputs share their sources' head passes when the static
     // slot is symmetric (the conjugate-reversal relation of the sources)
-    bool share = p4->accum_last_is_t(-1);
+    bool share = p4->accum_last_is_t(-1) && p4->reversal_ok();
     for (int l = 0; l < Nchi && share; ++l)
       for (int m = 0; m < Nchi; ++m)
         if (stat_h[m + Nchi * l] != stat_h[l + Nchi * m]) { share = false; break; }
@@ -589,7 +589,7 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* stat_h, double tem
       const int nsrc = (int)P.src[i].size();
       if (nsrc > 0)
         p4->exec_accum_last(srcs.p, N, nsrc, MulAccAccum<R>{accum_tab.p + off, acc.p, (unsigned)mesh.Nkall}, -1,
-                            (R)1, s, srcmap_d.p + off, md.rev);
+                            (R)1, s, srcmap_d.p + off);
       off += nsrc;
       gather_out(P.outs[i] % No, P.outs[i] / No, nsrc > 0);
     }
@@ -956,7 +956,7 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
   }
   // transposed outputs share the head passes of their sources (the vertex
   // differences' static slot is zero, so the conjugate-reversal relation holds)
-  const SigmaPlan P = plan_sigma(p4->accum_last_is_t(-1));
+  const SigmaPlan P = plan_sigma(p4->accum_last_is_t(-1) && p4->reversal_ok());
   accum_tab_h.clear();
   std::vector<int> smap;
   for (size_t i = 0; i < P.outs.size(); ++i)
@@ -999,10 +999,10 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
     if (grouped) {
       const cpx<R>* Y = ysum.p + (size_t)P.outs[i] * N;
       p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccumG<R>{accum_tab.p + off, Y, acc.p, S}, -1, (R)1, s,
-                          srcmap_d.p + off, md.rev);
+                          srcmap_d.p + off);
     } else {
       p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccum<R>{accum_tab.p + 3 * off, acc.p, S}, -1, (R)1, s,
-                          srcmap_d.p + off, md.rev);
+                          srcmap_d.p + off);
     }
     off += nsrc;
     gather_out(m1, l1);

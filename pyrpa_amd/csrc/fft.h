@@ -247,7 +247,9 @@ class FFTPlan {
   // read conjugated at (-r, -t) through rev when srcmap[c] & 1
   template <class AC>
   void exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale, hipStream_t s,
-                       const int* srcmap = nullptr, const int* rev = nullptr) const;
+                       const int* srcmap = nullptr) const;
+  // the conjugated reversed reads of a source map are available (grid range)
+  bool reversal_ok() const { return sprev_ok(dims_[0], dims_[1], dims_[2]); }
   // Boxes of the forward list's last pass (the last pass of a sign -1 or a
   // spatial-first transform) that a gather keeping the spatial points with
   // kept_sp[sp] >= 0 needs: per box index (after box_remap) 1 if the box
@@ -426,8 +428,8 @@ void FFTPlan<R>::exec_head(LD ld, cpx<R>* out, cpx<R>* tmp, int sign, hipStream_
 template <typename R>
 template <class AC>
 void FFTPlan<R>::exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale,
-                                 hipStream_t s, const int* srcmap, const int* rev) const {
-  if (srcmap && !rev) throw Error("exec_accum_last: a source map needs the reversal map");
+                                 hipStream_t s, const int* srcmap) const {
+  if (srcmap && !reversal_ok()) throw Error("exec_accum_last: grid too large for the reversed reads");
   const int sg = sign < 0 ? 0 : 1;
   const std::vector<BoxArgs>& bp = sg == 0 ? box_.fwd : box_.bwd;
   const int i = (int)bp.size() - 1;
@@ -446,7 +448,7 @@ void FFTPlan<R>::exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac
   if (grid > persist_) grid = persist_;
   hipLaunchKernelGGL((box_accum_kernel<R, 16, AC>), dim3((unsigned)grid), dim3(BOX_THREADS), shm, s, src, stride,
                      nsrc, ac, (const BoxArgs*)(argsd_[sg] + i), twb_[sg], (const int*)tabs_, scale, sg == 0 ? -1 : 1,
-                     srcmap, rev);
+                     srcmap, make_sprev(dims_[0], dims_[1], dims_[2]));
   GMOD_CHECK_LAUNCH();
 }
 

@@ -721,19 +721,39 @@ __device__ __forceinline__ void box_persistent(cpx<R>* lds, const LD& ld, const 
 template <typename R>
 struct GridLoad;  // fft.h
 
+// The spatial reversal r -> -r (each coordinate x -> (n - x) mod n) of a
+// spatial index on an nx x ny x nz grid, x fastest; the multiply-high
+// divisions are exact while S nx < 2^32 and ny nz ny < 2^32 (make_sprev)
+struct SpRev {
+  FDiv fx, fy;
+  unsigned nx, ny, nz;
+  __host__ __device__ __forceinline__ unsigned operator()(unsigned sp) const {
+    const unsigned q = fdiv(sp, fx), x = sp - q * nx;
+    const unsigned z = fdiv(q, fy), y = q - z * ny;
+    return (x ? nx - x : 0u) + nx * ((y ? ny - y : 0u) + ny * (z ? nz - z : 0u));
+  }
+};
+
+inline bool sprev_ok(int nx, int ny, int nz) {
+  const unsigned long long S = (unsigned long long)nx * ny * nz;
+  return S * (unsigned long long)nx < (1ull << 32) && (unsigned long long)ny * nz * ny < (1ull << 32);
+}
+
+inline SpRev make_sprev(int nx, int ny, int nz) {
+  return SpRev{make_fdiv((unsigned)nx), make_fdiv((unsigned)ny), (unsigned)nx, (unsigned)ny, (unsigned)nz};
+}
+
 // A source of the accumulating pass: the earlier passes' output stored at p,
-// read as stored, or (cr) conjugated at the reversed point (-r, -t) through
-// the spatial reversal map rev: the grid whose transform is conj X(-r, -t)
+// read as stored, or (cr) conjugated at the reversed point (-r, -t): the grid
+// whose transform is conj X(-r, -t)
 template <typename R>
 struct AccumSrcLoad {
   const cpx<R>* p;
-  const int* rev;
+  SpRev rv;
   unsigned S, M;
   int cr;
-  // key / value: the fetch issues the reversal-map loads of all its elements
-  // before their value loads
   using Key = unsigned;  // the spatial index read
-  __host__ __device__ Key key(unsigned sp) const { return cr ? (unsigned)rev[sp] : sp; }
+  __host__ __device__ Key key(unsigned sp) const { return cr ? rv(sp) : sp; }
   __host__ __device__ cpx<R> value(Key k, unsigned, unsigned t) const {
     const unsigned tt = (cr && t != 0) ? M - t : t;
     const cpx<R> v = p[k + mul24(S, tt)];
@@ -753,12 +773,12 @@ struct AccumSrcLoad {
 // v) (the contribution of transformed value v), side(c, sp, t, pre, v) (its
 // writes besides the sum) and finish(sp, t, sum).  MAXR caps the radices compiled in.
 // srcmap (optional): source c is buffer srcmap[c] >> 1 of src, conjugated at
-// (-r, -t) when srcmap[c] & 1 (rev: the spatial reversal map); else buffer c.
+// (-r, -t) when srcmap[c] & 1 (rv: the spatial reversal); else buffer c.
 template <typename R, int MAXR, class AC>
 __global__ void __launch_bounds__(BOX_THREADS)
 box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC ac, const BoxArgs* __restrict__ ap,
                  const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale, int sign,
-                 const int* __restrict__ srcmap, const int* __restrict__ rev) {
+                 const int* __restrict__ srcmap, SpRev rv) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const BoxArgs& a = *ap;
@@ -789,7 +809,7 @@ box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC a
   cpx<R> v[TV], sum[TM];
   auto srcld = [&](int cc) {
     const int e = srcmap ? srcmap[cc] : 2 * cc;
-    return AccumSrcLoad<R>{src + (long)(e >> 1) * src_stride, rev, a.S, a.M, e & 1};
+    return AccumSrcLoad<R>{src + (long)(e >> 1) * src_stride, rv, a.S, a.M, e & 1};
   };
   box_twiddles_to_lds<R>(lds, a, tw, tid, BOX_THREADS);  // ordered before use by the loop's first barrier
   unsigned b = blockIdx.x;
