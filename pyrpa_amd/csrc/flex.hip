@@ -431,8 +431,10 @@ static void mkself_driver(const FlexIn& A, zd* sig_out, double* mu_out, bool mix
       else if (rescale)
         rpa_vertex(Vout, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 0, err.p, s);
       else
+        // an increment iteration that keeps its reference reads only V - Vref
         rpa_vertex_fused(chi_src, Vout, NkNw, BatchLayout{1, NkNw}, Nchi, S_d.p, C_d.p, 1e-9,
-                         dv_fused ? Vref.p : nullptr, dv_fused ? e32->dv_buffer(nC) : nullptr, err.p, s);
+                         dv_fused ? Vref.p : nullptr, dv_fused ? e32->dv_buffer(nC) : nullptr, err.p, s,
+                         !dv_fused || rebase || verbose);
     }
     check_err(err, s, A.soc ? "get_V_soc_flex" : "get_vsigma_flex_nosoc");
     t_vert += secs(t0);

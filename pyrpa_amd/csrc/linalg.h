@@ -21,9 +21,12 @@ void rpa_vertex(zd* chi, long batch, BatchLayout L, int n, const double* S, cons
 // the 1e-9 threshold of the FLEX loop applied as it is read (thr > 0),
 // writes V_sigma to chi_out (may equal chi_in) and, when vprev is given,
 // dv = V_sigma - vprev in FP32 (the increment the mixed-precision loop
-// transforms)
+// transforms); with dv and store_v false, V_sigma itself may be left unstored
+// (chi_out then keeps what it held, or receives V_sigma where the
+// Gauss-Jordan kernel does not apply)
 void rpa_vertex_fused(const zd* chi_in, zd* chi_out, long batch, BatchLayout L, int n, const double* S,
-                      const double* C, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s);
+                      const double* C, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s,
+                      bool store_v = true);
 // spinful FLEX vertex in a batch (fUJ.f90:367-399): out = -V + V (I + chi0 V)^-1 chi0 V,
 // chi0 thresholded at thr (> 0) as it is read; out may equal chi_in
 void soc_vertex(const zd* chi_in, zd* out, long batch, BatchLayout L, int n, const double* V, double thr, int* err,

@@ -539,6 +539,7 @@ constexpr int GJ_MAT = 2 * GJ_WAVES;  // matrices per block
 // chi_in may equal chi_out (in place: a work-group reads all its matrices
 // before it writes any).  thr > 0 zeroes real and imaginary parts of the
 // input below thr as they are read; dv (with vprev) receives V - vprev in FP32.
+// chi_out == nullptr (with dv): the vertex itself is not stored.
 template <int N>
 __global__ void __launch_bounds__(64 * GJ_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* __restrict__ spg,
@@ -721,7 +722,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
       if (b < batch) {
         const zd v = Vbuf[m * NN + e];
         const long o = b + (long)e * es;
-        chi_out[o] = v;
+        if (chi_out) chi_out[o] = v;
         if (dv) dv[o] = cpx<float>(v - vprev[o]);
       }
     }
@@ -924,6 +925,7 @@ static void check_lds(size_t bytes) {
 static bool vertex_gj(const zd* in, zd* out, long batch, BatchLayout L, int n, const double* S, const double* C,
                       int mode, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s) {
   if (!(L.bs == 1 && n == 16)) return false;
+  if (!out && !dv) throw Error("vertex: neither the vertex nor its difference is stored");
   VSparse sp;
   if (!vertex_sparse(S, C, n, sp, s)) return false;
   DBuf<VSparse> spd(1);
@@ -938,9 +940,10 @@ static bool vertex_gj(const zd* in, zd* out, long batch, BatchLayout L, int n, c
 }
 
 void rpa_vertex_fused(const zd* chi_in, zd* chi_out, long batch, BatchLayout L, int n, const double* S,
-                      const double* C, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s) {
+                      const double* C, double thr, const zd* vprev, cpx<float>* dv, int* err, hipStream_t s,
+                      bool store_v) {
   if (batch <= 0) return;
-  if (vertex_gj(chi_in, chi_out, batch, L, n, S, C, 0, thr, vprev, dv, err, s)) return;
+  if (vertex_gj(chi_in, store_v || !dv ? chi_out : nullptr, batch, L, n, S, C, 0, thr, vprev, dv, err, s)) return;
   const long nel = (long)n * n * L.es;  // the batch's extent (L.bs == 1 layouts; general kernels otherwise)
   if (L.bs != 1) throw Error("rpa_vertex_fused: matrix-major layouts only");
   if (thr > 0.0) threshold_copy(chi_out, chi_in, nel, thr, s);
