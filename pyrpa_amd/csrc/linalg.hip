@@ -547,10 +547,15 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
             const zd* __restrict__ vprev, cpx<float>* __restrict__ dv, int* err) {
   static_assert(N == 16, "a 16-lane group per system");
   constexpr int NN = N * N;
+  // matrix stride in the LDS buffers, padded by one entry: the block-wide
+  // phases give consecutive lanes consecutive matrices (coalesced global
+  // access), whose entries then fall in different LDS banks
+  constexpr int MS = NN + 1;
   constexpr int PER = GJ_MAT * NN / (64 * GJ_WAVES);  // elements per thread in block-wide phases
+  static_assert(64 * GJ_WAVES == NN, "one matrix entry per thread in the block-wide phases");
   __shared__ VSparse sp;
-  __shared__ double2 Abuf_[GJ_MAT * NN];  // chi0, later X_s then X_c
-  __shared__ double2 Vbuf_[GJ_MAT * NN];  // A W (mode 0), then the vertex
+  __shared__ double2 Abuf_[GJ_MAT * MS];  // chi0, later X_s then X_c
+  __shared__ double2 Vbuf_[GJ_MAT * MS];  // A W (mode 0), then the vertex
   // per-group slots padded so that the four groups of a wavefront start 4
   // dwords apart in the LDS banks (their broadcast reads do not conflict)
   constexpr int PROW = 2 * N + 1;  // complex entries per group slot
@@ -567,7 +572,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
   const int g = lane >> 4, r = lane & 15;
   const int mloc = wave * 2 + (g >> 1);  // matrix of my group within the block
   const int sys = g & 1;                 // 0: S system, 1: C system
-  zd* A = Abuf + mloc * NN;
+  zd* A = Abuf + mloc * MS;
   zd* prow = reinterpret_cast<zd*>(Prow_) + (wave * 4 + g) * PROW;
   int* piv = Piv_ + (wave * 4 + g) * N;
   const double a1 = mode == 2 ? -0.5 : 1.5;  // coefficient of S X_s
@@ -599,7 +604,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
         if (fabs(v.re) < thr) v.re = 0.0;
         if (fabs(v.im) < thr) v.im = 0.0;
       }
-      Abuf[m * NN + e] = v;
+      Abuf[m * MS + e] = v;
     }
     __syncthreads();
     if (b0 + bstep < batch) fetch(b0 + bstep);
@@ -609,8 +614,8 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
         const int m = i / NN, e = i % NN, p = e % N, q = e / N;
         zd acc(0, 0);
 #pragma unroll
-        for (int t = 0; t < VW_NZ; ++t) acc += Abuf[m * NN + p + N * sp.idx[2][q][t]] * sp.val[2][q][t];
-        Vbuf[i] = acc;  // (A W)[p][q]
+        for (int t = 0; t < VW_NZ; ++t) acc += Abuf[m * MS + p + N * sp.idx[2][q][t]] * sp.val[2][q][t];
+        Vbuf[m * MS + e] = acc;  // (A W)[p][q]
       }
       __syncthreads();
     }
@@ -625,14 +630,14 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
       if (mode == 0) {
         zd acc(0, 0);
 #pragma unroll
-        for (int t = 0; t < VW_NZ; ++t) acc += Vbuf[m * NN + sp.idx[2][p][t] + N * q] * sp.val[2][p][t];
+        for (int t = 0; t < VW_NZ; ++t) acc += Vbuf[m * MS + sp.idx[2][p][t] + N * q] * sp.val[2][p][t];
         v -= acc * 0.25;
       }
       vinit[u] = v;
     }
     __syncthreads();  // every read of A W is complete
 #pragma unroll
-    for (int u = 0; u < PER; ++u) Vbuf[tid + u * 64 * GJ_WAVES] = vinit[u];
+    for (int u = 0; u < PER; ++u) Vbuf[u * MS + tid] = vinit[u];  // entry tid of matrix u
     // ---- my row of M = I -+ A K (K = S or C)
     zd mrow[N];
 #pragma unroll
@@ -711,8 +716,8 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
         zd acc(0, 0);
 #pragma unroll
         for (int t = 0; t < VW_NZ; ++t)
-          acc += Abuf[m * NN + sp.idx[pass][p][t] + N * q] * sp.val[pass][p][t];
-        Vbuf[i] += acc * coef;
+          acc += Abuf[m * MS + sp.idx[pass][p][t] + N * q] * sp.val[pass][p][t];
+        Vbuf[m * MS + e] += acc * coef;
       }
       __syncthreads();
     }
@@ -720,7 +725,7 @@ k_vertex_gj(const zd* chi_in, zd* chi_out, long batch, long es, const VSparse* _
       const int m = i % GJ_MAT, e = i / GJ_MAT;
       const long b = b0 + m;
       if (b < batch) {
-        const zd v = Vbuf[m * NN + e];
+        const zd v = Vbuf[m * MS + e];
         const long o = b + (long)e * es;
         if (chi_out) chi_out[o] = v;
         if (dv) dv[o] = cpx<float>(v - vprev[o]);
