@@ -222,7 +222,7 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
-                    help="host threads of the CPU baseline (the GPU box's CPU share is 16)")
+                    help="host threads of the CPU baseline (16: the host CPU share of one GPU)")
     ap.add_argument("--warm", default="coarse", choices=["coarse", "none"], help="warm start of the mixed arm")
     ap.add_argument("--fft", type=int, default=0, metavar="REPS",
                     help="time only the transform engine on the workload's grid (REPS transforms per precision)")
