@@ -73,7 +73,9 @@ class ConvEngine {
   // Transforms of Gref (and, with the tail, G0ref and the tau kernels at mu_ref).
   void set_reference(const zd* Gref, const zd* G0ref, bool tail, const double* eig_d, const zd* uni_d, double mu_ref,
                      double temp, hipStream_t s);
-  // Transforms of the reference vertex for all Nchi^2 pairs (full grid incl. the nu->inf slot).
+  // Transforms of the reference vertex for all Nchi^2 pairs (full grid incl.
+  // the nu->inf slot), one per transposed pair in the contraction form when
+  // the static slot is symmetric.
   void set_vertex_reference(const zd* Vref, const double* S_h, const double* C_h, hipStream_t s);
   // chi <- chi_base + chi0[G] - chi0[Gref] (tail: + the G0 and tau-reference
   // terms at mu vs mu_ref and dJ sawc), with the reference's partner /
@@ -139,8 +141,10 @@ class ConvEngine {
   // The Sigma outputs in the order their accumulating passes run: an output
   // (m1, l1) and its transpose (l1, m1) back to back, sharing the head passes
   // (the spatial transforms) of their vertex sources when `share`: the
-  // bosonic grid of pair (l, m) is conj B_ml(x, -t), so its head passes are
-  // conj X_ml(-r, -t), read through the accumulating pass's source map.
+  // bosonic grid of pair (l, m) is conj B_ml(x, -t) in every slot but slot
+  // 0, so its head passes are conj X_ml(-r, -t) plus, at t = 0, the spatial
+  // transform of the slot-0 mismatch (slot0_corr), read through the
+  // accumulating pass's source map.
   struct SigmaPlan {
     std::vector<int> outs;                               // m1 + No l1
     std::vector<std::vector<std::array<int, 3>>> src;    // per output: (m, l, source-map entry)
@@ -149,6 +153,33 @@ class ConvEngine {
   };
   SigmaPlan plan_sigma(bool share) const;
   DBuf<int> srcmap_d;
+  // The contraction form of the Sigma increment: the full transforms of the
+  // vertex differences (DVt) and of the reference vertex (FVr) are stored
+  // once per pair of transposed vertex pairs (the transform of pair (l, m)
+  // is conj X_ml(-r, t) plus the transformed slot-0 mismatch, the same for
+  // every t, after all passes), and one streaming kernel forms
+  // every output out_o = sum_ab FVr_o,ab Fd_ab + DVt_o,ab (Fr_ab + Fd_ab).
+  // grouped_table: per output o and orbital pair ab the vertex pair
+  // m + Nchi l that joins them (false unless each (o, ab) has exactly one);
+  // share_heads: a slot per stored transform, entry[m + Nchi l] = 2 slot
+  // (+1: read conjugated at -r); contract_form: whether this form applies.
+  bool grouped_table(std::vector<int>& ytab) const;
+  std::vector<std::array<int, 2>> share_heads(std::vector<int>& entry) const;
+  bool contract_form() const;
+  // The conjugate reversal is exact except in slot 0, where it needs
+  // V(k, nu_0) Hermitian in the pair index: slot0_corr writes, for nh
+  // stored transforms of pairs (m, l) read by (l, m), the spatial transform
+  // of D = B_lm(x, 0) - conj B_ml(x, 0) to dh[h S + sp] (offs_d: the V
+  // offsets of (m, l) and (l, m) per h).  head_offsets appends those offsets.
+  template <typename Sv>
+  void slot0_corr(cpx<R>* dh, const cpx<Sv>* V, const long* offs_d, int nh, hipStream_t s);
+  void head_offsets(std::vector<long>& offs, int m, int l) const;
+  DBuf<cpx<R>> dhF, dhD, dh_tmp;
+  DBuf<long> dhF_offs, dh_offs;
+  std::vector<int> fv_entry;  // FVr entry per vertex pair (set_vertex_reference)
+  bool fv_shared = false;     // FVr holds one transform per transposed pair
+  DBuf<cpx<R>> DVt;
+  DBuf<int> ctab_d;
   // an inverse transform whose first pass forms a product of 1-3 terms (PL:
   // ProductLoad or IncProductLoad), the term count fixed at compile time so
   // the operand pointers stay in scalar registers (a run-time term index puts

@@ -31,6 +31,12 @@ static bool sigma_ungrouped() {
   const char* e = getenv("GMOD_SIGMA_UNGROUPED");
   return e && *e && *e != '0';
 }
+// GMOD_SIGMA_ACCUM=1: the Sigma increment in the grouped accumulating-pass
+// form instead of the contraction form
+static bool sigma_accum_form() {
+  const char* e = getenv("GMOD_SIGMA_ACCUM");
+  return e && *e && *e != '0';
+}
 
 void OrbPairs::from_olist(const int64_t* olist, int Nchi_, int No_) {
   Nchi = Nchi_;
@@ -484,6 +490,69 @@ void ConvEngine<R>::chi0_split_tail(zd* chi, double temp, const double* eig_d, c
 }
 
 template <typename R>
+bool ConvEngine<R>::grouped_table(std::vector<int>& ytab) const {
+  const int No = op.No, Nchi = op.Nchi, NAB = No * No;
+  ytab.assign((size_t)NAB * NAB, -1);
+  for (int l = 0; l < Nchi; ++l)
+    for (int m = 0; m < Nchi; ++m) {
+      const int o = op.ol[m][0] + No * op.ol[l][0], ab = op.ol[m][1] + No * op.ol[l][1];
+      int& slot = ytab[(size_t)o * NAB + ab];
+      if (slot >= 0) return false;
+      slot = m + Nchi * l;
+    }
+  for (int v : ytab)
+    if (v < 0) return false;
+  return true;
+}
+
+template <typename R>
+std::vector<std::array<int, 2>> ConvEngine<R>::share_heads(std::vector<int>& entry) const {
+  const int Nchi = op.Nchi;
+  std::vector<std::array<int, 2>> heads;
+  entry.assign((size_t)Nchi * Nchi, -1);
+  for (int l = 0; l < Nchi; ++l)
+    for (int m = 0; m < Nchi; ++m) {
+      const int t = l + Nchi * m;  // the transposed pair
+      if (entry[t] >= 0) {
+        entry[m + Nchi * l] = (entry[t] & ~1) + 1;
+      } else {
+        entry[m + Nchi * l] = 2 * (int)heads.size();
+        heads.push_back({m, l});
+      }
+    }
+  return heads;
+}
+
+template <typename R>
+void ConvEngine<R>::head_offsets(std::vector<long>& offs, int m, int l) const {
+  const long NC = mesh.Nk * (long)mesh.Nw;
+  offs.push_back(NC * (m + (long)op.Nchi * l));
+  offs.push_back(NC * (l + (long)op.Nchi * m));
+}
+
+template <typename R>
+template <typename Sv>
+void ConvEngine<R>::slot0_corr(cpx<R>* dh, const cpx<Sv>* V, const long* offs_d, int nh, hipStream_t s) {
+  if (nh <= 0) return;
+  const unsigned S = (unsigned)mesh.Nkall;
+  const size_t n = (size_t)nh * S;
+  if (dh_tmp.n < n) dh_tmp.alloc(n);
+  const MeshDev md = mesh_dev(mesh);
+  const unsigned grid = (unsigned)std::min<long>(((long)n + 255) / 256, 256L * 64);
+  hipLaunchKernelGGL((k_slot0_diff<R, Sv>), dim3(grid), dim3(256), 0, s, dh, V, offs_d, nh, S, md);
+  GMOD_CHECK_LAUNCH();
+  fft_plan<R>(mesh.Nx, mesh.Ny, mesh.Nz, nh, 7).exec(dh, dh_tmp.p, -1, (R)1, s);
+}
+
+template <typename R>
+bool ConvEngine<R>::contract_form() const {
+  if (sigma_unfused() || sigma_ungrouped() || sigma_accum_form()) return false;
+  if (op.No > 4 || !p4->reversal_ok()) return false;
+  std::vector<int> ytab;
+  return grouped_table(ytab);
+}
+
+template <typename R>
 typename ConvEngine<R>::SigmaPlan ConvEngine<R>::plan_sigma(bool share) const {
   const int No = op.No, Nchi = op.Nchi;
   SigmaPlan P;
@@ -559,7 +628,8 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* stat_h, double tem
   };
   if (fused) {
     // transposed outputs share their sources' head passes when the static
-    // slot is symmetric (the conjugate-reversal relation of the sources)
+    // slot is symmetric (the conjugate-reversal relation of the sources, with
+    // the slot-0 correction)
     bool share = p4->accum_last_is_t(-1) && p4->reversal_ok();
     for (int l = 0; l < Nchi && share; ++l)
       for (int m = 0; m < Nchi; ++m)
@@ -577,8 +647,14 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* stat_h, double tem
     accum_tab.upload(accum_tab_h.data(), accum_tab_h.size(), s);
     srcmap_d.alloc(smap.size());
     srcmap_d.upload(smap.data(), smap.size(), s);
+    std::vector<long> hoffs;
+    for (auto& hs : P.heads)
+      for (auto& h : hs) head_offsets(hoffs, h[0], h[1]);
+    if (dh_offs.n < hoffs.size()) dh_offs.alloc(hoffs.size());
+    if (!hoffs.empty()) dh_offs.upload(hoffs.data(), hoffs.size(), s);
+    if (dhD.n < (size_t)P.nmax * mesh.Nkall) dhD.alloc((size_t)P.nmax * mesh.Nkall);
     GMOD_HIP(hipStreamSynchronize(s));
-    long off = 0;
+    long off = 0, hoff = 0;
     for (size_t i = 0; i < P.outs.size(); ++i) {
       for (auto& h : P.heads[i]) {
         const int m = h[0], l = h[1];
@@ -586,10 +662,15 @@ void ConvEngine<R>::sigma(zd* sig, const zd* V, const double* stat_h, double tem
         p4->exec_head(BosonLoad<R>{V, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md},
                       srcs.p + (long)h[2] * N, tmp.p, -1, s);
       }
+      // the heads of one output take consecutive slots
+      if (!P.heads[i].empty())
+        slot0_corr(dhD.p + (size_t)P.heads[i][0][2] * mesh.Nkall, V, dh_offs.p + 2 * hoff, (int)P.heads[i].size(),
+                   s);
+      hoff += (long)P.heads[i].size();
       const int nsrc = (int)P.src[i].size();
       if (nsrc > 0)
         p4->exec_accum_last(srcs.p, N, nsrc, MulAccAccum<R>{accum_tab.p + off, acc.p, (unsigned)mesh.Nkall}, -1,
-                            (R)1, s, srcmap_d.p + off);
+                            (R)1, s, srcmap_d.p + off, dhD.p);
       off += nsrc;
       gather_out(P.outs[i] % No, P.outs[i] / No, nsrc > 0);
     }
@@ -796,17 +877,47 @@ template <typename R>
 void ConvEngine<R>::set_vertex_reference(const zd* Vref, const double* S_h, const double* C_h, hipStream_t s) {
   const int Nchi = op.Nchi;
   const long NC = mesh.Nk * (long)mesh.Nw;
-  const size_t nb = (size_t)Nchi * Nchi * N;
-  if (FVr.n != nb) FVr.alloc(nb);
   const MeshDev md = mesh_dev(mesh);
   const unsigned S = (unsigned)mesh.Nkall;
-  for (int l = 0; l < Nchi; ++l)
-    for (int m = 0; m < Nchi; ++m) {
-      zd stat(1.5 * S_h[m + Nchi * l] - 0.5 * C_h[m + Nchi * l], 0);
-      cpx<R>* g = FVr.p + (size_t)(m + Nchi * l) * N;
-      p4->exec_fused(BosonLoad<R>{Vref, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md},
-                     GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
-    }
+  // the static slot 3/2 S - 1/2 C must be symmetric for the transposed pair
+  // to be the conjugate reversal
+  bool share = contract_form();
+  for (int l = 0; l < Nchi && share; ++l)
+    for (int m = 0; m < Nchi; ++m)
+      if (S_h[m + Nchi * l] != S_h[l + Nchi * m] || C_h[m + Nchi * l] != C_h[l + Nchi * m]) {
+        share = false;
+        break;
+      }
+  std::vector<std::array<int, 2>> heads;
+  if (share) {
+    heads = share_heads(fv_entry);
+  } else {
+    fv_entry.assign((size_t)Nchi * Nchi, 0);
+    for (int l = 0; l < Nchi; ++l)
+      for (int m = 0; m < Nchi; ++m) {
+        fv_entry[m + Nchi * l] = 2 * (m + Nchi * l);
+        heads.push_back({m, l});
+      }
+  }
+  fv_shared = share;
+  const size_t nb = heads.size() * (size_t)N;
+  if (FVr.n != nb) FVr.alloc(nb);
+  if (share) {
+    std::vector<long> hoffs;
+    for (auto& h : heads) head_offsets(hoffs, h[0], h[1]);
+    if (dhF_offs.n != hoffs.size()) dhF_offs.alloc(hoffs.size());
+    dhF_offs.upload(hoffs.data(), hoffs.size(), s);
+    if (dhF.n != heads.size() * (size_t)mesh.Nkall) dhF.alloc(heads.size() * (size_t)mesh.Nkall);
+    slot0_corr(dhF.p, Vref, dhF_offs.p, (int)heads.size(), s);
+    GMOD_HIP(hipStreamSynchronize(s));  // hoffs is released here
+  }
+  for (size_t h = 0; h < heads.size(); ++h) {
+    const int m = heads[h][0], l = heads[h][1];
+    zd stat(1.5 * S_h[m + Nchi * l] - 0.5 * C_h[m + Nchi * l], 0);
+    cpx<R>* g = FVr.p + h * N;
+    p4->exec_fused(BosonLoad<R>{Vref, NC * (m + (long)Nchi * l), NC * (l + (long)Nchi * m), stat, md},
+                   GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+  }
 }
 
 template <typename R>
@@ -910,32 +1021,68 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
     if (dvbuf.n != (size_t)nC) dvbuf.alloc(nC);
     diff_to_f32(dvbuf.p, V, Vref, nC, s);
   }
+  // the inverse transform of output (m1, l1) from grid g, gathered onto sig_base + sig
+  auto gather_grid = [&](cpx<R>* g, int m1, int l1) {
+    const long o = NkNw * (m1 + (long)No * l1);
+    GatherStore<R> gst{sig + o, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md};
+    gst.base = sig_base + o;
+    p4->exec_fused(GridLoad<R>{g, S}, gst, g, tmp.p, +1, invN, s);
+  };
+  auto gather_out = [&](int m1, int l1) { gather_grid(acc.p, m1, l1); };
+  std::vector<int> ytab;
+  if (contract_form()) {
+    // contraction form: the full transform of every vertex difference once
+    // per transposed pair, then all outputs in one streaming kernel
+    grouped_table(ytab);
+    std::vector<int> dent;
+    const std::vector<std::array<int, 2>> heads = share_heads(dent);
+    const size_t nb = heads.size() * (size_t)N;
+    if (DVt.n != nb) DVt.alloc(nb);
+    for (size_t h = 0; h < heads.size(); ++h) {
+      const int m = heads[h][0], l = heads[h][1];
+      const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
+      cpx<R>* g = DVt.p + h * N;
+      p4->exec_fused(BosonLoad<R, float>{dvbuf.p, oml, olm, zd(0, 0), md}, GridStore<R>{g, S}, g, tmp.p, -1, (R)1,
+                     s);
+    }
+    {
+      std::vector<long> hoffs;
+      for (auto& h : heads) head_offsets(hoffs, h[0], h[1]);
+      if (dh_offs.n < hoffs.size()) dh_offs.alloc(hoffs.size());
+      dh_offs.upload(hoffs.data(), hoffs.size(), s);
+      if (dhD.n < heads.size() * (size_t)mesh.Nkall) dhD.alloc(heads.size() * (size_t)mesh.Nkall);
+      slot0_corr(dhD.p, (const cpx<float>*)dvbuf.p, dh_offs.p, (int)heads.size(), s);
+      GMOD_HIP(hipStreamSynchronize(s));  // hoffs is released here
+    }
+    const int NAB = No * No;
+    if (fv_entry.size() != (size_t)Nchi * Nchi) throw Error("sigma_increment: no reference vertex transforms");
+    std::vector<int> ct(2 * ytab.size());
+    for (size_t i = 0; i < ytab.size(); ++i) {
+      ct[2 * i] = fv_entry[ytab[i]];
+      ct[2 * i + 1] = dent[ytab[i]];
+    }
+    if (ctab_d.n != ct.size()) ctab_d.alloc(ct.size());
+    ctab_d.upload(ct.data(), ct.size(), s);
+    if (ysum.n < (size_t)NAB * N) ysum.alloc((size_t)NAB * N);
+    {
+      // per point: two vertex grids and Fd per (output, pair), Fr, one write per output
+      const double per = 2.0 * NAB * NAB + 2.0 * NAB + NAB;
+      ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, per * (double)N * sizeof(cpx<R>), s);
+      sigma_inc_contract<R>(ysum.p, Fr.p, Fd.p, FVr.p, DVt.p, dhF.p, dhD.p, ctab_d.p, NAB, No, S, (unsigned)(N / S),
+                            mesh.rev.p, s);
+    }
+    GMOD_HIP(hipStreamSynchronize(s));  // ct is released at return
+    for (int o = 0; o < NAB; ++o) gather_grid(ysum.p + (size_t)o * N, o % No, o / No);
+    return;
+  }
+  if (fv_shared) throw Error("sigma_increment: the reference vertex transforms are stored in the contraction form");
   const bool fused = p4->accum_ok(-1) && !sigma_unfused();
   // grouped form (every output has one source per orbital pair (a, b), No <= 4):
   // the reference-vertex term Y_o = sum FVr Fd and Gn = Fr + Fd in one
   // streaming kernel, then the accumulating pass with one operand per source
-  bool grouped = fused && No <= 4 && !sigma_ungrouped();
-  std::vector<int> ytab;
-  if (grouped) {
-    const int NAB = No * No;
-    ytab.assign((size_t)NAB * NAB, -1);
-    for (int l = 0; l < Nchi && grouped; ++l)
-      for (int m = 0; m < Nchi; ++m) {
-        const int o = op.ol[m][0] + No * op.ol[l][0], ab = op.ol[m][1] + No * op.ol[l][1];
-        int& slot = ytab[(size_t)o * NAB + ab];
-        if (slot >= 0) { grouped = false; break; }
-        slot = m + Nchi * l;
-      }
-    for (int v : ytab) grouped = grouped && v >= 0;
-  }
+  bool grouped = fused && No <= 4 && !sigma_ungrouped() && grouped_table(ytab);
   // ysum: Y of the No^2 outputs, then Gn of the No^2 orbital pairs
   if (grouped && ysum.n != (size_t)2 * No * No * N) ysum.alloc((size_t)2 * No * No * N);
-  auto gather_out = [&](int m1, int l1) {
-    const long o = NkNw * (m1 + (long)No * l1);
-    GatherStore<R> gst{sig + o, temp / (double)mesh.Nkall, zd(0, 0), nullptr, 0, md};
-    gst.base = sig_base + o;
-    p4->exec_fused(GridLoad<R>{acc.p, S}, gst, acc.p, tmp.p, +1, invN, s);
-  };
   if (!fused) {
     for (int l1 = 0; l1 < No; ++l1)
       for (int m1 = 0; m1 < No; ++m1) {
@@ -955,7 +1102,8 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
     return;
   }
   // transposed outputs share the head passes of their sources (the vertex
-  // differences' static slot is zero, so the conjugate-reversal relation holds)
+  // differences' static slot is zero, so the conjugate-reversal relation
+  // holds, with the slot-0 correction)
   const SigmaPlan P = plan_sigma(p4->accum_last_is_t(-1) && p4->reversal_ok());
   accum_tab_h.clear();
   std::vector<int> smap;
@@ -976,6 +1124,12 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
   accum_tab.upload(accum_tab_h.data(), accum_tab_h.size(), s);
   srcmap_d.alloc(smap.size());
   srcmap_d.upload(smap.data(), smap.size(), s);
+  std::vector<long> hoffs;
+  for (auto& hs : P.heads)
+    for (auto& h : hs) head_offsets(hoffs, h[0], h[1]);
+  if (dh_offs.n < hoffs.size()) dh_offs.alloc(hoffs.size());
+  if (!hoffs.empty()) dh_offs.upload(hoffs.data(), hoffs.size(), s);
+  if (dhD.n < (size_t)P.nmax * mesh.Nkall) dhD.alloc((size_t)P.nmax * mesh.Nkall);
   if (grouped) {
     if (ytab_d.n != ytab.size()) ytab_d.alloc(ytab.size());
     ytab_d.upload(ytab.data(), ytab.size(), s);
@@ -986,23 +1140,28 @@ void ConvEngine<R>::sigma_increment(zd* sig, const zd* sig_base, const zd* V, co
     ProfScope ps(sizeof(R) == 4 ? PROF_FFT32 : PROF_FFT, (NAB * NAB + 4.0 * NAB) * (double)N * sizeof(cpx<R>), s);
     sigma_ref_contract<R>(ysum.p, ysum.p + (size_t)No * No * N, Fr.p, Fd.p, FVr.p, ytab_d.p, No * No, No, N, s);
   }
-  long off = 0;
+  long off = 0, hoff = 0;
   for (size_t i = 0; i < P.outs.size(); ++i) {
     for (auto& h : P.heads[i]) {
       const int m = h[0], l = h[1];
       const long oml = NC * (m + (long)Nchi * l), olm = NC * (l + (long)Nchi * m);
       p4->exec_head(BosonLoad<R, float>{dvbuf.p, oml, olm, zd(0, 0), md}, srcs.p + (long)h[2] * N, tmp.p, -1, s);
     }
+    // the heads of one output take consecutive slots
+    if (!P.heads[i].empty())
+      slot0_corr(dhD.p + (size_t)P.heads[i][0][2] * mesh.Nkall, (const cpx<float>*)dvbuf.p, dh_offs.p + 2 * hoff,
+                 (int)P.heads[i].size(), s);
+    hoff += (long)P.heads[i].size();
     const int nsrc = (int)P.src[i].size();
     const int m1 = P.outs[i] % No, l1 = P.outs[i] / No;
     if (nsrc == 0) continue;
     if (grouped) {
       const cpx<R>* Y = ysum.p + (size_t)P.outs[i] * N;
       p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccumG<R>{accum_tab.p + off, Y, acc.p, S}, -1, (R)1, s,
-                          srcmap_d.p + off);
+                          srcmap_d.p + off, dhD.p);
     } else {
       p4->exec_accum_last(srcs.p, N, nsrc, SigmaIncAccum<R>{accum_tab.p + 3 * off, acc.p, S}, -1, (R)1, s,
-                          srcmap_d.p + off);
+                          srcmap_d.p + off, dhD.p);
     }
     off += nsrc;
     gather_out(m1, l1);

@@ -244,10 +244,11 @@ class FFTPlan {
   template <class LD>
   void exec_head(LD ld, cpx<R>* out, cpx<R>* tmp, int sign, hipStream_t s) const;
   // srcmap (device, nsrc entries, optional): source c is buffer srcmap[c] >> 1,
-  // read conjugated at (-r, -t) through rev when srcmap[c] & 1
+  // read conjugated at (-r, -t) when srcmap[c] & 1, plus at t = 0 the
+  // correction dhat[(srcmap[c] >> 1) S + sp] (S spatial points per buffer)
   template <class AC>
   void exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale, hipStream_t s,
-                       const int* srcmap = nullptr) const;
+                       const int* srcmap = nullptr, const cpx<R>* dhat = nullptr) const;
   // the conjugated reversed reads of a source map are available (grid range)
   bool reversal_ok() const { return sprev_ok(dims_[0], dims_[1], dims_[2]); }
   // Boxes of the forward list's last pass (the last pass of a sign -1 or a
@@ -428,8 +429,9 @@ void FFTPlan<R>::exec_head(LD ld, cpx<R>* out, cpx<R>* tmp, int sign, hipStream_
 template <typename R>
 template <class AC>
 void FFTPlan<R>::exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac, int sign, R scale,
-                                 hipStream_t s, const int* srcmap) const {
+                                 hipStream_t s, const int* srcmap, const cpx<R>* dhat) const {
   if (srcmap && !reversal_ok()) throw Error("exec_accum_last: grid too large for the reversed reads");
+  if (srcmap && !dhat) throw Error("exec_accum_last: a source map needs the slot-0 corrections");
   const int sg = sign < 0 ? 0 : 1;
   const std::vector<BoxArgs>& bp = sg == 0 ? box_.fwd : box_.bwd;
   const int i = (int)bp.size() - 1;
@@ -448,7 +450,7 @@ void FFTPlan<R>::exec_accum_last(const cpx<R>* src, long stride, int nsrc, AC ac
   if (grid > persist_) grid = persist_;
   hipLaunchKernelGGL((box_accum_kernel<R, 16, AC>), dim3((unsigned)grid), dim3(BOX_THREADS), shm, s, src, stride,
                      nsrc, ac, (const BoxArgs*)(argsd_[sg] + i), twb_[sg], (const int*)tabs_, scale, sg == 0 ? -1 : 1,
-                     srcmap, make_sprev(dims_[0], dims_[1], dims_[2]));
+                     srcmap, make_sprev(dims_[0], dims_[1], dims_[2]), dhat);
   GMOD_CHECK_LAUNCH();
 }
 

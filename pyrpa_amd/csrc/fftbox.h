@@ -744,20 +744,24 @@ inline SpRev make_sprev(int nx, int ny, int nz) {
 }
 
 // A source of the accumulating pass: the earlier passes' output stored at p,
-// read as stored, or (cr) conjugated at the reversed point (-r, -t): the grid
-// whose transform is conj X(-r, -t)
+// read as stored, or (cr) conjugated at the reversed point (-r, -t) plus,
+// at t = 0, dh[sp] (the transformed slot-0 mismatch of the two grids): the
+// grid whose transform is conj X(-r, -t) + delta_t0 dh
 template <typename R>
 struct AccumSrcLoad {
   const cpx<R>* p;
+  const cpx<R>* dh;
   SpRev rv;
   unsigned S, M;
   int cr;
   using Key = unsigned;  // the spatial index read
   __host__ __device__ Key key(unsigned sp) const { return cr ? rv(sp) : sp; }
-  __host__ __device__ cpx<R> value(Key k, unsigned, unsigned t) const {
+  __host__ __device__ cpx<R> value(Key k, unsigned sp, unsigned t) const {
     const unsigned tt = (cr && t != 0) ? M - t : t;
     const cpx<R> v = p[k + mul24(S, tt)];
-    return cr ? cpx<R>(v.re, -v.im) : v;
+    if (!cr) return v;
+    const cpx<R> c(v.re, -v.im);
+    return t == 0 ? c + dh[sp] : c;
   }
   __host__ __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
 };
@@ -773,12 +777,13 @@ struct AccumSrcLoad {
 // v) (the contribution of transformed value v), side(c, sp, t, pre, v) (its
 // writes besides the sum) and finish(sp, t, sum).  MAXR caps the radices compiled in.
 // srcmap (optional): source c is buffer srcmap[c] >> 1 of src, conjugated at
-// (-r, -t) when srcmap[c] & 1 (rv: the spatial reversal); else buffer c.
+// (-r, -t) when srcmap[c] & 1 (rv: the spatial reversal), with the slot-0
+// correction dhat + (srcmap[c] >> 1) S; else buffer c.
 template <typename R, int MAXR, class AC>
 __global__ void __launch_bounds__(BOX_THREADS)
 box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC ac, const BoxArgs* __restrict__ ap,
                  const cpx<R>* __restrict__ tw, const int* __restrict__ tabs, R scale, int sign,
-                 const int* __restrict__ srcmap, SpRev rv) {
+                 const int* __restrict__ srcmap, SpRev rv, const cpx<R>* __restrict__ dhat) {
   extern __shared__ unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
   const BoxArgs& a = *ap;
@@ -809,7 +814,8 @@ box_accum_kernel(const cpx<R>* __restrict__ src, long src_stride, int nsrc, AC a
   cpx<R> v[TV], sum[TM];
   auto srcld = [&](int cc) {
     const int e = srcmap ? srcmap[cc] : 2 * cc;
-    return AccumSrcLoad<R>{src + (long)(e >> 1) * src_stride, rv, a.S, a.M, e & 1};
+    return AccumSrcLoad<R>{src + (long)(e >> 1) * src_stride, dhat ? dhat + (long)(e >> 1) * a.S : nullptr, rv,
+                           a.S, a.M, e & 1};
   };
   box_twiddles_to_lds<R>(lds, a, tw, tid, BOX_THREADS);  // ordered before use by the loop's first barrier
   unsigned b = blockIdx.x;

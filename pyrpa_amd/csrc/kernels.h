@@ -143,6 +143,27 @@ struct BosonLoad {
   __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(cpx<S>); }  // V_ml, V_lm slices
 };
 
+// The nu_0 mismatch of a transposed pair of bosonic grids: for the stored
+// transform of pair (m, l) read by pair (l, m) through the conjugate
+// reversal, D(x) = B_lm(x, 0) - conj B_ml(x, 0), zero when V(k, nu_0) is
+// Hermitian in the pair index (every other slot, and the nu -> infinity slot
+// when the static part is symmetric, obeys B_lm(x, t) = conj B_ml(x, -t) by
+// the construction of BosonLoad).  offs[2 h], offs[2 h + 1]: the V offsets of
+// (m, l) and (l, m) of entry h; out[h S + sp].
+template <typename R, typename Sv>
+__global__ void k_slot0_diff(cpx<R>* __restrict__ out, const cpx<Sv>* __restrict__ V, const long* __restrict__ offs,
+                             int nh, unsigned S, MeshDev m) {
+  const long tot = (long)nh * S;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+    const int h = (int)(i / S);
+    const unsigned sp = (unsigned)(i - (long)h * S);
+    const long oml = offs[2 * h], olm = offs[2 * h + 1];
+    const BosonLoad<R, Sv> bml{V, oml, olm, zd(0, 0), m}, blm{V, olm, oml, zd(0, 0), m};
+    const int kf = m.kf_s[sp];
+    out[i] = blm.value(kf, sp, 0u) - conj(bml.value(kf, sp, 0u));
+  }
+}
+
 // bosonic V_(m,l) of a spinful basis (feliashwithsoc.f90:545-573): a point
 // reached through -k reads s V(q)_(l',m') at positive nu and
 // s conj V(q)_(m',l') at negative nu, l' / m' the spin-flipped pairs and
@@ -534,6 +555,17 @@ struct SigmaIncAccumG {
 template <typename R>
 void sigma_ref_contract(cpx<R>* Y, cpx<R>* Gn, const cpx<R>* Fr, const cpx<R>* Fd, const cpx<R>* FVr,
                         const int* tab, int nout, int No, long N, hipStream_t s);
+
+// For every point (sp, t) of the full grid (N = S M) and Sigma output o
+// (nout of them): out_o = sum_ab FVr_e Fd_ab + DVt_d (Fr_ab + Fd_ab), where
+// ab = a + No b runs over the No^2 orbital pairs and (e, d) = tab[2 (o No^2
+// + ab)], tab[2 (o No^2 + ab) + 1] are stored-transform entries: grid e >> 1
+// at (sp, t), or conjugated at (rev[sp], t) plus the correction
+// dhF / dhD[(e >> 1) S + sp] (the transformed nu_0 mismatch) when e & 1.
+template <typename R>
+void sigma_inc_contract(cpx<R>* out, const cpx<R>* Fr, const cpx<R>* Fd, const cpx<R>* FVr, const cpx<R>* DVt,
+                        const cpx<R>* dhF, const cpx<R>* dhD, const int* tab, int nout, int No, unsigned S,
+                        unsigned M, const int* rev, hipStream_t s);
 
 template <typename R>
 struct MulAccAccum {

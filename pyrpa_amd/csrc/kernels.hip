@@ -531,12 +531,70 @@ void sigma_ref_contract(cpx<R>* Y, cpx<R>* Gn, const cpx<R>* Fr, const cpx<R>* F
   GMOD_CHECK_LAUNCH();
 }
 
+// One thread per spatial point and a stride of slots: the reversed point of
+// a thread is fixed, so its loads run along the grid (descending) as the
+// direct ones do.  The orbital-pair grids stay in registers across outputs.
+template <typename R, int NAB>
+__global__ void __launch_bounds__(256)
+k_sigma_inc_contract(cpx<R>* __restrict__ out, const cpx<R>* __restrict__ Fr, const cpx<R>* __restrict__ Fd,
+                     const cpx<R>* __restrict__ FVr, const cpx<R>* __restrict__ DVt,
+                     const cpx<R>* __restrict__ dhF, const cpx<R>* __restrict__ dhD, const int* __restrict__ tab,
+                     int nout, unsigned S, unsigned M, const int* __restrict__ rev) {
+  const unsigned sp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sp >= S) return;
+  const long N = (long)S * M;
+  const unsigned rs = (unsigned)rev[sp];
+  for (unsigned t = blockIdx.y; t < M; t += gridDim.y) {
+    const long e = sp + (long)S * t, er = rs + (long)S * t;
+    cpx<R> fd[NAB], gn[NAB];
+#pragma unroll
+    for (int ab = 0; ab < NAB; ++ab) {
+      fd[ab] = Fd[ab * N + e];
+      gn[ab] = Fr[ab * N + e] + fd[ab];
+    }
+    for (int o = 0; o < nout; ++o) {
+      const int* tb = tab + 2 * o * NAB;
+      cpx<R> acc(0, 0);
+#pragma unroll
+      for (int ab = 0; ab < NAB; ++ab) {
+        const int ef = tb[2 * ab], ed = tb[2 * ab + 1];
+        cpx<R> fv = FVr[(long)(ef >> 1) * N + ((ef & 1) ? er : e)];
+        cpx<R> dv = DVt[(long)(ed >> 1) * N + ((ed & 1) ? er : e)];
+        if (ef & 1) fv = conj(fv) + dhF[(long)(ef >> 1) * S + sp];
+        if (ed & 1) dv = conj(dv) + dhD[(long)(ed >> 1) * S + sp];
+        acc += fv * fd[ab] + dv * gn[ab];
+      }
+      out[(long)o * N + e] = acc;
+    }
+  }
+}
+
+template <typename R>
+void sigma_inc_contract(cpx<R>* out, const cpx<R>* Fr, const cpx<R>* Fd, const cpx<R>* FVr, const cpx<R>* DVt,
+                        const cpx<R>* dhF, const cpx<R>* dhD, const int* tab, int nout, int No, unsigned S,
+                        unsigned M, const int* rev, hipStream_t s) {
+  const unsigned gx = (S + 255) / 256;
+  const unsigned gy = (unsigned)std::max<long>(1, std::min<long>(M, (256L * 32 + gx - 1) / gx));
+  const dim3 g(gx, gy);
+  switch (No) {
+    case 1: hipLaunchKernelGGL((k_sigma_inc_contract<R, 1>), g, dim3(256), 0, s, out, Fr, Fd, FVr, DVt, dhF, dhD, tab, nout, S, M, rev); break;
+    case 2: hipLaunchKernelGGL((k_sigma_inc_contract<R, 4>), g, dim3(256), 0, s, out, Fr, Fd, FVr, DVt, dhF, dhD, tab, nout, S, M, rev); break;
+    case 3: hipLaunchKernelGGL((k_sigma_inc_contract<R, 9>), g, dim3(256), 0, s, out, Fr, Fd, FVr, DVt, dhF, dhD, tab, nout, S, M, rev); break;
+    case 4: hipLaunchKernelGGL((k_sigma_inc_contract<R, 16>), g, dim3(256), 0, s, out, Fr, Fd, FVr, DVt, dhF, dhD, tab, nout, S, M, rev); break;
+    default: throw Error("sigma_inc_contract: Norb > 4");
+  }
+  GMOD_CHECK_LAUNCH();
+}
+
 // ------------------------------------------------------------------ instantiations
 #define INST(R) \
   template void mul_acc<R>(cpx<R>*, const cpx<R>*, const cpx<R>*, bool, long, hipStream_t); \
   template void add_inplace<R>(cpx<R>*, const cpx<R>*, long, hipStream_t); \
   template void sigma_ref_contract<R>(cpx<R>*, cpx<R>*, const cpx<R>*, const cpx<R>*, const cpx<R>*, const int*, int, \
-                                      int, long, hipStream_t);
+                                      int, long, hipStream_t); \
+  template void sigma_inc_contract<R>(cpx<R>*, const cpx<R>*, const cpx<R>*, const cpx<R>*, const cpx<R>*, \
+                                      const cpx<R>*, const cpx<R>*, const int*, int, int, unsigned, unsigned, const int*, \
+                                      hipStream_t);
 INST(double)
 INST(float)
 

@@ -174,8 +174,9 @@ def test_mkself_mixed_precision_matches_fp64(F, case):
     assert abs(m32 - m64) < 1e-6
 
 
+@pytest.mark.parametrize("model", ["2orb", "3orb_random"])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, monkeypatch):
+def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, model, monkeypatch):
     """The Sigma step's accumulating last pass (all vertex pairs of one output
     transformed in one persistent pass, their sum kept in registers) against
     the per-pair passes that accumulate through memory
@@ -184,19 +185,38 @@ def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, monkeyp
     take the accumulating path).  Four iterations of FP64 FLEX and of the
     mixed loop (FP64 first iteration, then FP32 increments): the same
     arithmetic in the same order up to contraction, so 1e-12 (FP64) and 1e-6
-    (FP32 increments) of max|Sigma|."""
-    st = MD.two_orbital(Nx=16, Ny=16, Nz=4, Nw=32, temp=0.05)
-    S, C = O.get_scmat(st['olist'], st['site'], 0.8, 0.1)
-    args = (S, C, st['kmap'], st['invk'], st['olist'], st['hamk'], st['eig'], st['uni'], 0.0, 0.9, st['temp'],
+    (FP32 increments) of max|Sigma|.  The increments run in four forms: the
+    contraction form (default: one stored transform per transposed vertex
+    pair, all outputs in one streaming kernel), the grouped accumulating
+    pass (GMOD_SIGMA_ACCUM=1), the accumulating pass with three operands per
+    source (GMOD_SIGMA_UNGROUPED=1) and the per-pair passes.  The fused
+    forms transform one vertex pair of each transposed pair and read the
+    other as the conjugate reversal plus the transformed nu_0 mismatch; the
+    3-orbital model with complex hoppings (time reversal broken) has a
+    vertex that is not Hermitian at nu_0, so there the correction carries
+    weight."""
+    if model == "2orb":
+        st = MD.two_orbital(Nx=16, Ny=16, Nz=4, Nw=32, temp=0.05)
+        U, J, fill = 0.8, 0.1, 0.9
+    else:
+        st = MD.random_model(3, 16, 16, 4, 32, 0.1, 0.0, 0, 0, seed=7)
+        U, J, fill = 0.6, 0.1, 1.2
+    S, C = O.get_scmat(st['olist'], st['site'], U, J)
+    args = (S, C, st['kmap'], st['invk'], st['olist'], st['hamk'], st['eig'], st['uni'], 0.0, fill, st['temp'],
             st['Nw'], st['Nx'], st['Ny'], st['Nz'], False, False)
     opts = dict(eps=1e-14, pp=0.5, m_diis=1, scf_loop=4)
     try:
         F.set_precision(prec)
         monkeypatch.delenv("GMOD_SIGMA_UNFUSED", raising=False)
         monkeypatch.delenv("GMOD_SIGMA_UNGROUPED", raising=False)
+        monkeypatch.delenv("GMOD_SIGMA_ACCUM", raising=False)
         sf, mf = F.mkself(*args, **opts)
         nf = F.flex_stats()
         if prec == 1:
+            # the grouped accumulating pass instead of the contraction form
+            monkeypatch.setenv("GMOD_SIGMA_ACCUM", "1")
+            sa, ma = F.mkself(*args, **opts)
+            monkeypatch.delenv("GMOD_SIGMA_ACCUM")
             # the increments' accumulating pass reading all three operands per
             # source instead of the grouped reference-vertex term
             monkeypatch.setenv("GMOD_SIGMA_UNGROUPED", "1")
@@ -215,6 +235,7 @@ def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, monkeyp
     assert abs(mf - mu_) < tol
     if prec == 1:
         assert rel(sg, su) < tol and abs(mg - mu_) < tol
+        assert rel(sa, su) < tol and abs(ma - mu_) < tol
 
 
 def test_mkself_warm_start_from_memory_and_sigma_bin(F, tmp_path, monkeypatch):
