@@ -51,14 +51,14 @@ HBM_PEAK_GBS = 8000.0
 # (libs/src/fchi.f90:852-892; sw_tail as main.py:165 sets it)
 SW_TAIL = True
 # HBM bytes per launch of the FP32 convolution kernels (box_fft_kernel,
-# box_accum_kernel and the Sigma increment's reference-vertex contraction) on
+# box_accum_kernel and the Sigma increment's contraction kernel) on
 # the 4-orbital 32x32x16 x 1024-Matsubara workload (tail-corrected chi0, warm
 # start), averaged over the dispatches of the timed mixed solve: rocprofv3
 # FETCH_SIZE x 2 + WRITE_SIZE (KiB x 1024), both calibrated on a device copy of
 # known size (FETCH_SIZE reads half the bytes on this device), per kernel
 # class, weighted by the classes' dispatch counts in the full solve.  Measured
 # offline: counter runs cannot be part of the timed run.
-FP32_BOX_TRAFFIC_BYTES = {3: 1631471461}
+FP32_BOX_TRAFFIC_BYTES = {3: 1362024514}
 
 
 def log(*a):
@@ -330,7 +330,7 @@ def main():
                      "traffic": FP32_BOX_TRAFFIC_BYTES.get(args.config) if cls == "fft_pass_fp32" else None,
                      "kernel": f"box_fft_kernel<{'float' if cls == 'fft_pass_fp32' else 'double'}>, "
                                f"box_accum_kernel (the FFT passes, all instantiations) and the Sigma increment's "
-                               f"reference-vertex contraction", "avg_launch_us": round(1e3 * avg_ms, 2),
+                               f"contraction (k_sigma_inc_contract)", "avg_launch_us": round(1e3 * avg_ms, 2),
                      "launches": int(pc["launches"]), "algorithmic_bytes_per_launch": bytes_per,
                      "bytes_model": "per pass: N b per full grid read or written (fused operands included), "
                                     "irreducible (k,w) slices at their size; average over the launches"},
