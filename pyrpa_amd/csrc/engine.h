@@ -120,7 +120,8 @@ class ConvEngine {
   DBuf<cpx<R>> srcs;
   DBuf<cpx<R>*> accum_tab;
   // the grouped Sigma increment: Y of every output, then Gn = Fr + Fd of every
-  // orbital pair; ytab_d: per output and pair the reference-vertex grid
+  // orbital pair (the contraction form: the outputs themselves); ytab_d: per
+  // output and pair the reference-vertex grid
   DBuf<cpx<R>> ysum;
   DBuf<int> ytab_d;
   std::vector<cpx<R>*> accum_tab_h;
