@@ -174,7 +174,7 @@ def test_mkself_mixed_precision_matches_fp64(F, case):
     assert abs(m32 - m64) < 1e-6
 
 
-@pytest.mark.parametrize("model", ["2orb", "3orb_random"])
+@pytest.mark.parametrize("model", ["2orb", "3orb_random", "3orb_random_asym"])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, model, monkeypatch):
     """The Sigma step's accumulating last pass (all vertex pairs of one output
@@ -194,14 +194,20 @@ def test_mkself_accumulating_sigma_pass_matches_per_pair_passes(F, prec, model, 
     other as the conjugate reversal plus the transformed nu_0 mismatch; the
     3-orbital model with complex hoppings (time reversal broken) has a
     vertex that is not Hermitian at nu_0, so there the correction carries
-    weight."""
+    weight.  With one off-diagonal element of S perturbed (S not symmetric)
+    the static slot is not, so the FP64 step and the reference-vertex
+    transforms stay unshared while the vertex differences (static slot
+    zero) are shared: the contraction reads both kinds of entries."""
     if model == "2orb":
         st = MD.two_orbital(Nx=16, Ny=16, Nz=4, Nw=32, temp=0.05)
         U, J, fill = 0.8, 0.1, 0.9
-    else:
+    else:  # 3orb_random, 3orb_random_asym
         st = MD.random_model(3, 16, 16, 4, 32, 0.1, 0.0, 0, 0, seed=7)
         U, J, fill = 0.6, 0.1, 1.2
     S, C = O.get_scmat(st['olist'], st['site'], U, J)
+    if model == "3orb_random_asym":
+        S = S.copy()
+        S[0, 1] += 0.02
     args = (S, C, st['kmap'], st['invk'], st['olist'], st['hamk'], st['eig'], st['uni'], 0.0, fill, st['temp'],
             st['Nw'], st['Nx'], st['Ny'], st['Nz'], False, False)
     opts = dict(eps=1e-14, pp=0.5, m_diis=1, scf_loop=4)
