@@ -354,7 +354,14 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
     }
     const zd J = tail ? rb[l + Nchi * m] : zd(0, 0);
     zd* dst = chi + NkNw * (m + (long)Nchi * l);
-    const GatherStore<R> gst = gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, s);
+    GatherStore<R> gst = gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d.p : nullptr, 0, s);
+    // the partner copy written by the gather (a complete mesh: every slot of
+    // the slice is written)
+    const bool fuse_pr = !self_tail && pr[0] >= 0 && mesh.complete;
+    if (fuse_pr) {
+      gst.pdst = chi + NkNw * (pr[0] + (long)Nchi * pr[1]);
+      gst.psign = op.psign[m + Nchi * l];
+    }
     product_pass(pl, gst, s);
     if (self_tail) {
       conj_copy(dst, dst, NkNw, s);
@@ -367,7 +374,8 @@ void ConvEngine<R>::chi0(zd* chi, double temp, bool tail, const double* eig_d, c
       continue;
     }
     // the spinful partner carries the sign s_l1 s_l2 s_m1 s_m2 (fchi0withsoc.f90:107)
-    if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, op.psign[m + Nchi * l], s);
+    if (pr[0] >= 0 && !fuse_pr)
+      conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, op.psign[m + Nchi * l], s);
   }
   if (apply_threshold) threshold(chi, NkNw * Nchi * Nchi, 1e-9, s);
 }
@@ -998,8 +1006,12 @@ void ConvEngine<R>::chi0_increment(zd* chi, const zd* chi_base, const zd* G, con
     }
     GatherStore<R> gst = gather_sf(dst, tail ? 1.0 : w, J, tail ? sawc_d : nullptr, 0, s);
     gst.base = base;
+    // the partner copy written by the gather (a complete mesh: every slot of
+    // the slice is written)
+    const bool fuse_pr = pr[0] >= 0 && mesh.complete;
+    if (fuse_pr) gst.pdst = chi + NkNw * (pr[0] + (long)Nchi * pr[1]);
     product_pass(pl, gst, s);
-    if (pr[0] >= 0) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
+    if (pr[0] >= 0 && !fuse_pr) conj_copy(chi + NkNw * (pr[0] + (long)Nchi * pr[1]), dst, NkNw, s);
   }
 }
 

@@ -592,6 +592,10 @@ struct GatherStore {
   int acc;
   MeshDev m;
   const zd* base = nullptr;  // optional: dst = base + w v (FP64 reference + increment)
+  // optional: the partner pair's slice, written psign conj(dst value) at the
+  // same (k, w) (the reference's partner copy, fused)
+  zd* pdst = nullptr;
+  double psign = 1.0;
   // optional, for a last pass whose boxes span the t axis: per box (index
   // after box_remap), nonzero if it holds a kept spatial point; live_frac is
   // the fraction of boxes that do
@@ -619,11 +623,13 @@ struct GatherStore {
     zd x = zd(v) * w;
     if (sawc) x += q.js;
     if (base) x += q.bv;
-    dst[q.o] = acc ? q.old + x : x;
+    const zd f = acc ? q.old + x : x;
+    dst[q.o] = f;
+    if (pdst) pdst[q.o] = zd(psign * f.re, -psign * f.im);
   }
   __device__ void operator()(unsigned sp, unsigned t, cpx<R> v) const { commit(sp, t, prep(sp, t), v); }
   __host__ double algo_bytes(double) const {  // the kept (k, w) slice written, read too when accumulating / rebasing
-    return (1.0 + (acc ? 1.0 : 0.0) + (base ? 1.0 : 0.0)) * m.irr_elems() * sizeof(zd);
+    return (1.0 + (acc ? 1.0 : 0.0) + (base ? 1.0 : 0.0) + (pdst ? 1.0 : 0.0)) * m.irr_elems() * sizeof(zd);
   }
   __device__ bool skip(unsigned t) const { return (int)t >= m.Nw; }
   __device__ bool skip_box(unsigned lb) const { return live && !live[lb]; }
