@@ -126,6 +126,7 @@ class ConvEngine {
   DBuf<int> ytab_d;
   std::vector<cpx<R>*> accum_tab_h;
   DBuf<cpx<float>> dvbuf;                                  // vertex difference, k space
+  DBuf<cpx<float>> dgbuf;                                  // G - Gref (then G0 - G0ref), k space
   // boxes of p4's forward last pass (it spans the t axis) that hold a kept
   // spatial point: a spatial-first gather skips the others
   DBuf<unsigned char> tlive;

@@ -940,21 +940,35 @@ void ConvEngine<R>::chi0_increment(zd* chi, const zd* chi_base, const zd* G, con
   const MeshDev md = mesh_dev(mesh);
   const unsigned S = (unsigned)mesh.Nkall;
   const R invN = (R)(1.0 / (double)p4->transformed_points());
-  for (int b = 0; b < No; ++b)
-    for (int a = 0; a < No; ++a) {
-      cpx<R>* g = grid(Fd, a, b);
-      GreenDiffLoad<R> dl{GreenLoad<double>{G, a, b, No, md}, GreenLoad<double>{Gref, a, b, No, md}};
-      p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+  // the Green-function differences in FP64 and rounded to the grid's FP32
+  // once in k space (the values the difference load forms per grid point),
+  // then transformed from a quarter of the bytes
+  const long nG = NkNw * No * No;
+  auto diff_pass = [&](const zd* X, const zd* Xref, DBuf<cpx<R>>& F) {
+    if constexpr (sizeof(R) == 4) {
+      if (dgbuf.n != (size_t)nG) dgbuf.alloc(nG);
+      diff_to_f32(dgbuf.p, X, Xref, nG, s);
+      for (int b = 0; b < No; ++b)
+        for (int a = 0; a < No; ++a) {
+          cpx<R>* g = grid(F, a, b);
+          p4->exec_fused(GreenLoad<R, float>{dgbuf.p, a, b, No, md}, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+        }
+    } else {
+      for (int b = 0; b < No; ++b)
+        for (int a = 0; a < No; ++a) {
+          cpx<R>* g = grid(F, a, b);
+          GreenDiffLoad<R> dl{GreenLoad<double>{X, a, b, No, md}, GreenLoad<double>{Xref, a, b, No, md}};
+          p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+        }
     }
+  };
+  diff_pass(G, Gref, Fd);
   DBuf<zd> slice;
   if (tail) {
+    diff_pass(G0, G0ref, F0d);
     for (int b = 0; b < No; ++b)
-      for (int a = 0; a < No; ++a) {
-        cpx<R>* g = grid(F0d, a, b);
-        GreenDiffLoad<R> dl{GreenLoad<double>{G0, a, b, No, md}, GreenLoad<double>{G0ref, a, b, No, md}};
-        p4->exec_fused(dl, GridStore<R>{g, S}, g, tmp.p, -1, (R)1, s);
+      for (int a = 0; a < No; ++a)
         tau_grids(grid(TAd, a, b), grid(TBd, a, b), a, b, eig_d, uni_d, mu, mu_ref, true, temp, s);
-      }
   }
   for (auto& lm : op.irr) {
     auto pr = op.partner[lm[1] + Nchi * lm[0]];

@@ -55,9 +55,9 @@ MeshDev mesh_dev(const Mesh& m);
 // G_ab on the full grid (fchi.f90:563-583, fself.f90:226-236):
 //   slot t <  Nw : w_t ;  k direct -> G(k)_ab, k = -k_irr -> G(k_irr)_ba
 //   slot t >= Nw : -w_j with j = M-1-t ; direct -> conj G_ba, reversed -> conj G_ab
-template <typename R>
+template <typename R, typename S = double>
 struct GreenLoad {
-  const zd* G;
+  const cpx<S>* G;  // S = float: a difference formed once in k space
   int a, b, No;
   MeshDev m;
   // key / value split: the box FFT issues the index loads of all its
@@ -65,7 +65,7 @@ struct GreenLoad {
   // otherwise cost two memory latencies each); value() is branch-free
   using Key = int;  // packed 2 k + flag (Mesh::kf_s), decoded only in value()
   __device__ Key key(unsigned sp) const { return m.kf_s[sp]; }
-  using Raw = zd;  // raw / finish as for BosonLoad
+  using Raw = cpx<S>;  // raw / finish as for BosonLoad
   __device__ Raw raw(Key kf, unsigned, unsigned tu) const {
     const int t = (int)tu;
     const bool lo = t < m.Nw;
@@ -78,7 +78,7 @@ struct GreenLoad {
   __device__ cpx<R> finish(const Raw& v, unsigned tu) const { return cpx<R>((int)tu < m.Nw ? v : conj(v)); }
   __device__ cpx<R> value(Key kf, unsigned sp, unsigned t) const { return finish(raw(kf, sp, t), t); }
   __device__ cpx<R> operator()(unsigned sp, unsigned t) const { return value(key(sp), sp, t); }
-  __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(zd); }
+  __host__ double algo_bytes(double) const { return 2.0 * m.irr_elems() * sizeof(cpx<S>); }
 };
 
 // G_ab of a spinful basis on the full grid (fchi0withsoc.f90:57-75,
