@@ -58,7 +58,7 @@ SW_TAIL = True
 # known size (FETCH_SIZE reads half the bytes on this device), per kernel
 # class, weighted by the classes' dispatch counts in the full solve.  Measured
 # offline: counter runs cannot be part of the timed run.
-FP32_BOX_TRAFFIC_BYTES = {3: 1383386712}
+FP32_BOX_TRAFFIC_BYTES = {3: 1357645944}
 
 
 def log(*a):
